@@ -1,0 +1,157 @@
+/*
+ * zk_abi.h — C ABI of the MI355X Groth16/BN254 prover (libzkp.so).
+ *
+ * Plain C types only: pointers, sizes, status codes. Every function returns
+ * ZK_OK (0) or a nonzero status; zk_last_error() gives the message of the last
+ * failure on the calling thread. No exception crosses this boundary.
+ *
+ * Data conventions (same bytes as the reference, include/groth16.cuh:57-73):
+ *   field elements: 32 bytes = 4 x uint64 little-endian;
+ *   G1 affine: x[4], y[4]; G2 affine: x.a0, x.a1, y.a0, y.a1 (Montgomery Fp,
+ *   R = 2^256); the point at infinity is all zeros;
+ *   scalars (witness, MSM scalars, r, s): canonical Fr (< r);
+ *   NTT data: Montgomery Fr.
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef ZK_ABI_H
+#define ZK_ABI_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define ZK_OK 0
+#define ZK_E_ARG 1
+#define ZK_E_HIP 2
+#define ZK_E_IO 3
+#define ZK_E_FORMAT 4
+#define ZK_E_NOMEM 5
+#define ZK_E_STATE 6
+#define ZK_E_INTERNAL 7
+
+typedef struct zk_ctx zk_ctx;
+typedef struct zk_circuit zk_circuit;
+
+/* Same layout as Groth16Proof (include/groth16.cuh:69-73). */
+typedef struct zk_proof {
+    uint64_t pi_a[8];
+    uint64_t pi_b[16];
+    uint64_t pi_c[8];
+} zk_proof;
+
+/* Same fields as Groth16ZkeyHeader (include/groth16.cuh:79-86). */
+typedef struct zk_zkey_header {
+    uint32_t n_constraints; /* domain size as stored in the zkey */
+    uint32_t n_vars;
+    uint32_t n_public;
+    uint32_t domain_size;
+    int32_t log_domain;
+    uint64_t prime[4];
+} zk_zkey_header;
+
+const char* zk_last_error(void);
+const char* zk_version(void);
+
+/* Context: device, streams, NTT tables, workspace. Replaces the reference's
+ * global state (g_ntt_* in include/ntt_bn254.cuh:50-53); contexts are
+ * independent, calls on one context are serialised. */
+int zk_ctx_create(int device, zk_ctx** out);
+void zk_ctx_destroy(zk_ctx* ctx);
+
+/* ---- prove (include/groth16.cuh:119-123 groth16Prove) ----
+ * r, s: canonical blinding scalars; NULL means 0 (the reference's unblinded
+ * proof). */
+int zk_prove_files(zk_ctx* ctx, const char* zkey_path, const char* wtns_path, const uint64_t r[4], const uint64_t s[4],
+                   zk_proof* out);
+
+/* Resident circuit: the zkey is parsed and its CRS uploaded once
+ * (the reference re-reads it per proof, src/groth16.cu:1151,1248-1281). */
+int zk_circuit_load(zk_ctx* ctx, const char* zkey_path, zk_circuit** out);
+void zk_circuit_free(zk_circuit* c);
+/* info: [0]=n_vars [1]=n_public [2]=domain n [3]=log n [4]=nnzA [5]=nnzB [6]=device bytes */
+int zk_circuit_info(const zk_circuit* c, uint64_t info[7]);
+int zk_circuit_write_zkey(const zk_circuit* c, const char* path);
+/* witness: n_witness >= n_vars canonical values (host memory) */
+int zk_prove(zk_ctx* ctx, const zk_circuit* c, const uint64_t* witness, uint32_t n_witness, const uint64_t r[4],
+             const uint64_t s[4], zk_proof* out);
+/* same, witness already in device memory (n_vars x 32 bytes) */
+int zk_prove_device(zk_ctx* ctx, const zk_circuit* c, const void* d_witness, const uint64_t r[4], const uint64_t s[4],
+                    zk_proof* out);
+/* MSM window override for the next proofs on ctx (0 = per-size default) */
+int zk_set_windows(zk_ctx* ctx, int window_witness, int window_h);
+
+/* Batched proving: count independent proofs, possibly over different
+ * circuits, overlapped on the device (several workspace slots and streams).
+ * witnesses[i] is host memory; r, s are count x 4 words (NULL = zeros). */
+int zk_prove_batch(zk_ctx* ctx, int count, const zk_circuit* const* circuits, const uint64_t* const* witnesses,
+                   const uint64_t* r, const uint64_t* s, zk_proof* out);
+
+/* ---- serialization (groth16ProofToSnarkjsJson / groth16ProofToCalldata,
+ * include/groth16.cuh:149-167; public.json: src/main.cu:28-110) ---- */
+int zk_proof_to_json(const zk_proof* p, char* out, size_t out_size);
+int zk_proof_to_calldata(const zk_proof* p, uint8_t* out, size_t out_size);
+int zk_public_json(const char* wtns_path, uint32_t n_public, char* out, size_t out_size);
+/* groth16ReadZkeyHeader (include/groth16.cuh:177-180) */
+int zk_read_zkey_header(const char* zkey_path, zk_zkey_header* out);
+/* .wtns reader: *n_out = nVars; fills up to cap values (canonical) */
+int zk_read_wtns(const char* wtns_path, uint64_t* w, uint32_t cap, uint32_t* n_out);
+int zk_write_wtns(const char* wtns_path, const uint64_t* w, uint32_t n_vars);
+
+/* groth16SparseMvm (include/groth16.cuh:197-205; declared but never defined
+ * by the reference): host CSR-free form over 44-byte §4 entries, canonical
+ * witness in, canonical result out (domain_size x 4 words). */
+int zk_sparse_mvm(const uint8_t* coefs44, size_t n_coefs, uint8_t matrix_sel, const uint64_t* witness,
+                  uint32_t n_vars, uint64_t* out, uint32_t domain_size);
+
+/* ---- kernel-level entry points (host buffers), used by parity tests ----
+ * NTT (include/ntt_bn254.cuh:418 ntt): data n = 2^log_n Montgomery Fr, in
+ * place. kind 0 forward, 1 inverse, 2 coset forward (x_i * w_2n^i then NTT,
+ * snarkjs convention), 3 coset inverse. */
+int zk_ntt(zk_ctx* ctx, uint64_t* data, int log_n, int kind);
+/* MSM (include/msm_g1.cuh:918 msm_g1, include/msm_g2.cuh:859 msmG2):
+ * points AoS affine Montgomery (G1 8 words, G2 16 words per point), scalars
+ * canonical; window 0 = default. out affine Montgomery, *is_inf set. */
+int zk_msm_g1(zk_ctx* ctx, const uint64_t* points, const uint64_t* scalars, size_t n, int window, uint64_t out[8],
+              int* is_inf);
+int zk_msm_g2(zk_ctx* ctx, const uint64_t* points, const uint64_t* scalars, size_t n, int window, uint64_t out[16],
+              int* is_inf);
+/* SpMV over CSR with raw c*R^2 values (src/groth16.cu:144-167):
+ * out[i] = sum frMul(val, w[col]) (Montgomery), rows = n_rows. */
+int zk_spmv(zk_ctx* ctx, const uint32_t* row_ptr, const uint32_t* cols, const uint64_t* vals, uint32_t n_rows,
+            const uint64_t* witness, uint32_t n_vars, uint64_t* out);
+/* H pipeline (src/groth16.cu:1037-1090): aw, bw Montgomery, h canonical. */
+int zk_hpoly(zk_ctx* ctx, const uint64_t* aw, const uint64_t* bw, int log_n, uint64_t* h);
+
+/* ---- device-buffer variants (inputs resident in HBM; for benchmarks) ---- */
+int zk_dev_alloc(zk_ctx* ctx, size_t bytes, void** out);
+int zk_dev_free(zk_ctx* ctx, void* p);
+int zk_dev_upload(zk_ctx* ctx, void* dst, const void* src, size_t bytes);
+int zk_dev_download(zk_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* NTT of `batch` arrays d_in[i] -> d_out[i] (distinct buffers; d_in clobbered) */
+int zk_ntt_device(zk_ctx* ctx, void* const* d_in, void* const* d_out, int batch, int log_n, int kind);
+int zk_msm_g1_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t n, int window, uint64_t out[8],
+                     int* is_inf);
+int zk_msm_g2_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t n, int window, uint64_t out[16],
+                     int* is_inf);
+/* stream-ordered device time of the last kernel-level call, milliseconds
+ * ([0] total, [1] bucket phase (MSM) / last pass (NTT)) */
+int zk_last_timing(zk_ctx* ctx, float ms[2]);
+
+/* ---- synthetic workload (seeded; see the generator's recipe) ---- */
+/* builds the R1CS + trapdoor setup for (seed, m, n_pub) and makes it resident */
+int zk_synth_circuit(zk_ctx* ctx, uint64_t seed, uint32_t m, uint32_t n_pub, zk_circuit** out);
+/* satisfying witness; public inputs from the seed (pub_seed == seed) or fresh (any other value) */
+int zk_synth_witness(uint64_t seed, uint32_t m, uint32_t n_pub, uint64_t pub_seed, uint64_t* w_out);
+int zk_synth_blinding(uint64_t seed, uint64_t r[4], uint64_t s[4]);
+int zk_synth_uniform(uint64_t seed, size_t count, uint64_t* out);
+/* n_vars / domain of the synthetic circuit (m, n_pub) */
+int zk_synth_sizes(uint32_t m, uint32_t n_pub, uint32_t* n_vars, uint32_t* domain);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ZK_ABI_H */

@@ -1,0 +1,394 @@
+"""zyklop_amd: MI355X-native Groth16/BN254 prover (Python binding of libzkp.so).
+
+The compute path is the HIP library built from zyklop_amd/csrc for gfx950;
+this module only binds its C ABI (include/zk_abi.h) with ctypes. There is no
+CPU fallback: if libzkp.so is missing or no GPU is present the calls fail.
+
+Python mirror of the reference API (include/groth16.cuh):
+    groth16_prove(zkey, wtns)            -> Proof         (groth16Prove)
+    proof_to_snarkjs_json(proof)         -> str           (groth16ProofToSnarkjsJson)
+    proof_to_calldata(proof)             -> bytes         (groth16ProofToCalldata)
+    read_zkey_header(zkey)               -> dict          (groth16ReadZkeyHeader)
+    sparse_mvm(coefs, sel, w, n_vars, n) -> ndarray       (groth16SparseMvm)
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libzkp.so")
+_lib = None
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_vp = ctypes.c_void_p
+
+P = 0x30644E72E131A029B85045B68181585D97816A916871CA8D3C208C16D87CFD47
+R = 0x30644E72E131A029B85045B68181585D2833E84879B9709143E1F593F0000001
+
+
+class ZkError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[zk status {code}] {msg}")
+        self.code = code
+
+
+class Proof(ctypes.Structure):
+    _fields_ = [("pi_a", ctypes.c_uint64 * 8), ("pi_b", ctypes.c_uint64 * 16), ("pi_c", ctypes.c_uint64 * 8)]
+
+    def words(self):
+        return np.array(list(self.pi_a) + list(self.pi_b) + list(self.pi_c), dtype=np.uint64)
+
+
+class ZkeyHeader(ctypes.Structure):
+    _fields_ = [("n_constraints", ctypes.c_uint32), ("n_vars", ctypes.c_uint32), ("n_public", ctypes.c_uint32),
+                ("domain_size", ctypes.c_uint32), ("log_domain", ctypes.c_int32), ("prime", ctypes.c_uint64 * 4)]
+
+
+# name -> (restype, argtypes); every symbol include/zk_abi.h declares
+ABI = {
+    "zk_last_error": (ctypes.c_char_p, []),
+    "zk_version": (ctypes.c_char_p, []),
+    "zk_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "zk_ctx_destroy": (None, [_vp]),
+    "zk_prove_files": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p, _u64p, _u64p, ctypes.POINTER(Proof)]),
+    "zk_circuit_load": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "zk_circuit_free": (None, [_vp]),
+    "zk_circuit_info": (ctypes.c_int, [_vp, _u64p]),
+    "zk_circuit_write_zkey": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "zk_prove": (ctypes.c_int, [_vp, _vp, _u64p, ctypes.c_uint32, _u64p, _u64p, ctypes.POINTER(Proof)]),
+    "zk_prove_device": (ctypes.c_int, [_vp, _vp, _vp, _u64p, _u64p, ctypes.POINTER(Proof)]),
+    "zk_set_windows": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "zk_prove_batch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_u64p), _u64p, _u64p,
+                                      ctypes.POINTER(Proof)]),
+    "zk_proof_to_json": (ctypes.c_int, [ctypes.POINTER(Proof), ctypes.c_char_p, ctypes.c_size_t]),
+    "zk_proof_to_calldata": (ctypes.c_int, [ctypes.POINTER(Proof), _u8p, ctypes.c_size_t]),
+    "zk_public_json": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t]),
+    "zk_read_zkey_header": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ZkeyHeader)]),
+    "zk_read_wtns": (ctypes.c_int, [ctypes.c_char_p, _u64p, ctypes.c_uint32, _u32p]),
+    "zk_write_wtns": (ctypes.c_int, [ctypes.c_char_p, _u64p, ctypes.c_uint32]),
+    "zk_sparse_mvm": (ctypes.c_int, [_u8p, ctypes.c_size_t, ctypes.c_uint8, _u64p, ctypes.c_uint32, _u64p,
+                                     ctypes.c_uint32]),
+    "zk_ntt": (ctypes.c_int, [_vp, _u64p, ctypes.c_int, ctypes.c_int]),
+    "zk_msm_g1": (ctypes.c_int, [_vp, _u64p, _u64p, ctypes.c_size_t, ctypes.c_int, _u64p, ctypes.POINTER(ctypes.c_int)]),
+    "zk_msm_g2": (ctypes.c_int, [_vp, _u64p, _u64p, ctypes.c_size_t, ctypes.c_int, _u64p, ctypes.POINTER(ctypes.c_int)]),
+    "zk_spmv": (ctypes.c_int, [_vp, _u32p, _u32p, _u64p, ctypes.c_uint32, _u64p, ctypes.c_uint32, _u64p]),
+    "zk_hpoly": (ctypes.c_int, [_vp, _u64p, _u64p, ctypes.c_int, _u64p]),
+    "zk_dev_alloc": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    "zk_dev_free": (ctypes.c_int, [_vp, _vp]),
+    "zk_dev_upload": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "zk_dev_download": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "zk_ntt_device": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int]),
+    "zk_msm_g1_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_int, _u64p,
+                                        ctypes.POINTER(ctypes.c_int)]),
+    "zk_msm_g2_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_int, _u64p,
+                                        ctypes.POINTER(ctypes.c_int)]),
+    "zk_last_timing": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
+    "zk_synth_circuit": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_vp)]),
+    "zk_synth_witness": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, _u64p]),
+    "zk_synth_blinding": (ctypes.c_int, [ctypes.c_uint64, _u64p, _u64p]),
+    "zk_synth_uniform": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_size_t, _u64p]),
+    "zk_synth_sizes": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p]),
+}
+
+
+def build():
+    """Compile libzkp.so for gfx950 in-tree (hipcc cross-compiles without a GPU)."""
+    subprocess.check_call(["make", "-s", "-j8", "-C", os.path.join(_HERE, "csrc")])
+
+
+def lib():
+    """Load libzkp.so; raises if it has not been built (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ZkError(-1, f"{LIB_PATH} is missing: run zyklop_amd.build() (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in ABI.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise ZkError(rc, lib().zk_last_error().decode(errors="replace"))
+
+
+def u64p(a):
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_u64p)
+
+
+def u32p(a):
+    assert a.dtype == np.uint32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_u32p)
+
+
+def _opt4(v):
+    if v is None:
+        return None
+    a = np.ascontiguousarray(v, dtype=np.uint64).reshape(4)
+    return a
+
+
+class Context:
+    """One device context (streams, NTT tables, workspace)."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        _check(lib().zk_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().zk_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # ---- kernel-level ----
+    def ntt(self, data, log_n, kind):
+        d = np.ascontiguousarray(data, dtype=np.uint64).copy()
+        _check(lib().zk_ntt(self.h, u64p(d), log_n, kind))
+        return d
+
+    def msm_g1(self, points, scalars, window=0):
+        pts = np.ascontiguousarray(points, dtype=np.uint64)
+        sc = np.ascontiguousarray(scalars, dtype=np.uint64)
+        out = np.zeros(8, np.uint64)
+        inf = ctypes.c_int(0)
+        _check(lib().zk_msm_g1(self.h, u64p(pts), u64p(sc), sc.size // 4, window, u64p(out), ctypes.byref(inf)))
+        return out, bool(inf.value)
+
+    def msm_g2(self, points, scalars, window=0):
+        pts = np.ascontiguousarray(points, dtype=np.uint64)
+        sc = np.ascontiguousarray(scalars, dtype=np.uint64)
+        out = np.zeros(16, np.uint64)
+        inf = ctypes.c_int(0)
+        _check(lib().zk_msm_g2(self.h, u64p(pts), u64p(sc), sc.size // 4, window, u64p(out), ctypes.byref(inf)))
+        return out, bool(inf.value)
+
+    def spmv(self, row_ptr, cols, vals, witness):
+        rp = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+        cc = np.ascontiguousarray(cols, dtype=np.uint32)
+        vv = np.ascontiguousarray(vals, dtype=np.uint64)
+        w = np.ascontiguousarray(witness, dtype=np.uint64)
+        n_rows = rp.size - 1
+        out = np.zeros((n_rows, 4), np.uint64)
+        _check(lib().zk_spmv(self.h, u32p(rp), u32p(cc), u64p(vv), n_rows, u64p(w), w.size // 4, u64p(out)))
+        return out
+
+    def hpoly(self, aw, bw, log_n):
+        a = np.ascontiguousarray(aw, dtype=np.uint64)
+        b = np.ascontiguousarray(bw, dtype=np.uint64)
+        h = np.zeros_like(a)
+        _check(lib().zk_hpoly(self.h, u64p(a), u64p(b), log_n, u64p(h)))
+        return h
+
+    def last_timing(self):
+        ms = (ctypes.c_float * 2)()
+        _check(lib().zk_last_timing(self.h, ms))
+        return float(ms[0]), float(ms[1])
+
+    def set_windows(self, w_witness=0, w_h=0):
+        _check(lib().zk_set_windows(self.h, w_witness, w_h))
+
+    # ---- device buffers ----
+    def dev_alloc(self, nbytes):
+        p = _vp()
+        _check(lib().zk_dev_alloc(self.h, nbytes, ctypes.byref(p)))
+        return p
+
+    def dev_free(self, p):
+        _check(lib().zk_dev_free(self.h, p))
+
+    def upload(self, dst, arr):
+        a = np.ascontiguousarray(arr)
+        _check(lib().zk_dev_upload(self.h, dst, a.ctypes.data_as(_vp), a.nbytes))
+
+    def download(self, src, nbytes, dtype=np.uint64):
+        out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        _check(lib().zk_dev_download(self.h, out.ctypes.data_as(_vp), src, nbytes))
+        return out
+
+    def ntt_device(self, d_in, d_out, log_n, kind):
+        n = len(d_in)
+        ins = (_vp * n)(*d_in)
+        outs = (_vp * n)(*d_out)
+        _check(lib().zk_ntt_device(self.h, ins, outs, n, log_n, kind))
+
+    def msm_g1_device(self, d_points, d_scalars, n, window=0):
+        out = np.zeros(8, np.uint64)
+        inf = ctypes.c_int(0)
+        _check(lib().zk_msm_g1_device(self.h, d_points, d_scalars, n, window, u64p(out), ctypes.byref(inf)))
+        return out, bool(inf.value)
+
+    def msm_g2_device(self, d_points, d_scalars, n, window=0):
+        out = np.zeros(16, np.uint64)
+        inf = ctypes.c_int(0)
+        _check(lib().zk_msm_g2_device(self.h, d_points, d_scalars, n, window, u64p(out), ctypes.byref(inf)))
+        return out, bool(inf.value)
+
+    # ---- circuits and proofs ----
+    def load_circuit(self, zkey_path):
+        h = _vp()
+        _check(lib().zk_circuit_load(self.h, zkey_path.encode(), ctypes.byref(h)))
+        return Circuit(self, h)
+
+    def synth_circuit(self, seed, m, n_pub):
+        h = _vp()
+        _check(lib().zk_synth_circuit(self.h, seed, m, n_pub, ctypes.byref(h)))
+        return Circuit(self, h)
+
+    def prove(self, circuit, witness, r=None, s=None):
+        w = np.ascontiguousarray(witness, dtype=np.uint64)
+        pr = Proof()
+        r4, s4 = _opt4(r), _opt4(s)
+        _check(lib().zk_prove(self.h, circuit.h, u64p(w), w.size // 4, u64p(r4) if r4 is not None else None,
+                              u64p(s4) if s4 is not None else None, ctypes.byref(pr)))
+        return pr
+
+    def prove_device(self, circuit, d_witness, r=None, s=None):
+        pr = Proof()
+        r4, s4 = _opt4(r), _opt4(s)
+        _check(lib().zk_prove_device(self.h, circuit.h, d_witness, u64p(r4) if r4 is not None else None,
+                                     u64p(s4) if s4 is not None else None, ctypes.byref(pr)))
+        return pr
+
+    def prove_files(self, zkey, wtns, r=None, s=None):
+        pr = Proof()
+        r4, s4 = _opt4(r), _opt4(s)
+        _check(lib().zk_prove_files(self.h, zkey.encode(), wtns.encode(), u64p(r4) if r4 is not None else None,
+                                    u64p(s4) if s4 is not None else None, ctypes.byref(pr)))
+        return pr
+
+    def prove_batch(self, circuits, witnesses, r=None, s=None):
+        n = len(circuits)
+        ws = [np.ascontiguousarray(w, dtype=np.uint64) for w in witnesses]
+        ch = (_vp * n)(*[c.h for c in circuits])
+        wp = (_u64p * n)(*[u64p(w) for w in ws])
+        rr = np.ascontiguousarray(r, dtype=np.uint64).reshape(-1) if r is not None else None
+        ss = np.ascontiguousarray(s, dtype=np.uint64).reshape(-1) if s is not None else None
+        out = (Proof * n)()
+        _check(lib().zk_prove_batch(self.h, n, ch, wp, u64p(rr) if rr is not None else None,
+                                    u64p(ss) if ss is not None else None, out))
+        return list(out)
+
+
+class Circuit:
+    """A circuit resident in HBM (CSR of A/B + all CRS bases)."""
+
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.h = h
+        info = np.zeros(7, np.uint64)
+        _check(lib().zk_circuit_info(h, u64p(info)))
+        (self.n_vars, self.n_public, self.n, self.log_n, self.nnz_a, self.nnz_b,
+         self.device_bytes) = (int(x) for x in info)
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().zk_circuit_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.free()
+
+    def write_zkey(self, path):
+        _check(lib().zk_circuit_write_zkey(self.h, path.encode()))
+
+
+# ---- reference-API mirror (include/groth16.cuh) ----
+_default_ctx = None
+
+
+def _ctx():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def groth16_prove(zkey_path, wtns_path, r=None, s=None):
+    """groth16Prove (include/groth16.cuh:119-123); r = s = None is the reference's unblinded proof."""
+    return _ctx().prove_files(zkey_path, wtns_path, r, s)
+
+
+def proof_to_snarkjs_json(proof):
+    buf = ctypes.create_string_buffer(4096)
+    _check(lib().zk_proof_to_json(ctypes.byref(proof), buf, 4096))
+    return buf.value.decode()
+
+
+def proof_to_calldata(proof):
+    buf = (ctypes.c_uint8 * 256)()
+    _check(lib().zk_proof_to_calldata(ctypes.byref(proof), buf, 256))
+    return bytes(buf)
+
+
+def public_json(wtns_path, n_public):
+    size = 96 * (n_public + 2) + 16
+    buf = ctypes.create_string_buffer(size)
+    _check(lib().zk_public_json(wtns_path.encode(), n_public, buf, size))
+    return buf.value.decode()
+
+
+def read_zkey_header(zkey_path):
+    h = ZkeyHeader()
+    _check(lib().zk_read_zkey_header(zkey_path.encode(), ctypes.byref(h)))
+    return {"n_constraints": h.n_constraints, "n_vars": h.n_vars, "n_public": h.n_public,
+            "domain_size": h.domain_size, "log_domain": h.log_domain,
+            "prime": sum(int(w) << (64 * i) for i, w in enumerate(h.prime))}
+
+
+def read_wtns(path):
+    n = ctypes.c_uint32(0)
+    _check(lib().zk_read_wtns(path.encode(), None, 0, ctypes.byref(n)))
+    w = np.zeros((n.value, 4), np.uint64)
+    _check(lib().zk_read_wtns(path.encode(), u64p(w), n.value, ctypes.byref(n)))
+    return w
+
+
+def write_wtns(path, w):
+    a = np.ascontiguousarray(w, dtype=np.uint64)
+    _check(lib().zk_write_wtns(path.encode(), u64p(a), a.size // 4))
+
+
+def sparse_mvm(coefs44, matrix_sel, witness, n_vars, domain_size):
+    """groth16SparseMvm (include/groth16.cuh:197-205): canonical in, canonical out."""
+    c = np.frombuffer(coefs44, dtype=np.uint8)
+    w = np.ascontiguousarray(witness, dtype=np.uint64)
+    out = np.zeros((domain_size, 4), np.uint64)
+    _check(lib().zk_sparse_mvm(c.ctypes.data_as(_u8p), c.size // 44, matrix_sel, u64p(w), n_vars, u64p(out),
+                               domain_size))
+    return out
+
+
+def synth_witness(seed, m, n_pub, pub_seed=None):
+    nv = ctypes.c_uint32(0)
+    dom = ctypes.c_uint32(0)
+    _check(lib().zk_synth_sizes(m, n_pub, ctypes.byref(nv), ctypes.byref(dom)))
+    w = np.zeros((nv.value, 4), np.uint64)
+    _check(lib().zk_synth_witness(seed, m, n_pub, seed if pub_seed is None else pub_seed, u64p(w)))
+    return w
+
+
+def synth_blinding(seed):
+    r = np.zeros(4, np.uint64)
+    s = np.zeros(4, np.uint64)
+    _check(lib().zk_synth_blinding(seed, u64p(r), u64p(s)))
+    return r, s
+
+
+def synth_uniform(seed, count):
+    out = np.zeros((count, 4), np.uint64)
+    _check(lib().zk_synth_uniform(seed, count, u64p(out)))
+    return out

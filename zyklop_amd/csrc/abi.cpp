@@ -1,0 +1,558 @@
+// extern "C" entry points (include/zk_abi.h). Exceptions stop here and become
+// status codes plus a per-thread message.
+#include <string.h>
+
+#include <string>
+
+#include "../../include/zk_abi.h"
+#include "msm.hpp"
+#include "ntt.hpp"
+#include "prover.hpp"
+#include "qap.hpp"
+#include "synth.hpp"
+#include "zkey.hpp"
+
+struct zk_ctx {
+    zk::Context impl;
+    std::vector<std::unique_ptr<zk::ProveSlot>> batch_slots;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    float last_ms[2] = {0, 0};
+    explicit zk_ctx(int dev) : impl(dev) {
+        ZK_HIP(hipEventCreate(&ev0));
+        ZK_HIP(hipEventCreate(&ev1));
+        ZK_HIP(hipEventCreate(&ev2));
+    }
+    ~zk_ctx() {
+        batch_slots.clear();
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (ev2) (void)hipEventDestroy(ev2);
+    }
+};
+
+struct zk_circuit {
+    std::unique_ptr<zk::Circuit> impl;
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        g_err.clear();
+        return ZK_OK;
+    } catch (const zk::Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host out of memory";
+        return ZK_E_NOMEM;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return ZK_E_INTERNAL;
+    } catch (...) {
+        g_err = "unknown error";
+        return ZK_E_INTERNAL;
+    }
+}
+
+void need(bool ok, const char* what) {
+    if (!ok) throw zk::Error(ZK_E_ARG, what);
+}
+
+void check_scalars(const uint64_t* sc, size_t n) {
+    for (size_t i = 0; i < n; i++)
+        if (!zkh::Fr::is_canonical(sc + 4 * i))
+            throw zk::Error(ZK_E_ARG, "scalar " + std::to_string(i) + " is not canonical (>= r)");
+}
+
+zk::ProveSlot& ctx_slot(zk_ctx* ctx) {
+    if (!ctx->impl.slot) ctx->impl.slot = std::make_unique<zk::ProveSlot>();
+    return *ctx->impl.slot;
+}
+
+template <class HT>
+void put_result(const zkh::Xyzz<HT>& p, uint64_t* out, int* is_inf) {
+    const int lw = sizeof(HT) / 8;
+    zkh::Aff<HT> a = zkh::xyzz_to_aff(p);
+    if (is_inf) *is_inf = a.inf ? 1 : 0;
+    if (a.inf) { memset(out, 0, 16 * lw); return; }
+    memcpy(out, &a.x, 8 * lw);
+    memcpy(out + lw, &a.y, 8 * lw);
+}
+
+void msm_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t n, int window, bool g2, uint64_t* out,
+                int* is_inf) {
+    zk::Context& C = ctx->impl;
+    hipStream_t st = C.stream;
+    zk::MsmSort sort;
+    zk::MsmScratch scr;
+    zk::MsmPartials part;
+    ZK_HIP(hipEventRecord(ctx->ev0, st));
+    sort.run(d_scalars, n, window, st);
+    ZK_HIP(hipEventRecord(ctx->ev1, st));
+    zk::msm_launch(sort, d_points, g2, scr, part, st);
+    ZK_HIP(hipEventRecord(ctx->ev2, st));
+    ZK_HIP(hipStreamSynchronize(st));
+    ZK_HIP(hipEventElapsedTime(&ctx->last_ms[0], ctx->ev0, ctx->ev2));
+    ZK_HIP(hipEventElapsedTime(&ctx->last_ms[1], ctx->ev1, ctx->ev2));
+    if (g2) put_result(zk::msm_finish_g2(part), out, is_inf);
+    else put_result(zk::msm_finish_g1(part), out, is_inf);
+}
+
+void ntt_device(zk_ctx* ctx, void* const* in, void* const* out, int batch, int log_n, int kind) {
+    need(kind >= 0 && kind <= 3, "NTT kind must be 0..3");
+    zk::Context& C = ctx->impl;
+    zk::NttPlan& p = C.ntt_plan(log_n);
+    const void* pre = kind == 2 ? p.coset_fwd.p : nullptr;
+    const void* post = kind == 1 ? p.inv_plain.p : kind == 3 ? p.inv_uncoset.p : nullptr;
+    ZK_HIP(hipEventRecord(ctx->ev0, C.stream));
+    zk::ntt_run(p, in, out, batch, kind == 1 || kind == 3, pre, post, C.stream);
+    ZK_HIP(hipEventRecord(ctx->ev2, C.stream));
+    ZK_HIP(hipStreamSynchronize(C.stream));
+    ZK_HIP(hipEventElapsedTime(&ctx->last_ms[0], ctx->ev0, ctx->ev2));
+    ctx->last_ms[1] = ctx->last_ms[0];
+}
+}  // namespace
+
+extern "C" {
+
+const char* zk_last_error(void) { return g_err.c_str(); }
+const char* zk_version(void) { return "zyklop_amd 0.1 (gfx950)"; }
+
+int zk_ctx_create(int device, zk_ctx** out) {
+    return guard([&] {
+        need(out != nullptr, "out is NULL");
+        *out = new zk_ctx(device);
+    });
+}
+
+void zk_ctx_destroy(zk_ctx* ctx) { delete ctx; }
+
+int zk_circuit_load(zk_ctx* ctx, const char* zkey_path, zk_circuit** out) {
+    return guard([&] {
+        need(ctx && zkey_path && out, "NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        auto c = new zk_circuit;
+        try {
+            c->impl = zk::circuit_from_zkey(ctx->impl, zkey_path);
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *out = c;
+    });
+}
+
+void zk_circuit_free(zk_circuit* c) { delete c; }
+
+int zk_circuit_info(const zk_circuit* c, uint64_t info[7]) {
+    return guard([&] {
+        need(c && info, "NULL argument");
+        const zk::Circuit& k = *c->impl;
+        info[0] = k.n_vars; info[1] = k.n_public; info[2] = k.n; info[3] = (uint64_t)k.log_n;
+        info[4] = k.nnzA; info[5] = k.nnzB; info[6] = k.device_bytes();
+    });
+}
+
+int zk_circuit_write_zkey(const zk_circuit* c, const char* path) {
+    return guard([&] {
+        need(c && path, "NULL argument");
+        zk::circuit_write_zkey(*c->impl, path);
+    });
+}
+
+int zk_prove(zk_ctx* ctx, const zk_circuit* c, const uint64_t* witness, uint32_t n_witness, const uint64_t r[4],
+             const uint64_t s[4], zk_proof* out) {
+    return guard([&] {
+        need(ctx && c && witness && out, "NULL argument");
+        need(n_witness >= c->impl->n_vars, "witness shorter than nVars");
+        check_scalars(witness, c->impl->n_vars);
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        zk::ProveSlot& sl = ctx_slot(ctx);
+        sl.enqueue(ctx->impl, *c->impl, witness, false, r, s);
+        sl.finish(out);
+    });
+}
+
+int zk_prove_device(zk_ctx* ctx, const zk_circuit* c, const void* d_witness, const uint64_t r[4], const uint64_t s[4],
+                    zk_proof* out) {
+    return guard([&] {
+        need(ctx && c && d_witness && out, "NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        zk::ProveSlot& sl = ctx_slot(ctx);
+        sl.enqueue(ctx->impl, *c->impl, (const uint64_t*)d_witness, true, r, s);
+        sl.finish(out);
+    });
+}
+
+int zk_set_windows(zk_ctx* ctx, int window_witness, int window_h) {
+    return guard([&] {
+        need(ctx != nullptr, "NULL ctx");
+        need(window_witness == 0 || (window_witness >= 4 && window_witness <= 22), "window must be 0 or 4..22");
+        need(window_h == 0 || (window_h >= 4 && window_h <= 22), "window must be 0 or 4..22");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        zk::ProveSlot& sl = ctx_slot(ctx);
+        sl.window_w = window_witness;
+        sl.window_h = window_h;
+        for (auto& b : ctx->batch_slots) { b->window_w = window_witness; b->window_h = window_h; }
+    });
+}
+
+int zk_prove_files(zk_ctx* ctx, const char* zkey_path, const char* wtns_path, const uint64_t r[4], const uint64_t s[4],
+                   zk_proof* out) {
+    return guard([&] {
+        need(ctx && zkey_path && wtns_path && out, "NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        auto c = zk::circuit_from_zkey(ctx->impl, zkey_path);
+        uint32_t nw = 0;
+        std::vector<uint64_t> w = zk::read_wtns(wtns_path, &nw);
+        ZK_REQUIRE(nw >= c->n_vars, ZK_E_FORMAT,
+                   "witness has " + std::to_string(nw) + " entries, need " + std::to_string(c->n_vars));
+        check_scalars(w.data(), c->n_vars);
+        zk::ProveSlot& sl = ctx_slot(ctx);
+        sl.enqueue(ctx->impl, *c, w.data(), false, r, s);
+        sl.finish(out);
+    });
+}
+
+int zk_prove_batch(zk_ctx* ctx, int count, const zk_circuit* const* circuits, const uint64_t* const* witnesses,
+                   const uint64_t* r, const uint64_t* s, zk_proof* out) {
+    return guard([&] {
+        need(ctx && circuits && witnesses && out && count >= 0, "bad argument");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        for (int i = 0; i < count; i++) {
+            need(circuits[i] && witnesses[i], "NULL circuit or witness");
+            check_scalars(witnesses[i], circuits[i]->impl->n_vars);
+        }
+        const int nslots = 3;
+        while ((int)ctx->batch_slots.size() < nslots) {
+            ctx->batch_slots.push_back(std::make_unique<zk::ProveSlot>());
+            zk::ProveSlot& b = *ctx->batch_slots.back();
+            zk::ProveSlot& d = ctx_slot(ctx);
+            b.window_w = d.window_w;
+            b.window_h = d.window_h;
+        }
+        int next = 0;
+        int inflight = std::min(count, nslots);
+        for (; next < inflight; next++)
+            ctx->batch_slots[next % nslots]->enqueue(ctx->impl, *circuits[next]->impl, witnesses[next], false,
+                                                     r ? r + 4 * next : nullptr, s ? s + 4 * next : nullptr);
+        for (int i = 0; i < count; i++) {
+            zk::ProveSlot& sl = *ctx->batch_slots[i % nslots];
+            sl.finish(&out[i]);
+            if (next < count) {
+                sl.enqueue(ctx->impl, *circuits[next]->impl, witnesses[next], false, r ? r + 4 * next : nullptr,
+                           s ? s + 4 * next : nullptr);
+                next++;
+            }
+        }
+    });
+}
+
+int zk_proof_to_json(const zk_proof* p, char* out, size_t out_size) {
+    return guard([&] {
+        need(p && out && out_size > 0, "bad argument");
+        std::string j = zk::proof_to_json(*p);
+        need(j.size() + 1 <= out_size, "output buffer too small");
+        memcpy(out, j.c_str(), j.size() + 1);
+    });
+}
+
+int zk_proof_to_calldata(const zk_proof* p, uint8_t* out, size_t out_size) {
+    return guard([&] {
+        need(p && out, "NULL argument");
+        need(out_size >= 256, "calldata buffer must hold 256 bytes");
+        zk::proof_to_calldata(*p, out);
+    });
+}
+
+int zk_public_json(const char* wtns_path, uint32_t n_public, char* out, size_t out_size) {
+    return guard([&] {
+        need(wtns_path && out, "NULL argument");
+        uint32_t nw = 0;
+        std::vector<uint64_t> w = zk::read_wtns(wtns_path, &nw);
+        ZK_REQUIRE(nw >= n_public + 1, ZK_E_FORMAT, "witness has fewer than nPublic + 1 entries");
+        std::string j = zk::public_json(w.data(), n_public);
+        need(j.size() + 1 <= out_size, "output buffer too small");
+        memcpy(out, j.c_str(), j.size() + 1);
+    });
+}
+
+int zk_read_zkey_header(const char* zkey_path, zk_zkey_header* out) {
+    return guard([&] {
+        need(zkey_path && out, "NULL argument");
+        zk::ZkeyHeader h;
+        zk::read_zkey_header(zkey_path, h);
+        out->n_constraints = h.domain_size;
+        out->n_vars = h.n_vars;
+        out->n_public = h.n_public;
+        out->domain_size = h.domain_size;
+        out->log_domain = h.log_domain;
+        memcpy(out->prime, h.r, 32);
+    });
+}
+
+int zk_read_wtns(const char* wtns_path, uint64_t* w, uint32_t cap, uint32_t* n_out) {
+    return guard([&] {
+        need(wtns_path && n_out, "NULL argument");
+        uint32_t n = 0;
+        std::vector<uint64_t> v = zk::read_wtns(wtns_path, &n);
+        *n_out = n;
+        if (w) memcpy(w, v.data(), 32ull * std::min(n, cap));
+    });
+}
+
+int zk_write_wtns(const char* wtns_path, const uint64_t* w, uint32_t n_vars) {
+    return guard([&] {
+        need(wtns_path && w, "NULL argument");
+        zk::write_wtns(wtns_path, w, n_vars);
+    });
+}
+
+int zk_sparse_mvm(const uint8_t* coefs44, size_t n_coefs, uint8_t matrix_sel, const uint64_t* witness, uint32_t n_vars,
+                  uint64_t* out, uint32_t domain_size) {
+    return guard([&] {
+        need(coefs44 && witness && out, "NULL argument");
+        need(matrix_sel <= 2, "matrix_sel must be 0, 1 or 2");
+        std::vector<zkh::Fr> acc(domain_size, zkh::Fr::zero());
+        for (size_t i = 0; i < n_coefs; i++) {
+            const uint8_t* e = coefs44 + 44 * i;
+            uint32_t mat, row, wire;
+            memcpy(&mat, e, 4); memcpy(&row, e + 4, 4); memcpy(&wire, e + 8, 4);
+            if (mat != matrix_sel) continue;
+            ZK_REQUIRE(row < domain_size && wire < n_vars, ZK_E_FORMAT, "coefficient index out of range");
+            // frMul(c R^2, w) = (c w) R
+            acc[row] = acc[row] + zkh::Fr::from_raw((const uint64_t*)(e + 12)) * zkh::Fr::from_raw(witness + 4ull * wire);
+        }
+        for (uint32_t i = 0; i < domain_size; i++) acc[i].canonical(out + 4ull * i);
+    });
+}
+
+int zk_ntt(zk_ctx* ctx, uint64_t* data, int log_n, int kind) {
+    return guard([&] {
+        need(ctx && data, "NULL argument");
+        need(log_n >= 1 && log_n <= 27, "log_n must be 1..27");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        size_t bytes = 32ull << log_n;
+        zk::DevBuf a(bytes), b(bytes);
+        ZK_HIP(hipMemcpy(a.p, data, bytes, hipMemcpyHostToDevice));
+        void* in[1] = {a.p};
+        void* outp[1] = {b.p};
+        ntt_device(ctx, in, outp, 1, log_n, kind);
+        ZK_HIP(hipMemcpy(data, b.p, bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int zk_ntt_device(zk_ctx* ctx, void* const* d_in, void* const* d_out, int batch, int log_n, int kind) {
+    return guard([&] {
+        need(ctx && d_in && d_out, "NULL argument");
+        need(log_n >= 1 && log_n <= 27, "log_n must be 1..27");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        ntt_device(ctx, d_in, d_out, batch, log_n, kind);
+    });
+}
+
+static int msm_host(zk_ctx* ctx, const uint64_t* points, const uint64_t* scalars, size_t n, int window, bool g2,
+                    uint64_t* out, int* is_inf) {
+    return guard([&] {
+        need(ctx && (points || !n) && (scalars || !n) && out, "NULL argument");
+        check_scalars(scalars, n);
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        size_t pb = (g2 ? 128 : 64) * n;
+        zk::DevBuf dp(pb), ds(32 * n);
+        if (n) {
+            ZK_HIP(hipMemcpy(dp.p, points, pb, hipMemcpyHostToDevice));
+            ZK_HIP(hipMemcpy(ds.p, scalars, 32 * n, hipMemcpyHostToDevice));
+        }
+        msm_device(ctx, dp.p, ds.p, n, window, g2, out, is_inf);
+    });
+}
+
+int zk_msm_g1(zk_ctx* ctx, const uint64_t* points, const uint64_t* scalars, size_t n, int window, uint64_t out[8],
+              int* is_inf) {
+    return msm_host(ctx, points, scalars, n, window, false, out, is_inf);
+}
+
+int zk_msm_g2(zk_ctx* ctx, const uint64_t* points, const uint64_t* scalars, size_t n, int window, uint64_t out[16],
+              int* is_inf) {
+    return msm_host(ctx, points, scalars, n, window, true, out, is_inf);
+}
+
+int zk_msm_g1_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t n, int window, uint64_t out[8],
+                     int* is_inf) {
+    return guard([&] {
+        need(ctx && out, "NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        msm_device(ctx, d_points, d_scalars, n, window, false, out, is_inf);
+    });
+}
+
+int zk_msm_g2_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t n, int window, uint64_t out[16],
+                     int* is_inf) {
+    return guard([&] {
+        need(ctx && out, "NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        msm_device(ctx, d_points, d_scalars, n, window, true, out, is_inf);
+    });
+}
+
+int zk_spmv(zk_ctx* ctx, const uint32_t* row_ptr, const uint32_t* cols, const uint64_t* vals, uint32_t n_rows,
+            const uint64_t* witness, uint32_t n_vars, uint64_t* out) {
+    return guard([&] {
+        need(ctx && row_ptr && witness && out, "NULL argument");
+        uint32_t nnz = row_ptr[n_rows];
+        for (uint32_t i = 0; i < n_rows; i++) need(row_ptr[i] <= row_ptr[i + 1], "row_ptr must be non-decreasing");
+        for (uint32_t k = 0; k < nnz; k++) need(cols[k] < n_vars, "column index out of range");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        zk::Context& C = ctx->impl;
+        zk::DevBuf dr(4ull * (n_rows + 1)), dc(4ull * nnz + 4), dv(32ull * nnz + 32), dw(32ull * n_vars + 32);
+        zk::DevBuf a(32ull * n_rows + 32), b(32ull * n_rows + 32), c(32ull * n_rows + 32);
+        ZK_HIP(hipMemcpy(dr.p, row_ptr, 4ull * (n_rows + 1), hipMemcpyHostToDevice));
+        if (nnz) {
+            ZK_HIP(hipMemcpy(dc.p, cols, 4ull * nnz, hipMemcpyHostToDevice));
+            ZK_HIP(hipMemcpy(dv.p, vals, 32ull * nnz, hipMemcpyHostToDevice));
+        }
+        ZK_HIP(hipMemcpy(dw.p, witness, 32ull * n_vars, hipMemcpyHostToDevice));
+        zk::spmv_abc(dr.as<uint32_t>(), dc.as<uint32_t>(), dv.p, n_rows, dr.as<uint32_t>(), dc.as<uint32_t>(), dv.p, 0,
+                     dw.p, n_rows, a.p, b.p, c.p, C.stream);
+        ZK_HIP(hipStreamSynchronize(C.stream));
+        ZK_HIP(hipMemcpy(out, a.p, 32ull * n_rows, hipMemcpyDeviceToHost));
+    });
+}
+
+int zk_hpoly(zk_ctx* ctx, const uint64_t* aw, const uint64_t* bw, int log_n, uint64_t* h) {
+    return guard([&] {
+        need(ctx && aw && bw && h, "NULL argument");
+        need(log_n >= 1 && log_n <= 27, "log_n must be 1..27");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        zk::Context& C = ctx->impl;
+        zk::NttPlan& p = C.ntt_plan(log_n);
+        size_t n = (size_t)1 << log_n, bytes = 32 * n;
+        zk::DevBuf a(bytes), b(bytes), c(bytes), t0(bytes), t1(bytes), t2(bytes), dh(bytes);
+        ZK_HIP(hipMemcpy(a.p, aw, bytes, hipMemcpyHostToDevice));
+        ZK_HIP(hipMemcpy(b.p, bw, bytes, hipMemcpyHostToDevice));
+        zk::pointwise_mul(a.p, b.p, c.p, n, C.stream);
+        void* abc[3] = {a.p, b.p, c.p};
+        void* tmp[3] = {t0.p, t1.p, t2.p};
+        ZK_HIP(hipEventRecord(ctx->ev0, C.stream));
+        zk::ntt_run(p, abc, tmp, 3, true, nullptr, p.inv_coset.p, C.stream);
+        zk::ntt_run(p, tmp, abc, 3, false, nullptr, nullptr, C.stream);
+        zk::h_final(a.p, b.p, c.p, dh.p, n, C.stream);
+        ZK_HIP(hipEventRecord(ctx->ev2, C.stream));
+        ZK_HIP(hipStreamSynchronize(C.stream));
+        ZK_HIP(hipEventElapsedTime(&ctx->last_ms[0], ctx->ev0, ctx->ev2));
+        ZK_HIP(hipMemcpy(h, dh.p, bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int zk_dev_alloc(zk_ctx* ctx, size_t bytes, void** out) {
+    return guard([&] {
+        need(ctx && out, "NULL argument");
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes ? bytes : 256);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw zk::Error(ZK_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        }
+        *out = p;
+    });
+}
+
+int zk_dev_free(zk_ctx* ctx, void* p) {
+    return guard([&] {
+        need(ctx != nullptr, "NULL ctx");
+        if (p) ZK_HIP(hipFree(p));
+    });
+}
+
+int zk_dev_upload(zk_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    return guard([&] {
+        need(ctx && dst && src, "NULL argument");
+        ZK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    });
+}
+
+int zk_dev_download(zk_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    return guard([&] {
+        need(ctx && dst && src, "NULL argument");
+        ZK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int zk_last_timing(zk_ctx* ctx, float ms[2]) {
+    return guard([&] {
+        need(ctx && ms, "NULL argument");
+        ms[0] = ctx->last_ms[0];
+        ms[1] = ctx->last_ms[1];
+    });
+}
+
+int zk_synth_circuit(zk_ctx* ctx, uint64_t seed, uint32_t m, uint32_t n_pub, zk_circuit** out) {
+    return guard([&] {
+        need(ctx && out, "NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        auto c = new zk_circuit;
+        try {
+            c->impl = zk::circuit_synthetic(ctx->impl, seed, m, n_pub);
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *out = c;
+    });
+}
+
+int zk_synth_witness(uint64_t seed, uint32_t m, uint32_t n_pub, uint64_t pub_seed, uint64_t* w_out) {
+    return guard([&] {
+        need(w_out != nullptr, "NULL argument");
+        zk::SynthR1cs r1;
+        zk::synth_r1cs(seed, m, n_pub, r1);
+        std::vector<zkh::Fr> pub = zk::synth_public(pub_seed, n_pub);
+        std::vector<uint64_t> w;
+        zk::synth_witness_from_public(r1, pub, w);
+        memcpy(w_out, w.data(), w.size() * 8);
+    });
+}
+
+int zk_synth_blinding(uint64_t seed, uint64_t r[4], uint64_t s[4]) {
+    return guard([&] {
+        need(r && s, "NULL argument");
+        zk::synth_blinding(seed, r, s);
+    });
+}
+
+int zk_synth_uniform(uint64_t seed, size_t count, uint64_t* out) {
+    return guard([&] {
+        need(out != nullptr, "NULL argument");
+        zk::synth_uniform(seed, count, out);
+    });
+}
+
+int zk_synth_sizes(uint32_t m, uint32_t n_pub, uint32_t* n_vars, uint32_t* domain) {
+    return guard([&] {
+        need(n_vars && domain, "NULL argument");
+        *n_vars = 1 + n_pub + m;
+        uint64_t need_rows = (uint64_t)m + n_pub + 1, d = 1;
+        while (d < need_rows) d <<= 1;
+        *domain = (uint32_t)d;
+    });
+}
+
+}  // extern "C"

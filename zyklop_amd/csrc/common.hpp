@@ -1,0 +1,102 @@
+// Host-side plumbing shared by the library: HIP error handling (errors become
+// C++ exceptions inside the library and status codes at the C-ABI), device
+// buffers, and small launch helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/zk_abi.h"
+
+#include <stdexcept>
+#include <string>
+
+namespace zk {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// Status codes (ZK_OK, ZK_E_ARG, ...) are the C-ABI's, from zk_abi.h.
+
+#define ZK_HIP(x)                                                                                 \
+    do {                                                                                          \
+        hipError_t _e = (x);                                                                      \
+        if (_e != hipSuccess)                                                                     \
+            throw ::zk::Error(ZK_E_HIP, std::string(#x) + ": " + hipGetErrorString(_e) +    \
+                                                  " (" + __FILE__ + ":" + std::to_string(__LINE__) + ")"); \
+    } while (0)
+
+#define ZK_REQUIRE(cond, code, msg)                         \
+    do {                                                    \
+        if (!(cond)) throw ::zk::Error((code), (msg));      \
+    } while (0)
+
+// Owning device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) n = 256;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            throw Error(ZK_E_NOMEM, "hipMalloc(" + std::to_string(n) + " bytes) failed: " + hipGetErrorString(e));
+        }
+        bytes = n;
+    }
+    // grow-only reallocation
+    void ensure(size_t n) {
+        if (n > bytes) alloc(n);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// Owning pinned host allocation (hipHostMalloc).
+struct HostBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf() { release(); }
+    void ensure(size_t n) {
+        if (n <= bytes) return;
+        release();
+        ZK_HIP(hipHostMalloc(&p, n, hipHostMallocDefault));
+        bytes = n;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+inline int ilog2(uint64_t x) {
+    int r = -1;
+    while (x) { x >>= 1; r++; }
+    return r;
+}
+
+}  // namespace zk

@@ -1,0 +1,388 @@
+// Pippenger MSM kernels for G1/G2 on gfx950 (see msm.hpp for the design).
+#include <algorithm>
+#include <type_traits>
+
+#include "curve.hpp"
+#include "msm.hpp"
+#include "qap.hpp"
+#include "scan.hpp"
+
+namespace zk {
+
+constexpr uint32_t L_MAX = 96;        // a bucket above this many entries takes the chunked path
+constexpr uint32_t CHUNK = 256 * 16;  // entries per chunk of an oversized bucket
+constexpr int L1 = 8;                 // buckets per level-1 reduction thread
+constexpr int L2 = 4;                 // level-1 segments per level-2 thread
+constexpr int LOG_L2 = 2;
+template <class T> struct RedBlk;
+template <> struct RedBlk<Fp> { static constexpr int V = 256; };
+template <> struct RedBlk<Fq2> { static constexpr int V = 128; };
+
+struct ChunkRec { uint32_t bucket, start, len, pad; };
+struct LargeRec { uint32_t bucket, first, count, pad; };
+
+int msm_default_c(size_t n) {
+    int lg = ilog2(n ? n : 1);
+    int c = lg - 4;
+    if (c < 4) c = 4;
+    if (c > 18) c = 18;
+    return c;
+}
+
+MsmGeom msm_geom(size_t n, int c, bool g2) {
+    if (c <= 0) c = msm_default_c(n);
+    ZK_REQUIRE(c >= 4 && c <= 22, ZK_E_ARG, "MSM window must be in [4, 22]");
+    MsmGeom g;
+    g.c = c;
+    g.W = (255 + c - 1) / c;
+    g.K = 1u << (c - 1);
+    g.nseg = g.K / L1;
+    uint32_t per_blk = (uint32_t)(g2 ? RedBlk<Fq2>::V : RedBlk<Fp>::V) * L2;
+    g.nblk = (g.nseg + per_blk - 1) / per_blk;
+    return g;
+}
+
+// ---------------------------------------------------------------- digits
+// Signed c-bit digits of a canonical scalar, least significant window first:
+// v = window bits + carry; v > 2^(c-1) becomes v - 2^c with a carry into the
+// next window (include/msm_g1.cuh:271-329 uses the same recoding).
+struct DigitIter {
+    uint32_t s[8];
+    uint32_t carry, mask, half, c;
+    ZK_DEV void init(const uint32_t* sc, size_t i, int cc) {
+        const uint4* p = reinterpret_cast<const uint4*>(sc) + 2 * i;
+        uint4 a = p[0], b = p[1];
+        s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w; s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
+        carry = 0;
+        c = (uint32_t)cc;
+        mask = (1u << cc) - 1;
+        half = 1u << (cc - 1);
+    }
+    // returns magnitude (0 = skip) and sign
+    ZK_DEV uint32_t next(bool& neg) {
+        uint32_t v = (s[0] & mask) + carry;
+#pragma unroll
+        for (int k = 0; k < 7; k++) s[k] = __builtin_amdgcn_alignbit(s[k + 1], s[k], c);
+        s[7] >>= c;
+        if (v > half) {
+            carry = 1;
+            neg = true;
+            return (mask + 1) - v;
+        }
+        carry = 0;
+        neg = false;
+        return v;
+    }
+};
+
+__global__ void k_msm_hist(const uint32_t* __restrict__ sc, size_t n, int c, int W, uint32_t K,
+                           uint32_t* __restrict__ counts) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    DigitIter it;
+    it.init(sc, i, c);
+    for (int w = 0; w < W; w++) {
+        bool ng;
+        uint32_t mag = it.next(ng);
+        if (mag) atomicAdd(&counts[(size_t)w * K + mag - 1], 1u);
+    }
+}
+
+__global__ void k_msm_scatter(const uint32_t* __restrict__ sc, size_t n, int c, int W, uint32_t K,
+                              uint32_t* __restrict__ cursor, uint32_t* __restrict__ entries) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    DigitIter it;
+    it.init(sc, i, c);
+    for (int w = 0; w < W; w++) {
+        bool ng;
+        uint32_t mag = it.next(ng);
+        if (mag) {
+            uint32_t pos = atomicAdd(&cursor[(size_t)w * K + mag - 1], 1u);
+            entries[pos] = (uint32_t)i | (ng ? 0x80000000u : 0u);
+        }
+    }
+}
+
+void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
+    ZK_REQUIRE(n_ < 0x80000000ull, ZK_E_ARG, "MSM size must be < 2^31");
+    n = n_;
+    g = msm_geom(n, c, false);
+    size_t nb = (size_t)g.W * g.K;
+    counts.ensure((nb + 1) * 4);
+    offsets.ensure((nb + 1) * 4 + (size_t)4 * (cdiv(nb + 1, SCAN_TILE) + 1));
+    entries.ensure(std::max<size_t>(1, (size_t)g.W * n) * 4);
+    fill_u32(counts.p, 0u, nb + 1, st);
+    if (n) {
+        k_msm_hist<<<cdiv(n, 256), 256, 0, st>>>((const uint32_t*)d_scalars, n, g.c, g.W, g.K, counts.as<uint32_t>());
+        ZK_HIP(hipGetLastError());
+    }
+    uint32_t* off = offsets.as<uint32_t>();
+    exclusive_scan(counts.as<uint32_t>(), off, off + nb + 1, nb + 1, st);
+    // scatter uses counts as the running cursor (re-seeded from the offsets)
+    ZK_HIP(hipMemcpyAsync(counts.p, off, (nb + 1) * 4, hipMemcpyDeviceToDevice, st));
+    if (n) {
+        k_msm_scatter<<<cdiv(n, 256), 256, 0, st>>>((const uint32_t*)d_scalars, n, g.c, g.W, g.K, counts.as<uint32_t>(),
+                                                   entries.as<uint32_t>());
+        ZK_HIP(hipGetLastError());
+    }
+}
+
+// ---------------------------------------------------------------- buckets
+template <class T>
+__global__ void __launch_bounds__(256) k_msm_accum(const uint32_t* __restrict__ entries, const uint32_t* __restrict__ offsets,
+                                                   uint32_t nb, const void* __restrict__ points, void* __restrict__ buckets,
+                                                   uint32_t* __restrict__ counters, ChunkRec* __restrict__ chunks,
+                                                   LargeRec* __restrict__ large) {
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    uint32_t st = offsets[b], en = offsets[b + 1];
+    uint32_t cnt = en - st;
+    if (cnt > L_MAX) {
+        uint32_t nch = (cnt + CHUNK - 1) / CHUNK;
+        uint32_t first = atomicAdd(&counters[0], nch);
+        for (uint32_t j = 0; j < nch; j++) {
+            uint32_t s0 = st + j * CHUNK;
+            chunks[first + j] = {b, s0, min(CHUNK, en - s0), 0u};
+        }
+        uint32_t li = atomicAdd(&counters[1], 1u);
+        large[li] = {b, first, nch, 0u};
+        return;
+    }
+    Xyzz<T> acc = xyzz_inf<T>();
+    for (uint32_t e = st; e < en; e++) {
+        uint32_t ref = entries[e];
+        Aff<T> P = PointIO<T>::load_aff(points, ref & 0x7fffffffu);
+        if (aff_is_inf(P)) continue;
+        if (ref >> 31) P.y = PointIO<T>::neg_y(P.y);
+        xyzz_madd(acc, P);
+    }
+    PointIO<T>::store_xyzz(buckets, b, acc);
+}
+
+// tree-sum of one XYZZ per thread across the block (blockDim.x must be a power of 2)
+template <class T>
+__device__ Xyzz<T> block_sum(Xyzz<T> v, Xyzz<T>* sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    for (int h = blockDim.x >> 1; h >= 1; h >>= 1) {
+        __syncthreads();
+        if (t < h) sh[t] = xyzz_add(sh[t], sh[t + h]);
+    }
+    __syncthreads();
+    Xyzz<T> r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// one workgroup per chunk of an oversized bucket (grid-stride over the chunk list)
+template <class T>
+__global__ void __launch_bounds__(256) k_msm_chunks(const uint32_t* __restrict__ entries, const void* __restrict__ points,
+                                                    const uint32_t* __restrict__ counters,
+                                                    const ChunkRec* __restrict__ chunks, void* __restrict__ partials) {
+    __shared__ Xyzz<T> sh[256];
+    const uint32_t nch = counters[0];
+    for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
+        ChunkRec cr = chunks[ci];
+        Xyzz<T> acc = xyzz_inf<T>();
+        for (uint32_t e = threadIdx.x; e < cr.len; e += blockDim.x) {
+            uint32_t ref = entries[cr.start + e];
+            Aff<T> P = PointIO<T>::load_aff(points, ref & 0x7fffffffu);
+            if (aff_is_inf(P)) continue;
+            if (ref >> 31) P.y = PointIO<T>::neg_y(P.y);
+            xyzz_madd(acc, P);
+        }
+        Xyzz<T> s = block_sum<T>(acc, sh);
+        if (threadIdx.x == 0) PointIO<T>::store_xyzz(partials, ci, s);
+    }
+}
+
+// one workgroup per oversized bucket: sum its chunk partials
+template <class T>
+__global__ void __launch_bounds__(256) k_msm_merge(const uint32_t* __restrict__ counters, const LargeRec* __restrict__ large,
+                                                   const void* __restrict__ partials, void* __restrict__ buckets) {
+    __shared__ Xyzz<T> sh[256];
+    const uint32_t nl = counters[1];
+    for (uint32_t li = blockIdx.x; li < nl; li += gridDim.x) {
+        LargeRec lr = large[li];
+        Xyzz<T> acc = xyzz_inf<T>();
+        for (uint32_t j = threadIdx.x; j < lr.count; j += blockDim.x)
+            acc = xyzz_add(acc, PointIO<T>::load_xyzz(partials, lr.first + j));
+        Xyzz<T> s = block_sum<T>(acc, sh);
+        if (threadIdx.x == 0) PointIO<T>::store_xyzz(buckets, lr.bucket, s);
+    }
+}
+
+// ---------------------------------------------------------------- reduction
+// Level 1: thread j owns buckets [j*L1, (j+1)*L1) of one window; with bucket
+// b holding digit b+1: R_j = sum B_b, T_j = sum (b - j*L1 + 1) B_b, so the
+// window value is sum_j (T_j + j*L1*R_j).
+template <class T>
+__global__ void __launch_bounds__(256) k_msm_reduce1(const void* __restrict__ buckets, uint32_t K, int W,
+                                                     void* __restrict__ R, void* __restrict__ Tt) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t nseg = K / L1;
+    if (j >= (uint32_t)W * nseg) return;
+    uint32_t w = j / nseg, jj = j - w * nseg;
+    size_t base = (size_t)w * K + (size_t)jj * L1;
+    Xyzz<T> run = xyzz_inf<T>(), acc = xyzz_inf<T>();
+    for (int b = L1 - 1; b >= 0; b--) {
+        run = xyzz_add(run, PointIO<T>::load_xyzz(buckets, base + b));
+        acc = xyzz_add(acc, run);
+    }
+    PointIO<T>::store_xyzz(R, j, run);
+    PointIO<T>::store_xyzz(Tt, j, acc);
+}
+
+// Level 2: input per window is M pairs (R_j, T_j) worth sum_j (T_j + j*U*R_j),
+// U = 2^logU. Block blk covers j in [blk*BLK*L2, (blk+1)*BLK*L2); thread t owns
+// L2 consecutive j. Output per block: R'' = sum R_j and
+// T'' = sum (T_j + (j - j_blk) * U * R_j), so the window value is
+// sum_blk (T''_blk + blk * (U*L2*BLK) * R''_blk).
+template <class T, int BLK>
+__global__ void __launch_bounds__(BLK) k_msm_reduce2(const void* __restrict__ Rin, const void* __restrict__ Tin, uint32_t M,
+                                                     int logU, uint32_t nblk, void* __restrict__ Rout,
+                                                     void* __restrict__ Tout) {
+    __shared__ Xyzz<T> shA[BLK];
+    __shared__ Xyzz<T> shB[BLK];
+    const int t = threadIdx.x;
+    const uint32_t w = blockIdx.x / nblk, blk = blockIdx.x - w * nblk;
+    const uint32_t j0 = (blk * BLK + t) * L2;
+    Xyzz<T> run = xyzz_inf<T>(), acc = xyzz_inf<T>(), tsum = xyzz_inf<T>();
+    for (int q = L2 - 1; q >= 0; q--) {
+        uint32_t j = j0 + q;
+        if (j >= M) continue;
+        acc = xyzz_add(acc, run);
+        run = xyzz_add(run, PointIO<T>::load_xyzz(Rin, (size_t)w * M + j));
+        tsum = xyzz_add(tsum, PointIO<T>::load_xyzz(Tin, (size_t)w * M + j));
+    }
+    // this thread's share: tsum + U * acc  (acc = sum (j - j0) R_j)
+    for (int d = 0; d < logU; d++) acc = xyzz_dbl(acc);
+    Xyzz<T> tp = xyzz_add(tsum, acc);
+    // suffix scan of run across the block: S_t = sum_{t' >= t} run_t'
+    shA[t] = run;
+    for (int d = 1; d < BLK; d <<= 1) {
+        __syncthreads();
+        Xyzz<T> mine = shA[t];
+        Xyzz<T> other = (t + d < BLK) ? shA[t + d] : xyzz_inf<T>();
+        __syncthreads();
+        shA[t] = xyzz_add(mine, other);
+    }
+    __syncthreads();
+    Xyzz<T> total_r = shA[0];
+    // sum_{t>=1} S_t = sum_t t * run_t ; reduce it together with tp
+    Xyzz<T> a = (t >= 1) ? shA[t] : xyzz_inf<T>();
+    __syncthreads();
+    shA[t] = a;
+    shB[t] = tp;
+    for (int h = BLK >> 1; h >= 1; h >>= 1) {
+        __syncthreads();
+        if (t < h) {
+            shA[t] = xyzz_add(shA[t], shA[t + h]);
+            shB[t] = xyzz_add(shB[t], shB[t + h]);
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        // T'' = sum tp + (U * L2) * sum_t t * run_t
+        Xyzz<T> wsum = shA[0];
+        int sh = logU + LOG_L2;
+        for (int d = 0; d < sh; d++) wsum = xyzz_dbl(wsum);
+        Xyzz<T> tt = xyzz_add(shB[0], wsum);
+        PointIO<T>::store_xyzz(Rout, blockIdx.x, total_r);
+        PointIO<T>::store_xyzz(Tout, blockIdx.x, tt);
+    }
+}
+
+void MsmScratch::ensure(const MsmGeom& g, size_t n_entries, bool g2) {
+    size_t xb = g2 ? 256 : 128;
+    size_t nb = (size_t)g.W * g.K;
+    buckets.ensure(nb * xb);
+    r1.ensure((size_t)g.W * g.nseg * xb);
+    t1.ensure((size_t)g.W * g.nseg * xb);
+    r2.ensure((size_t)g.W * g.nblk * xb);
+    t2.ensure((size_t)g.W * g.nblk * xb);
+    size_t max_chunks = n_entries / CHUNK + n_entries / L_MAX + 2;
+    size_t max_large = n_entries / L_MAX + 2;
+    chunks.ensure(max_chunks * sizeof(ChunkRec));
+    large.ensure(max_large * sizeof(LargeRec));
+    partials.ensure(max_chunks * xb);
+    counters.ensure(64);
+}
+
+template <class T>
+static void launch_t(const MsmSort& sort, const void* d_points, MsmScratch& scr, MsmPartials& out, hipStream_t st) {
+    const bool g2 = std::is_same<T, Fq2>::value;
+    const MsmGeom g = msm_geom(sort.n, sort.g.c, g2);
+    scr.ensure(g, (size_t)g.W * sort.n, g2);
+    const uint32_t nb = (uint32_t)((size_t)g.W * g.K);
+    fill_u32(scr.counters.p, 0u, 16, st);
+    k_msm_accum<T><<<cdiv(nb, 128), 128, 0, st>>>(sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(), nb, d_points,
+                                                  scr.buckets.p, scr.counters.as<uint32_t>(), scr.chunks.as<ChunkRec>(),
+                                                  scr.large.as<LargeRec>());
+    ZK_HIP(hipGetLastError());
+    k_msm_chunks<T><<<512, 256, 0, st>>>(sort.entries.as<uint32_t>(), d_points, scr.counters.as<uint32_t>(),
+                                          scr.chunks.as<ChunkRec>(), scr.partials.p);
+    ZK_HIP(hipGetLastError());
+    k_msm_merge<T><<<256, 256, 0, st>>>(scr.counters.as<uint32_t>(), scr.large.as<LargeRec>(), scr.partials.p,
+                                         scr.buckets.p);
+    ZK_HIP(hipGetLastError());
+    uint32_t nseg_all = (uint32_t)g.W * g.nseg;
+    k_msm_reduce1<T><<<cdiv(nseg_all, 128), 128, 0, st>>>(scr.buckets.p, g.K, g.W, scr.r1.p, scr.t1.p);
+    ZK_HIP(hipGetLastError());
+    constexpr int BLK = RedBlk<T>::V;
+    k_msm_reduce2<T, BLK><<<(unsigned)(g.W * g.nblk), BLK, 0, st>>>(scr.r1.p, scr.t1.p, g.nseg, ilog2(L1), g.nblk,
+                                                                     scr.r2.p, scr.t2.p);
+    ZK_HIP(hipGetLastError());
+    size_t xb = g2 ? 256 : 128;
+    size_t half = (size_t)g.W * g.nblk * xb;
+    out.g = g;
+    out.g2 = g2;
+    out.host.ensure(2 * half);
+    ZK_HIP(hipMemcpyAsync(out.host.p, scr.r2.p, half, hipMemcpyDeviceToHost, st));
+    ZK_HIP(hipMemcpyAsync((char*)out.host.p + half, scr.t2.p, half, hipMemcpyDeviceToHost, st));
+}
+
+void msm_launch(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, MsmPartials& out, hipStream_t st) {
+    if (g2) launch_t<Fq2>(sort, d_points, scr, out, st);
+    else launch_t<Fp>(sort, d_points, scr, out, st);
+}
+
+// ---------------------------------------------------------------- host finish
+template <class HT>
+static HT load_h(const uint64_t* p);
+template <>
+zkh::Fp load_h<zkh::Fp>(const uint64_t* p) { return zkh::Fp::from_raw(p); }
+template <>
+zkh::Fp2 load_h<zkh::Fp2>(const uint64_t* p) { return {zkh::Fp::from_raw(p), zkh::Fp::from_raw(p + 4)}; }
+
+template <class HT>
+static zkh::Xyzz<HT> finish_t(const MsmPartials& p) {
+    const MsmGeom& g = p.g;
+    const int lw = sizeof(HT) / 8;  // u64 words per coordinate
+    const uint64_t* R = p.host.as<uint64_t>();
+    const uint64_t* Tt = R + (size_t)g.W * g.nblk * 4 * lw;
+    int logU = ilog2(L1) + ilog2(L2) + ilog2(p.g2 ? RedBlk<Fq2>::V : RedBlk<Fp>::V);
+    auto ld = [&](const uint64_t* base, size_t i) {
+        const uint64_t* q = base + i * 4 * lw;
+        return zkh::Xyzz<HT>{load_h<HT>(q), load_h<HT>(q + lw), load_h<HT>(q + 2 * lw), load_h<HT>(q + 3 * lw)};
+    };
+    zkh::Xyzz<HT> total = zkh::Xyzz<HT>::infinity();
+    for (int w = g.W - 1; w >= 0; w--) {
+        for (int d = 0; d < g.c; d++) total = zkh::xyzz_dbl(total);
+        zkh::Xyzz<HT> run = zkh::Xyzz<HT>::infinity(), acc = run, tsum = run;
+        for (int b = (int)g.nblk - 1; b >= 0; b--) {
+            acc = zkh::xyzz_add(acc, run);
+            run = zkh::xyzz_add(run, ld(R, (size_t)w * g.nblk + b));
+            tsum = zkh::xyzz_add(tsum, ld(Tt, (size_t)w * g.nblk + b));
+        }
+        for (int d = 0; d < logU; d++) acc = zkh::xyzz_dbl(acc);
+        total = zkh::xyzz_add(total, zkh::xyzz_add(tsum, acc));
+    }
+    return total;
+}
+
+zkh::Xyzz<zkh::Fp> msm_finish_g1(const MsmPartials& p) { return finish_t<zkh::Fp>(p); }
+zkh::Xyzz<zkh::Fp2> msm_finish_g2(const MsmPartials& p) { return finish_t<zkh::Fp2>(p); }
+
+}  // namespace zk

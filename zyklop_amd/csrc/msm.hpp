@@ -1,0 +1,69 @@
+// Pippenger multi-scalar multiplication on gfx950, G1 and G2.
+//
+// Reference: msm_g1 (include/msm_g1.cuh:714-913), msm_g2_device
+// (include/msm_g2.cuh:669-854). Same mathematics (signed c-bit digits,
+// buckets, running-sum reduction), different machine design:
+//   * digit sort: device atomics histogram + own LDS/wave scan + atomic
+//     scatter, no CUB and no host synchronisation;
+//   * one sort can serve several MSMs that share a scalar vector;
+//   * buckets in XYZZ coordinates, one thread per bucket; oversized buckets
+//     are split into fixed chunks reduced by whole workgroups (device-side
+//     work lists, no host round trip);
+//   * bucket reduction: thread-level running sums over segments, then an LDS
+//     suffix-scan combine per workgroup; the host finishes a few points per
+//     window and runs Horner across windows.
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+#include "common.hpp"
+#include "host_field.hpp"
+
+namespace zk {
+
+struct MsmGeom {
+    int c;          // window bits
+    int W;          // windows = ceil(255 / c)
+    uint32_t K;     // buckets per window = 2^(c-1)
+    uint32_t nseg;  // level-1 segments per window = K / L1
+    uint32_t nblk;  // level-2 blocks per window
+};
+
+int msm_default_c(size_t n);
+MsmGeom msm_geom(size_t n, int c, bool g2);
+
+// Per-scalar-vector digit sort (shared by the MSMs over the same scalars).
+struct MsmSort {
+    MsmGeom g{};
+    size_t n = 0;
+    DevBuf counts, offsets, entries;
+    // sorts canonical scalars (n x 32 B, device) into bucket order
+    void run(const void* d_scalars, size_t n, int c, hipStream_t st);
+};
+
+// Result of the device part of one MSM: level-2 partials per window, on the host.
+struct MsmPartials {
+    MsmGeom g{};
+    bool g2 = false;
+    HostBuf host;       // [W][nblk] R then [W][nblk] T, XYZZ each
+};
+
+// Device scratch for the bucket phase of one MSM.
+struct MsmScratch {
+    DevBuf buckets, r1, t1, r2, t2, chunks, large, partials, counters;
+    size_t cap_buckets = 0;
+    void ensure(const MsmGeom& g, size_t n_entries, bool g2);
+};
+
+// Launch bucket accumulation + GPU reduction for points (affine, Montgomery,
+// AoS: G1 64 B / G2 128 B per point) against a finished sort; queue the D2H of
+// the partials into out.host on st. The caller synchronises st before finish.
+void msm_launch(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, MsmPartials& out,
+                hipStream_t st);
+
+// Host finish: combine the partials into the MSM value.
+zkh::Xyzz<zkh::Fp> msm_finish_g1(const MsmPartials& p);
+zkh::Xyzz<zkh::Fp2> msm_finish_g2(const MsmPartials& p);
+
+}  // namespace zk

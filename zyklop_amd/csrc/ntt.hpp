@@ -1,0 +1,43 @@
+// Fr NTT on gfx950: natural order in and out, w_n = 5^((r-1)/n) (the
+// reference's convention, include/ntt_bn254.cuh:190-261,300-464).
+//
+// Design: four-step recursion n = R_0 * R_1 * ... * R_{p-1} with every R_j <=
+// 256. Each pass is one kernel over LDS tiles of 2048 elements (T columns of
+// R_j points) doing a radix-2 DIT DFT in LDS, then the inter-pass twiddle
+// w_{M_j}^{c k} from a precomputed [k][c] table (coalesced). Passes 0..p-2
+// run in place; the last pass writes the digit-reversed result out of place
+// with consecutive lanes on consecutive output indices. An optional
+// element-wise table multiplies the input (first pass) or the output (last
+// pass): that is how the coset shift and n^-1 are fused instead of running
+// separate element-wise passes (src/groth16.cu:914-921,1066-1076).
+#pragma once
+#include <stdint.h>
+
+#include "common.hpp"
+
+namespace zk {
+
+struct NttPlan {
+    int log_n = -1;
+    int np = 0;
+    int lr[4] = {0, 0, 0, 0};
+    size_t n = 0;
+    // per-pass twiddle tables (forward / inverse), concatenated; offsets per pass
+    DevBuf tw_fwd, tw_inv, small_fwd, small_inv;
+    size_t tw_off[4] = {0, 0, 0, 0};
+    size_t small_off[4] = {0, 0, 0, 0};
+    // element-wise tables (Montgomery Fr, n entries each)
+    DevBuf coset_fwd;   // w_2n^i           (coset forward prologue)
+    DevBuf inv_coset;   // n^-1 * w_2n^i    (iNTT then coset shift: H pipeline)
+    DevBuf inv_plain;   // n^-1             (plain inverse)
+    DevBuf inv_uncoset; // n^-1 * w_2n^-i   (inverse of the coset NTT)
+    void build(int log_n, hipStream_t st);
+};
+
+// in -> out (in is clobbered unless in == out and np == 1). Each of the
+// `batch` arrays has n Montgomery Fr elements. pre/post: optional n-entry
+// element-wise multipliers (device), indexed by natural element index.
+void ntt_run(const NttPlan& p, void* const* in, void* const* out, int batch, bool inverse, const void* pre,
+             const void* post, hipStream_t st);
+
+}  // namespace zk

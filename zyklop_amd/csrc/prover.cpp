@@ -1,0 +1,367 @@
+#include "prover.hpp"
+
+#include <string.h>
+
+#include <cstdio>
+
+#include "qap.hpp"
+#include "synth.hpp"
+
+namespace zk {
+
+using zkh::Fp;
+using zkh::Fp2;
+using zkh::Fr;
+
+// ------------------------------------------------------------------ context
+Context::Context(int dev) : device(dev) {
+    int count = 0;
+    ZK_HIP(hipGetDeviceCount(&count));
+    ZK_REQUIRE(dev >= 0 && dev < count, ZK_E_ARG, "no such HIP device: " + std::to_string(dev));
+    ZK_HIP(hipSetDevice(dev));
+    ZK_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+}
+
+Context::~Context() {
+    (void)hipSetDevice(device);
+    slot.reset();
+    ntt_plans.clear();
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+NttPlan& Context::ntt_plan(int log_n) {
+    auto it = ntt_plans.find(log_n);
+    if (it != ntt_plans.end()) return *it->second;
+    auto p = std::make_unique<NttPlan>();
+    p->build(log_n, stream);
+    NttPlan& ref = *p;
+    ntt_plans[log_n] = std::move(p);
+    return ref;
+}
+
+// fixed-base tables: T[j][d] = d * 2^(8j) * G, affine Montgomery
+template <class T>
+static void comb_table(const zkh::Aff<T>& g, std::vector<uint64_t>& out) {
+    const int lw = sizeof(T) / 8;
+    out.assign((size_t)32 * 256 * 2 * lw, 0);
+    zkh::Xyzz<T> base = zkh::Xyzz<T>::from_aff(g);
+    for (int j = 0; j < 32; j++) {
+        zkh::Xyzz<T> acc = zkh::Xyzz<T>::infinity();
+        for (int d = 0; d < 256; d++) {
+            if (d) {
+                zkh::Aff<T> a = zkh::xyzz_to_aff(acc);
+                uint64_t* dst = out.data() + ((size_t)j * 256 + d) * 2 * lw;
+                memcpy(dst, &a.x, 8 * lw);
+                memcpy(dst + lw, &a.y, 8 * lw);
+            }
+            acc = zkh::xyzz_add(acc, base);
+        }
+        for (int k = 0; k < 8; k++) base = zkh::xyzz_dbl(base);
+    }
+}
+
+static zkh::Aff<Fp> g1_generator() {
+    return {Fp::from_u64(1), Fp::from_u64(2), false};
+}
+static zkh::Aff<Fp2> g2_generator() {
+    // canonical coordinates of the BN254 G2 generator (EIP-197)
+    static const uint64_t x0[4] = {0x46debd5cd992f6edULL, 0x674322d4f75edaddULL, 0x426a00665e5c4479ULL, 0x1800deef121f1e76ULL};
+    static const uint64_t x1[4] = {0x97e485b7aef312c2ULL, 0xf1aa493335a9e712ULL, 0x7260bfb731fb5d25ULL, 0x198e9393920d483aULL};
+    static const uint64_t y0[4] = {0x4ce6cc0166fa7daaULL, 0xe3d1e7690c43d37bULL, 0x4aab71808dcb408fULL, 0x12c85ea5db8c6debULL};
+    static const uint64_t y1[4] = {0x55acdadcd122975bULL, 0xbc4b313370b38ef3ULL, 0xec9e99ad690c3395ULL, 0x090689d0585ff075ULL};
+    auto m = [](const uint64_t* v) { return Fp::to_mont(Fp::from_raw(v)); };
+    return {{m(x0), m(x1)}, {m(y0), m(y1)}, false};
+}
+
+void Context::ensure_comb() {
+    if (comb_g1.p && comb_g2.p) return;
+    std::vector<uint64_t> t1, t2;
+    comb_table(g1_generator(), t1);
+    comb_table(g2_generator(), t2);
+    comb_g1.alloc(t1.size() * 8);
+    comb_g2.alloc(t2.size() * 8);
+    ZK_HIP(hipMemcpy(comb_g1.p, t1.data(), t1.size() * 8, hipMemcpyHostToDevice));
+    ZK_HIP(hipMemcpy(comb_g2.p, t2.data(), t2.size() * 8, hipMemcpyHostToDevice));
+}
+
+// ------------------------------------------------------------------ circuits
+size_t Circuit::device_bytes() const {
+    return rowA.bytes + colA.bytes + valA.bytes + rowB.bytes + colB.bytes + valB.bytes + A.bytes + B1.bytes + B2.bytes +
+           C.bytes + H.bytes;
+}
+
+static void upload(DevBuf& d, const void* src, size_t bytes) {
+    d.alloc(bytes);
+    if (bytes) ZK_HIP(hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice));
+}
+
+static void upload_csr(Circuit& c, const QapCsr& csr) {
+    upload(c.rowA, csr.rowA.data(), csr.rowA.size() * 4);
+    upload(c.colA, csr.colA.data(), csr.colA.size() * 4);
+    upload(c.valA, csr.valA.data(), csr.valA.size() * 8);
+    upload(c.rowB, csr.rowB.data(), csr.rowB.size() * 4);
+    upload(c.colB, csr.colB.data(), csr.colB.size() * 4);
+    upload(c.valB, csr.valB.data(), csr.valB.size() * 8);
+    c.n_rowsA = (uint32_t)csr.rowA.size() - 1;
+    c.n_rowsB = (uint32_t)csr.rowB.size() - 1;
+    c.nnzA = csr.colA.size();
+    c.nnzB = csr.colB.size();
+}
+
+// extended base arrays: [bases..., e0, e1, e2, e3] with per-slot extra points
+static void build_ext(DevBuf& d, const uint8_t* lead_zeros_then, size_t lead_zero_pts, const uint8_t* pts, size_t npts,
+                      const uint64_t* const extras[4], size_t pt_bytes) {
+    (void)lead_zeros_then;
+    size_t total = lead_zero_pts + npts + 4;
+    std::vector<uint8_t> h(total * pt_bytes, 0);
+    if (npts) memcpy(h.data() + lead_zero_pts * pt_bytes, pts, npts * pt_bytes);
+    for (int k = 0; k < 4; k++)
+        if (extras[k]) memcpy(h.data() + (lead_zero_pts + npts + k) * pt_bytes, extras[k], pt_bytes);
+    upload(d, h.data(), h.size());
+}
+
+static void finish_circuit_bases(Circuit& c, const uint8_t* A, const uint8_t* B1, const uint8_t* B2, const uint8_t* C,
+                                 const uint8_t* H) {
+    const ZkeyHeader& h = c.hdr;
+    const size_t nv = c.n_vars, l = c.n_public;
+    const uint64_t* exA[4] = {h.alpha1, h.delta1, nullptr, nullptr};
+    const uint64_t* exB1[4] = {h.beta1, nullptr, h.delta1, nullptr};
+    const uint64_t* exB2[4] = {h.beta2, nullptr, h.delta2, nullptr};
+    const uint64_t* exC[4] = {nullptr, nullptr, nullptr, h.delta1};
+    build_ext(c.A, nullptr, 0, A, nv, exA, 64);
+    build_ext(c.B1, nullptr, 0, B1, nv, exB1, 64);
+    build_ext(c.B2, nullptr, 0, B2, nv, exB2, 128);
+    build_ext(c.C, nullptr, l + 1, C, nv - l - 1, exC, 64);
+    upload(c.H, H, (size_t)c.n * 64);
+}
+
+std::unique_ptr<Circuit> circuit_from_zkey(Context& ctx, const std::string& path) {
+    ZkeyView zv(path);
+    auto c = std::make_unique<Circuit>();
+    c->ctx = &ctx;
+    c->hdr = zv.hdr;
+    c->n_vars = zv.hdr.n_vars;
+    c->n_public = zv.hdr.n_public;
+    c->n = zv.hdr.domain_size;
+    c->log_n = zv.hdr.log_domain;
+    ZK_REQUIRE(c->log_n >= 1 && c->log_n <= 27, ZK_E_FORMAT, "domain size must be 2^1..2^27");
+    QapCsr csr;
+    build_csr(zv.coefs, zv.n_coefs, c->n, c->n_vars, csr);
+    upload_csr(*c, csr);
+    c->coefs.assign(zv.coefs, zv.coefs + 44ull * zv.n_coefs);
+    if (zv.IC) c->ic.assign((const uint64_t*)zv.IC, (const uint64_t*)zv.IC + 8ull * (c->n_public + 1));
+    else c->ic.assign(8ull * (c->n_public + 1), 0);
+    finish_circuit_bases(*c, zv.A, zv.B1, zv.B2, zv.C, zv.H);
+    return c;
+}
+
+std::unique_ptr<Circuit> circuit_synthetic(Context& ctx, uint64_t seed, uint32_t m, uint32_t n_pub) {
+    ZK_REQUIRE(m >= 1, ZK_E_ARG, "synthetic circuit needs m >= 1");
+    SynthR1cs r1;
+    synth_r1cs(seed, m, n_pub, r1);
+    ZK_REQUIRE(r1.log_n <= 27, ZK_E_ARG, "synthetic circuit too large");
+    SynthSetupExps ex;
+    synth_setup(r1, ex);
+    auto c = std::make_unique<Circuit>();
+    c->ctx = &ctx;
+    c->n_vars = r1.n_vars;
+    c->n_public = n_pub;
+    c->n = r1.n;
+    c->log_n = r1.log_n;
+    synth_coef_bytes(r1, c->coefs);
+    QapCsr csr;
+    build_csr(c->coefs.data(), (uint32_t)(c->coefs.size() / 44), c->n, c->n_vars, csr);
+    upload_csr(*c, csr);
+    // CRS points on the device: e * G for every exponent
+    ctx.ensure_comb();
+    size_t n1 = ex.g1.size() / 4, n2 = ex.g2.size() / 4;
+    DevBuf de1(n1 * 32), de2(n2 * 32), dp1(n1 * 64), dp2(n2 * 128);
+    ZK_HIP(hipMemcpy(de1.p, ex.g1.data(), n1 * 32, hipMemcpyHostToDevice));
+    ZK_HIP(hipMemcpy(de2.p, ex.g2.data(), n2 * 32, hipMemcpyHostToDevice));
+    fixed_base_mul(ctx.comb_g1.p, de1.p, n1, dp1.p, false, ctx.stream);
+    fixed_base_mul(ctx.comb_g2.p, de2.p, n2, dp2.p, true, ctx.stream);
+    ZK_HIP(hipStreamSynchronize(ctx.stream));
+    std::vector<uint8_t> h1(n1 * 64), h2(n2 * 128);
+    ZK_HIP(hipMemcpy(h1.data(), dp1.p, h1.size(), hipMemcpyDeviceToHost));
+    ZK_HIP(hipMemcpy(h2.data(), dp2.p, h2.size(), hipMemcpyDeviceToHost));
+    ZkeyHeader& h = c->hdr;
+    memcpy(h.q, zkh::FpP::M, 32);
+    memcpy(h.r, zkh::FrP::M, 32);
+    h.n_vars = c->n_vars;
+    h.n_public = n_pub;
+    h.domain_size = c->n;
+    h.log_domain = c->log_n;
+    memcpy(h.alpha1, h1.data() + 0 * 64, 64);
+    memcpy(h.beta1, h1.data() + 1 * 64, 64);
+    memcpy(h.delta1, h1.data() + 2 * 64, 64);
+    memcpy(h.beta2, h2.data() + 0 * 128, 128);
+    memcpy(h.gamma2, h2.data() + 1 * 128, 128);
+    memcpy(h.delta2, h2.data() + 2 * 128, 128);
+    c->ic.assign((const uint64_t*)(h1.data() + ex.off_ic * 64),
+                 (const uint64_t*)(h1.data() + ex.off_ic * 64) + 8ull * (n_pub + 1));
+    finish_circuit_bases(*c, h1.data() + ex.off_a * 64, h1.data() + ex.off_b1 * 64, h2.data() + ex.off_b2 * 128,
+                         h1.data() + ex.off_c * 64, h1.data() + ex.off_h * 64);
+    return c;
+}
+
+void circuit_write_zkey(const Circuit& c, const std::string& path) {
+    const size_t nv = c.n_vars, l = c.n_public;
+    std::vector<uint64_t> A(nv * 8), B1(nv * 8), B2(nv * 16), C((nv - l - 1) * 8 + 8), H((size_t)c.n * 8);
+    ZK_HIP(hipMemcpy(A.data(), c.A.p, nv * 64, hipMemcpyDeviceToHost));
+    ZK_HIP(hipMemcpy(B1.data(), c.B1.p, nv * 64, hipMemcpyDeviceToHost));
+    ZK_HIP(hipMemcpy(B2.data(), c.B2.p, nv * 128, hipMemcpyDeviceToHost));
+    if (nv - l - 1) ZK_HIP(hipMemcpy(C.data(), (const char*)c.C.p + (l + 1) * 64, (nv - l - 1) * 64, hipMemcpyDeviceToHost));
+    ZK_HIP(hipMemcpy(H.data(), c.H.p, (size_t)c.n * 64, hipMemcpyDeviceToHost));
+    ZkeyWriterInput in;
+    in.hdr = c.hdr;
+    in.IC = c.ic.data();
+    in.n_coefs = (uint32_t)(c.coefs.size() / 44);
+    in.coefs = c.coefs.data();
+    in.A = A.data(); in.B1 = B1.data(); in.B2 = B2.data(); in.C = C.data(); in.H = H.data();
+    write_zkey(path, in);
+}
+
+// ------------------------------------------------------------------ proving
+ProveSlot::ProveSlot() {
+    ZK_HIP(hipStreamCreateWithFlags(&s0, hipStreamNonBlocking));
+    ZK_HIP(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
+    ZK_HIP(hipEventCreateWithFlags(&ev_scal, hipEventDisableTiming));
+    ZK_HIP(hipEventCreateWithFlags(&ev_done1, hipEventDisableTiming));
+}
+
+ProveSlot::~ProveSlot() {
+    if (s0) { (void)hipStreamSynchronize(s0); (void)hipStreamDestroy(s0); }
+    if (s1) { (void)hipStreamSynchronize(s1); (void)hipStreamDestroy(s1); }
+    if (ev_scal) (void)hipEventDestroy(ev_scal);
+    if (ev_done1) (void)hipEventDestroy(ev_done1);
+}
+
+size_t ProveSlot::device_bytes() const {
+    size_t b = scal.bytes + aw.bytes + bw.bytes + cw.bytes + t0.bytes + t1.bytes + t2.bytes + h.bytes;
+    b += sortW.counts.bytes + sortW.offsets.bytes + sortW.entries.bytes;
+    b += sortH.counts.bytes + sortH.offsets.bytes + sortH.entries.bytes;
+    for (const MsmScratch* s : {&scrA, &scrB1, &scrC, &scrB2, &scrH})
+        b += s->buckets.bytes + s->r1.bytes + s->t1.bytes + s->r2.bytes + s->t2.bytes + s->chunks.bytes + s->large.bytes +
+             s->partials.bytes;
+    return b;
+}
+
+static bool is_zero4(const uint64_t* v) { return !(v[0] | v[1] | v[2] | v[3]); }
+
+void ProveSlot::enqueue(Context& ctx, const Circuit& c, const uint64_t* witness, bool on_dev, const uint64_t r_[4],
+                        const uint64_t s_[4]) {
+    circ = &c;
+    for (int i = 0; i < 4; i++) { r[i] = r_ ? r_[i] : 0; s[i] = s_ ? s_[i] : 0; }
+    ZK_REQUIRE(Fr::is_canonical(r) && Fr::is_canonical(s), ZK_E_ARG, "blinding scalars must be canonical (< r)");
+    const size_t nv = c.n_vars, nS = nv + 4, n = c.n;
+    NttPlan& plan = ctx.ntt_plan(c.log_n);
+    scal.ensure(nS * 32);
+    for (DevBuf* b : {&aw, &bw, &cw, &t0, &t1, &t2, &h}) b->ensure(n * 32);
+    // scalars: witness, then [1, r, s, -r*s]
+    if (on_dev) ZK_HIP(hipMemcpyAsync(scal.p, witness, nv * 32, hipMemcpyDeviceToDevice, s0));
+    else ZK_HIP(hipMemcpyAsync(scal.p, witness, nv * 32, hipMemcpyHostToDevice, s0));
+    uint64_t ex[16] = {1, 0, 0, 0};
+    memcpy(ex + 4, r, 32);
+    memcpy(ex + 8, s, 32);
+    Fr rm = Fr::to_mont(Fr::from_raw(r)), sm = Fr::to_mont(Fr::from_raw(s));
+    (-(rm * sm)).canonical(ex + 12);
+    write_scalars4((char*)scal.p + nv * 32, ex, s0);
+    ZK_HIP(hipEventRecord(ev_scal, s0));
+    // H pipeline on s0
+    spmv_abc(c.rowA.as<uint32_t>(), c.colA.as<uint32_t>(), c.valA.p, c.n_rowsA, c.rowB.as<uint32_t>(),
+             c.colB.as<uint32_t>(), c.valB.p, c.n_rowsB, scal.p, n, aw.p, bw.p, cw.p, s0);
+    void* abc[3] = {aw.p, bw.p, cw.p};
+    void* tmp[3] = {t0.p, t1.p, t2.p};
+    ntt_run(plan, abc, tmp, 3, true, nullptr, plan.inv_coset.p, s0);  // coefficients * w_2n^i
+    ntt_run(plan, tmp, abc, 3, false, nullptr, nullptr, s0);          // values at w_2n^(2j+1)
+    h_final(aw.p, bw.p, cw.p, h.p, n, s0);
+    sortH.run(h.p, n, window_h, s0);
+    msm_launch(sortH, c.H.p, false, scrH, pH, s0);
+    // witness MSMs on s1 (one shared sort)
+    ZK_HIP(hipStreamWaitEvent(s1, ev_scal, 0));
+    sortW.run(scal.p, nS, window_w, s1);
+    msm_launch(sortW, c.A.p, false, scrA, pA, s1);
+    msm_launch(sortW, c.B1.p, false, scrB1, pB1, s1);
+    msm_launch(sortW, c.C.p, false, scrC, pC, s1);
+    msm_launch(sortW, c.B2.p, true, scrB2, pB2, s1);
+}
+
+template <class T>
+static void put_aff(const zkh::Xyzz<T>& p, uint64_t* dst) {
+    const int lw = sizeof(T) / 8;
+    zkh::Aff<T> a = zkh::xyzz_to_aff(p);
+    if (a.inf) { memset(dst, 0, 16 * lw); return; }
+    memcpy(dst, &a.x, 8 * lw);
+    memcpy(dst + lw, &a.y, 8 * lw);
+}
+
+void ProveSlot::finish(zk_proof* out) {
+    ZK_HIP(hipStreamSynchronize(s1));
+    auto A = msm_finish_g1(pA);
+    auto B1 = msm_finish_g1(pB1);
+    auto Cw = msm_finish_g1(pC);
+    auto B2 = msm_finish_g2(pB2);
+    ZK_HIP(hipStreamSynchronize(s0));
+    auto Hp = msm_finish_g1(pH);
+    auto C = zkh::xyzz_add(Cw, Hp);
+    if (!is_zero4(s)) C = zkh::xyzz_add(C, zkh::xyzz_mul(A, s));
+    if (!is_zero4(r)) C = zkh::xyzz_add(C, zkh::xyzz_mul(B1, r));
+    put_aff(A, out->pi_a);
+    put_aff(B2, out->pi_b);
+    put_aff(C, out->pi_c);
+}
+
+// ------------------------------------------------------------------ serialization
+static std::string dec_of_mont(const uint64_t* m) {
+    uint64_t c[4];
+    Fp::from_raw(m).canonical(c);
+    return zkh::to_decimal(c);
+}
+
+std::string proof_to_json(const zk_proof& p) {
+    char buf[2048];
+    snprintf(buf, sizeof buf,
+             "{\n"
+             " \"pi_a\": [\"%s\",\"%s\",\"1\"],\n"
+             " \"pi_b\": [[\"%s\",\"%s\"],[\"%s\",\"%s\"],[\"1\",\"0\"]],\n"
+             " \"pi_c\": [\"%s\",\"%s\",\"1\"],\n"
+             " \"protocol\": \"groth16\",\n"
+             " \"curve\": \"bn128\"\n"
+             "}",
+             dec_of_mont(p.pi_a).c_str(), dec_of_mont(p.pi_a + 4).c_str(), dec_of_mont(p.pi_b).c_str(),
+             dec_of_mont(p.pi_b + 4).c_str(), dec_of_mont(p.pi_b + 8).c_str(), dec_of_mont(p.pi_b + 12).c_str(),
+             dec_of_mont(p.pi_c).c_str(), dec_of_mont(p.pi_c + 4).c_str());
+    return buf;
+}
+
+static void be32(const uint64_t* mont, uint8_t* out) {
+    uint64_t c[4];
+    Fp::from_raw(mont).canonical(c);
+    for (int i = 0; i < 4; i++) {
+        uint64_t v = c[3 - i];
+        for (int j = 0; j < 8; j++) out[8 * i + j] = (uint8_t)(v >> (8 * (7 - j)));
+    }
+}
+
+void proof_to_calldata(const zk_proof& p, uint8_t out[256]) {
+    be32(p.pi_a, out);
+    be32(p.pi_a + 4, out + 32);
+    be32(p.pi_b + 4, out + 64);   // x.a1
+    be32(p.pi_b, out + 96);       // x.a0
+    be32(p.pi_b + 12, out + 128); // y.a1
+    be32(p.pi_b + 8, out + 160);  // y.a0
+    be32(p.pi_c, out + 192);
+    be32(p.pi_c + 4, out + 224);
+}
+
+std::string public_json(const uint64_t* w, uint32_t n_public) {
+    std::string s = "[\n";
+    for (uint32_t i = 0; i < n_public; i++) {
+        s += " \"" + zkh::to_decimal(w + 4ull * (i + 1)) + "\"";
+        s += (i + 1 < n_public) ? ",\n" : "\n";
+    }
+    s += "]\n";
+    return s;
+}
+
+}  // namespace zk

@@ -1,0 +1,92 @@
+// Groth16 prover orchestration on one MI355X.
+//
+// Reference: groth16Prove (src/groth16.cu:1143-1397). Differences by design:
+//   * a circuit is loaded once (CSR of A/B and every CRS base resident in
+//     HBM) and then proves any number of witnesses;
+//   * the four MSMs over [w | 1, r, s, -rs] (A, B1, C in G1, B2 in G2) share
+//     one digit sort: the base arrays carry alpha1/beta1/beta2/delta points in
+//     four extra slots, so the blinding terms and the alpha/beta offsets fall
+//     out of the MSMs themselves (snarkjs formula; r = s = 0 reproduces the
+//     reference's unblinded proof bit for bit);
+//   * the H pipeline (SpMV, iNTT + coset, NTT, h) and the witness MSMs run on
+//     two streams of one workspace slot; several slots overlap proofs;
+//   * only short serial tails run on the host: per-window combination of the
+//     MSM partials, s*pi_A + r*B1, and the affine conversion.
+#pragma once
+#include <stdint.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "../../include/zk_abi.h"
+#include "common.hpp"
+#include "msm.hpp"
+#include "ntt.hpp"
+#include "zkey.hpp"
+
+namespace zk {
+
+struct Context;
+
+struct Circuit {
+    Context* ctx = nullptr;
+    uint32_t n_vars = 0, n_public = 0, n = 0;
+    int log_n = 0;
+    uint32_t n_rowsA = 0, n_rowsB = 0;
+    size_t nnzA = 0, nnzB = 0;
+    DevBuf rowA, colA, valA, rowB, colB, valB;
+    // extended bases, n_vars + 4 points each (see header comment); H: n points
+    DevBuf A, B1, B2, C, H;
+    ZkeyHeader hdr;               // VK in file layout
+    std::vector<uint8_t> coefs;   // §4 bytes, kept to write the circuit back out
+    std::vector<uint64_t> ic;     // (l+1) x 8
+    size_t device_bytes() const;
+};
+
+struct ProveSlot {
+    hipStream_t s0 = nullptr, s1 = nullptr;
+    hipEvent_t ev_scal = nullptr, ev_done1 = nullptr;
+    DevBuf scal, aw, bw, cw, t0, t1, t2, h, wtmp;
+    MsmSort sortW, sortH;
+    MsmScratch scrA, scrB1, scrC, scrB2, scrH;
+    MsmPartials pA, pB1, pC, pB2, pH;
+    uint64_t r[4], s[4];
+    const Circuit* circ = nullptr;
+    int window_w = 0, window_h = 0;  // 0 = default per size
+    ProveSlot();
+    ~ProveSlot();
+    ProveSlot(const ProveSlot&) = delete;
+    ProveSlot& operator=(const ProveSlot&) = delete;
+    // queue all device work for one proof; witness is host (canonical) or device
+    void enqueue(Context& ctx, const Circuit& c, const uint64_t* witness, bool witness_on_device, const uint64_t r[4],
+                 const uint64_t s[4]);
+    // wait for the device work and run the host tail
+    void finish(zk_proof* out);
+    size_t device_bytes() const;
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::map<int, std::unique_ptr<NttPlan>> ntt_plans;
+    std::unique_ptr<ProveSlot> slot;   // slot used by the one-shot prove entry point
+    DevBuf comb_g1, comb_g2;           // fixed-base tables for synthetic setup
+    explicit Context(int dev);
+    ~Context();
+    NttPlan& ntt_plan(int log_n);
+    void ensure_comb();
+};
+
+std::unique_ptr<Circuit> circuit_from_zkey(Context& ctx, const std::string& path);
+std::unique_ptr<Circuit> circuit_synthetic(Context& ctx, uint64_t seed, uint32_t m, uint32_t n_pub);
+void circuit_write_zkey(const Circuit& c, const std::string& path);
+
+// serialization (src/groth16.cu:1432-1557, src/main.cu:28-110)
+std::string proof_to_json(const zk_proof& p);
+void proof_to_calldata(const zk_proof& p, uint8_t out[256]);
+std::string public_json(const uint64_t* w, uint32_t n_public);
+
+}  // namespace zk
