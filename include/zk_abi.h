@@ -90,6 +90,26 @@ int zk_set_windows(zk_ctx* ctx, int window_witness, int window_h);
 int zk_prove_batch(zk_ctx* ctx, int count, const zk_circuit* const* circuits, const uint64_t* const* witnesses,
                    const uint64_t* r, const uint64_t* s, zk_proof* out);
 
+/* Same, witnesses already in device memory (n_vars x 32 bytes each). */
+int zk_prove_batch_device(zk_ctx* ctx, int count, const zk_circuit* const* circuits, const void* const* d_witnesses,
+                          const uint64_t* r, const uint64_t* s, zk_proof* out);
+
+/* number of workspace slots (proofs in flight) used by the batch entry points, 1..16 */
+int zk_set_batch_slots(zk_ctx* ctx, int n);
+
+/* Per-phase device timing of proofs (profiling events on the prover's own
+ * streams; off by default). zk_prove_stats fills up to n values for the last
+ * proof finished by zk_prove / zk_prove_device on ctx:
+ *  [0] SpMV (+ witness copy) ms   [1] 3 x iNTT+coset ms   [2] 3 x NTT ms
+ *  [3] h + sort(h) ms             [4] shared witness sort ms
+ *  [5..9]  bucket-accumulation kernel ms for A, B1, C, B2, H
+ *  [10..14] whole MSM (accumulation .. partials on host) ms, same order
+ *  [15] digit entries of the witness sort  [16] of the h sort
+ *  [17] window c (witness) [18] c (h) [19] windows W (witness) [20] W (h)
+ *  [21] stream-0 span ms  [22] stream-1 span ms  [23] host tail ms */
+int zk_set_profiling(zk_ctx* ctx, int on);
+int zk_prove_stats(zk_ctx* ctx, double* out, int n);
+
 /* ---- serialization (groth16ProofToSnarkjsJson / groth16ProofToCalldata,
  * include/groth16.cuh:149-167; public.json: src/main.cu:28-110) ---- */
 int zk_proof_to_json(const zk_proof* p, char* out, size_t out_size);
@@ -146,6 +166,9 @@ int zk_last_timing(zk_ctx* ctx, float ms[2]);
 int zk_synth_circuit(zk_ctx* ctx, uint64_t seed, uint32_t m, uint32_t n_pub, zk_circuit** out);
 /* satisfying witness; public inputs from the seed (pub_seed == seed) or fresh (any other value) */
 int zk_synth_witness(uint64_t seed, uint32_t m, uint32_t n_pub, uint64_t pub_seed, uint64_t* w_out);
+/* satisfying witness of a circuit built by zk_synth_circuit, public inputs
+ * drawn from pub_seed's stream (n_vars x 4 words); reuses the resident R1CS */
+int zk_circuit_witness(const zk_circuit* c, uint64_t pub_seed, uint64_t* w_out);
 int zk_synth_blinding(uint64_t seed, uint64_t r[4], uint64_t s[4]);
 int zk_synth_uniform(uint64_t seed, size_t count, uint64_t* out);
 /* n_vars / domain of the synthetic circuit (m, n_pub) */

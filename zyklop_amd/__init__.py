@@ -64,6 +64,11 @@ ABI = {
     "zk_set_windows": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "zk_prove_batch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_u64p), _u64p, _u64p,
                                       ctypes.POINTER(Proof)]),
+    "zk_prove_batch_device": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _u64p, _u64p,
+                                             ctypes.POINTER(Proof)]),
+    "zk_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "zk_set_batch_slots": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "zk_prove_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
     "zk_proof_to_json": (ctypes.c_int, [ctypes.POINTER(Proof), ctypes.c_char_p, ctypes.c_size_t]),
     "zk_proof_to_calldata": (ctypes.c_int, [ctypes.POINTER(Proof), _u8p, ctypes.c_size_t]),
     "zk_public_json": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t]),
@@ -90,6 +95,7 @@ ABI = {
     "zk_last_timing": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
     "zk_synth_circuit": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_vp)]),
     "zk_synth_witness": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, _u64p]),
+    "zk_circuit_witness": (ctypes.c_int, [_vp, ctypes.c_uint64, _u64p]),
     "zk_synth_blinding": (ctypes.c_int, [ctypes.c_uint64, _u64p, _u64p]),
     "zk_synth_uniform": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_size_t, _u64p]),
     "zk_synth_sizes": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p]),
@@ -283,6 +289,35 @@ class Context:
         return list(out)
 
 
+    def prove_batch_device(self, circuits, d_witnesses, r=None, s=None):
+        n = len(circuits)
+        ch = (_vp * n)(*[c.h for c in circuits])
+        wp = (_vp * n)(*d_witnesses)
+        rr = np.ascontiguousarray(r, dtype=np.uint64).reshape(-1) if r is not None else None
+        ss = np.ascontiguousarray(s, dtype=np.uint64).reshape(-1) if s is not None else None
+        out = (Proof * n)()
+        _check(lib().zk_prove_batch_device(self.h, n, ch, wp, u64p(rr) if rr is not None else None,
+                                           u64p(ss) if ss is not None else None, out))
+        return list(out)
+
+    def set_batch_slots(self, n):
+        _check(lib().zk_set_batch_slots(self.h, n))
+
+    def set_profiling(self, on=True):
+        _check(lib().zk_set_profiling(self.h, 1 if on else 0))
+
+    STAT_NAMES = ["spmv_ms", "intt3_ms", "ntt3_ms", "h_sort_ms", "sort_w_ms",
+                  "acc_A_ms", "acc_B1_ms", "acc_C_ms", "acc_B2_ms", "acc_H_ms",
+                  "msm_A_ms", "msm_B1_ms", "msm_C_ms", "msm_B2_ms", "msm_H_ms",
+                  "entries_w", "entries_h", "c_w", "c_h", "windows_w", "windows_h",
+                  "stream0_ms", "stream1_ms", "host_tail_ms"]
+
+    def prove_stats(self):
+        buf = (ctypes.c_double * 24)()
+        _check(lib().zk_prove_stats(self.h, buf, 24))
+        return dict(zip(self.STAT_NAMES, [float(x) for x in buf]))
+
+
 class Circuit:
     """A circuit resident in HBM (CSR of A/B + all CRS bases)."""
 
@@ -304,6 +339,12 @@ class Circuit:
 
     def write_zkey(self, path):
         _check(lib().zk_circuit_write_zkey(self.h, path.encode()))
+
+    def witness(self, pub_seed):
+        """Satisfying witness of a synthetic circuit with public inputs from pub_seed."""
+        w = np.zeros((self.n_vars, 4), np.uint64)
+        _check(lib().zk_circuit_witness(self.h, pub_seed, u64p(w)))
+        return w
 
 
 # ---- reference-API mirror (include/groth16.cuh) ----

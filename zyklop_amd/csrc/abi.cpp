@@ -17,6 +17,8 @@ struct zk_ctx {
     std::vector<std::unique_ptr<zk::ProveSlot>> batch_slots;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     float last_ms[2] = {0, 0};
+    bool profiling = false;
+    int batch_slots_n = 3;
     explicit zk_ctx(int dev) : impl(dev) {
         ZK_HIP(hipEventCreate(&ev0));
         ZK_HIP(hipEventCreate(&ev1));
@@ -71,6 +73,35 @@ void check_scalars(const uint64_t* sc, size_t n) {
 zk::ProveSlot& ctx_slot(zk_ctx* ctx) {
     if (!ctx->impl.slot) ctx->impl.slot = std::make_unique<zk::ProveSlot>();
     return *ctx->impl.slot;
+}
+
+// Several workspace slots in flight: while the host finishes proof i, the
+// device already runs the next ones queued on the other slots' streams.
+void run_batch(zk_ctx* ctx, int count, const zk_circuit* const* circuits, const uint64_t* const* witnesses, bool on_dev,
+               const uint64_t* r, const uint64_t* s, zk_proof* out) {
+    const int nslots = ctx->batch_slots_n;
+    while ((int)ctx->batch_slots.size() < nslots) {
+        ctx->batch_slots.push_back(std::make_unique<zk::ProveSlot>());
+        zk::ProveSlot& b = *ctx->batch_slots.back();
+        zk::ProveSlot& d = ctx_slot(ctx);
+        b.window_w = d.window_w;
+        b.window_h = d.window_h;
+    }
+    for (int i = 0; i < count; i++) need(circuits[i] && witnesses[i], "NULL circuit or witness");
+    int next = 0;
+    const int inflight = std::min(count, nslots);
+    for (; next < inflight; next++)
+        ctx->batch_slots[next % nslots]->enqueue(ctx->impl, *circuits[next]->impl, witnesses[next], on_dev,
+                                                 r ? r + 4 * next : nullptr, s ? s + 4 * next : nullptr);
+    for (int i = 0; i < count; i++) {
+        zk::ProveSlot& sl = *ctx->batch_slots[i % nslots];
+        sl.finish(&out[i]);
+        if (next < count) {
+            sl.enqueue(ctx->impl, *circuits[next]->impl, witnesses[next], on_dev, r ? r + 4 * next : nullptr,
+                       s ? s + 4 * next : nullptr);
+            next++;
+        }
+    }
 }
 
 template <class HT>
@@ -226,34 +257,52 @@ int zk_prove_batch(zk_ctx* ctx, int count, const zk_circuit* const* circuits, co
                    const uint64_t* r, const uint64_t* s, zk_proof* out) {
     return guard([&] {
         need(ctx && circuits && witnesses && out && count >= 0, "bad argument");
-        std::lock_guard<std::mutex> lk(ctx->impl.mu);
-        ZK_HIP(hipSetDevice(ctx->impl.device));
         for (int i = 0; i < count; i++) {
             need(circuits[i] && witnesses[i], "NULL circuit or witness");
             check_scalars(witnesses[i], circuits[i]->impl->n_vars);
         }
-        const int nslots = 3;
-        while ((int)ctx->batch_slots.size() < nslots) {
-            ctx->batch_slots.push_back(std::make_unique<zk::ProveSlot>());
-            zk::ProveSlot& b = *ctx->batch_slots.back();
-            zk::ProveSlot& d = ctx_slot(ctx);
-            b.window_w = d.window_w;
-            b.window_h = d.window_h;
-        }
-        int next = 0;
-        int inflight = std::min(count, nslots);
-        for (; next < inflight; next++)
-            ctx->batch_slots[next % nslots]->enqueue(ctx->impl, *circuits[next]->impl, witnesses[next], false,
-                                                     r ? r + 4 * next : nullptr, s ? s + 4 * next : nullptr);
-        for (int i = 0; i < count; i++) {
-            zk::ProveSlot& sl = *ctx->batch_slots[i % nslots];
-            sl.finish(&out[i]);
-            if (next < count) {
-                sl.enqueue(ctx->impl, *circuits[next]->impl, witnesses[next], false, r ? r + 4 * next : nullptr,
-                           s ? s + 4 * next : nullptr);
-                next++;
-            }
-        }
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        run_batch(ctx, count, circuits, witnesses, false, r, s, out);
+    });
+}
+
+int zk_prove_batch_device(zk_ctx* ctx, int count, const zk_circuit* const* circuits, const void* const* d_witnesses,
+                          const uint64_t* r, const uint64_t* s, zk_proof* out) {
+    return guard([&] {
+        need(ctx && circuits && d_witnesses && out && count >= 0, "bad argument");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        run_batch(ctx, count, circuits, (const uint64_t* const*)d_witnesses, true, r, s, out);
+    });
+}
+
+int zk_set_batch_slots(zk_ctx* ctx, int n) {
+    return guard([&] {
+        need(ctx != nullptr, "NULL ctx");
+        need(n >= 1 && n <= 16, "slots must be 1..16");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ctx->batch_slots_n = n;
+        if ((int)ctx->batch_slots.size() > n) ctx->batch_slots.resize(n);
+    });
+}
+
+int zk_set_profiling(zk_ctx* ctx, int on) {
+    return guard([&] {
+        need(ctx != nullptr, "NULL ctx");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ctx->profiling = on != 0;
+        if (on) ctx_slot(ctx).enable_timing();
+    });
+}
+
+int zk_prove_stats(zk_ctx* ctx, double* out, int n) {
+    return guard([&] {
+        need(ctx && out, "NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        zk::ProveSlot& sl = ctx_slot(ctx);
+        need(sl.timed, "profiling is off (zk_set_profiling)");
+        for (int i = 0; i < n && i < zk::ProveSlot::N_STATS; i++) out[i] = sl.stats[i];
     });
 }
 
@@ -525,6 +574,18 @@ int zk_synth_witness(uint64_t seed, uint32_t m, uint32_t n_pub, uint64_t pub_see
         zk::SynthR1cs r1;
         zk::synth_r1cs(seed, m, n_pub, r1);
         std::vector<zkh::Fr> pub = zk::synth_public(pub_seed, n_pub);
+        std::vector<uint64_t> w;
+        zk::synth_witness_from_public(r1, pub, w);
+        memcpy(w_out, w.data(), w.size() * 8);
+    });
+}
+
+int zk_circuit_witness(const zk_circuit* c, uint64_t pub_seed, uint64_t* w_out) {
+    return guard([&] {
+        need(c && w_out, "NULL argument");
+        ZK_REQUIRE(c->impl->synth != nullptr, ZK_E_STATE, "circuit was not built by zk_synth_circuit");
+        const zk::SynthR1cs& r1 = *c->impl->synth;
+        std::vector<zkh::Fr> pub = zk::synth_public(pub_seed, r1.l);
         std::vector<uint64_t> w;
         zk::synth_witness_from_public(r1, pub, w);
         memcpy(w_out, w.data(), w.size() * 8);

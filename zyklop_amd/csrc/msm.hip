@@ -294,6 +294,19 @@ __global__ void __launch_bounds__(BLK) k_msm_reduce2(const void* __restrict__ Ri
     }
 }
 
+void MsmScratch::enable_timing() {
+    if (ev_acc0) return;
+    ZK_HIP(hipEventCreate(&ev_acc0));
+    ZK_HIP(hipEventCreate(&ev_acc1));
+    ZK_HIP(hipEventCreate(&ev_end));
+}
+
+MsmScratch::~MsmScratch() {
+    if (ev_acc0) (void)hipEventDestroy(ev_acc0);
+    if (ev_acc1) (void)hipEventDestroy(ev_acc1);
+    if (ev_end) (void)hipEventDestroy(ev_end);
+}
+
 void MsmScratch::ensure(const MsmGeom& g, size_t n_entries, bool g2) {
     size_t xb = g2 ? 256 : 128;
     size_t nb = (size_t)g.W * g.K;
@@ -317,10 +330,12 @@ static void launch_t(const MsmSort& sort, const void* d_points, MsmScratch& scr,
     scr.ensure(g, (size_t)g.W * sort.n, g2);
     const uint32_t nb = (uint32_t)((size_t)g.W * g.K);
     fill_u32(scr.counters.p, 0u, 16, st);
+    if (scr.ev_acc0) ZK_HIP(hipEventRecord(scr.ev_acc0, st));
     k_msm_accum<T><<<cdiv(nb, 128), 128, 0, st>>>(sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(), nb, d_points,
                                                   scr.buckets.p, scr.counters.as<uint32_t>(), scr.chunks.as<ChunkRec>(),
                                                   scr.large.as<LargeRec>());
     ZK_HIP(hipGetLastError());
+    if (scr.ev_acc1) ZK_HIP(hipEventRecord(scr.ev_acc1, st));
     k_msm_chunks<T><<<512, 256, 0, st>>>(sort.entries.as<uint32_t>(), d_points, scr.counters.as<uint32_t>(),
                                           scr.chunks.as<ChunkRec>(), scr.partials.p);
     ZK_HIP(hipGetLastError());
@@ -341,6 +356,7 @@ static void launch_t(const MsmSort& sort, const void* d_points, MsmScratch& scr,
     out.host.ensure(2 * half);
     ZK_HIP(hipMemcpyAsync(out.host.p, scr.r2.p, half, hipMemcpyDeviceToHost, st));
     ZK_HIP(hipMemcpyAsync((char*)out.host.p + half, scr.t2.p, half, hipMemcpyDeviceToHost, st));
+    if (scr.ev_end) ZK_HIP(hipEventRecord(scr.ev_end, st));
 }
 
 void msm_launch(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, MsmPartials& out, hipStream_t st) {

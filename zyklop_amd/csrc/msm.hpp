@@ -53,6 +53,10 @@ struct MsmPartials {
 struct MsmScratch {
     DevBuf buckets, r1, t1, r2, t2, chunks, large, partials, counters;
     size_t cap_buckets = 0;
+    // optional events around the bucket-accumulation kernel (profiling)
+    hipEvent_t ev_acc0 = nullptr, ev_acc1 = nullptr, ev_end = nullptr;
+    void enable_timing();
+    ~MsmScratch();
     void ensure(const MsmGeom& g, size_t n_entries, bool g2);
 };
 

@@ -2,6 +2,7 @@
 
 #include <string.h>
 
+#include <chrono>
 #include <cstdio>
 
 #include "qap.hpp"
@@ -201,6 +202,7 @@ std::unique_ptr<Circuit> circuit_synthetic(Context& ctx, uint64_t seed, uint32_t
                  (const uint64_t*)(h1.data() + ex.off_ic * 64) + 8ull * (n_pub + 1));
     finish_circuit_bases(*c, h1.data() + ex.off_a * 64, h1.data() + ex.off_b1 * 64, h2.data() + ex.off_b2 * 128,
                          h1.data() + ex.off_c * 64, h1.data() + ex.off_h * 64);
+    c->synth = std::make_unique<SynthR1cs>(std::move(r1));
     return c;
 }
 
@@ -229,7 +231,16 @@ ProveSlot::ProveSlot() {
     ZK_HIP(hipEventCreateWithFlags(&ev_done1, hipEventDisableTiming));
 }
 
+void ProveSlot::enable_timing() {
+    if (timed) return;
+    for (auto& e : ev) ZK_HIP(hipEventCreate(&e));
+    for (MsmScratch* m : {&scrA, &scrB1, &scrC, &scrB2, &scrH}) m->enable_timing();
+    timed = true;
+}
+
 ProveSlot::~ProveSlot() {
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
     if (s0) { (void)hipStreamSynchronize(s0); (void)hipStreamDestroy(s0); }
     if (s1) { (void)hipStreamSynchronize(s1); (void)hipStreamDestroy(s1); }
     if (ev_scal) (void)hipEventDestroy(ev_scal);
@@ -257,6 +268,7 @@ void ProveSlot::enqueue(Context& ctx, const Circuit& c, const uint64_t* witness,
     NttPlan& plan = ctx.ntt_plan(c.log_n);
     scal.ensure(nS * 32);
     for (DevBuf* b : {&aw, &bw, &cw, &t0, &t1, &t2, &h}) b->ensure(n * 32);
+    if (timed) ZK_HIP(hipEventRecord(ev[0], s0));
     // scalars: witness, then [1, r, s, -r*s]
     if (on_dev) ZK_HIP(hipMemcpyAsync(scal.p, witness, nv * 32, hipMemcpyDeviceToDevice, s0));
     else ZK_HIP(hipMemcpyAsync(scal.p, witness, nv * 32, hipMemcpyHostToDevice, s0));
@@ -270,16 +282,21 @@ void ProveSlot::enqueue(Context& ctx, const Circuit& c, const uint64_t* witness,
     // H pipeline on s0
     spmv_abc(c.rowA.as<uint32_t>(), c.colA.as<uint32_t>(), c.valA.p, c.n_rowsA, c.rowB.as<uint32_t>(),
              c.colB.as<uint32_t>(), c.valB.p, c.n_rowsB, scal.p, n, aw.p, bw.p, cw.p, s0);
+    if (timed) ZK_HIP(hipEventRecord(ev[1], s0));
     void* abc[3] = {aw.p, bw.p, cw.p};
     void* tmp[3] = {t0.p, t1.p, t2.p};
     ntt_run(plan, abc, tmp, 3, true, nullptr, plan.inv_coset.p, s0);  // coefficients * w_2n^i
+    if (timed) ZK_HIP(hipEventRecord(ev[2], s0));
     ntt_run(plan, tmp, abc, 3, false, nullptr, nullptr, s0);          // values at w_2n^(2j+1)
+    if (timed) ZK_HIP(hipEventRecord(ev[3], s0));
     h_final(aw.p, bw.p, cw.p, h.p, n, s0);
     sortH.run(h.p, n, window_h, s0);
     msm_launch(sortH, c.H.p, false, scrH, pH, s0);
     // witness MSMs on s1 (one shared sort)
     ZK_HIP(hipStreamWaitEvent(s1, ev_scal, 0));
+    if (timed) ZK_HIP(hipEventRecord(ev[4], s1));
     sortW.run(scal.p, nS, window_w, s1);
+    if (timed) ZK_HIP(hipEventRecord(ev[5], s1));
     msm_launch(sortW, c.A.p, false, scrA, pA, s1);
     msm_launch(sortW, c.B1.p, false, scrB1, pB1, s1);
     msm_launch(sortW, c.C.p, false, scrC, pC, s1);
@@ -296,12 +313,16 @@ static void put_aff(const zkh::Xyzz<T>& p, uint64_t* dst) {
 }
 
 void ProveSlot::finish(zk_proof* out) {
+    using clk = std::chrono::steady_clock;
     ZK_HIP(hipStreamSynchronize(s1));
+    auto h0 = clk::now();
     auto A = msm_finish_g1(pA);
     auto B1 = msm_finish_g1(pB1);
     auto Cw = msm_finish_g1(pC);
     auto B2 = msm_finish_g2(pB2);
+    double host_ms = std::chrono::duration<double, std::milli>(clk::now() - h0).count();
     ZK_HIP(hipStreamSynchronize(s0));
+    h0 = clk::now();
     auto Hp = msm_finish_g1(pH);
     auto C = zkh::xyzz_add(Cw, Hp);
     if (!is_zero4(s)) C = zkh::xyzz_add(C, zkh::xyzz_mul(A, s));
@@ -309,6 +330,42 @@ void ProveSlot::finish(zk_proof* out) {
     put_aff(A, out->pi_a);
     put_aff(B2, out->pi_b);
     put_aff(C, out->pi_c);
+    host_ms += std::chrono::duration<double, std::milli>(clk::now() - h0).count();
+    if (timed) {
+        collect_stats();
+        stats[23] = host_ms;
+    }
+}
+
+void ProveSlot::collect_stats() {
+    auto el = [](hipEvent_t a, hipEvent_t b) {
+        float ms = 0;
+        ZK_HIP(hipEventElapsedTime(&ms, a, b));
+        return (double)ms;
+    };
+    for (double& d : stats) d = 0;
+    stats[0] = el(ev[0], ev[1]);   // scalar upload + SpMV
+    stats[1] = el(ev[1], ev[2]);   // 3 x (iNTT + coset), batched
+    stats[2] = el(ev[2], ev[3]);   // 3 x NTT, batched
+    stats[3] = el(ev[3], scrH.ev_acc0);  // h + sort of h
+    stats[4] = el(ev[4], ev[5]);   // shared sort of [w | 1, r, s, -rs]
+    MsmScratch* m[5] = {&scrA, &scrB1, &scrC, &scrB2, &scrH};
+    for (int i = 0; i < 5; i++) {
+        stats[5 + i] = el(m[i]->ev_acc0, m[i]->ev_acc1);
+        stats[10 + i] = el(m[i]->ev_acc0, m[i]->ev_end);
+    }
+    uint32_t ew = 0, eh = 0;
+    size_t nbw = (size_t)sortW.g.W * sortW.g.K, nbh = (size_t)sortH.g.W * sortH.g.K;
+    ZK_HIP(hipMemcpy(&ew, sortW.offsets.as<uint32_t>() + nbw, 4, hipMemcpyDeviceToHost));
+    ZK_HIP(hipMemcpy(&eh, sortH.offsets.as<uint32_t>() + nbh, 4, hipMemcpyDeviceToHost));
+    stats[15] = ew;
+    stats[16] = eh;
+    stats[17] = sortW.g.c;
+    stats[18] = sortH.g.c;
+    stats[19] = sortW.g.W;
+    stats[20] = sortH.g.W;
+    stats[21] = el(ev[0], scrH.ev_end);   // stream 0 span
+    stats[22] = el(ev[4], scrB2.ev_end);  // stream 1 span
 }
 
 // ------------------------------------------------------------------ serialization

@@ -24,6 +24,7 @@
 #include "common.hpp"
 #include "msm.hpp"
 #include "ntt.hpp"
+#include "synth.hpp"
 #include "zkey.hpp"
 
 namespace zk {
@@ -42,6 +43,7 @@ struct Circuit {
     ZkeyHeader hdr;               // VK in file layout
     std::vector<uint8_t> coefs;   // §4 bytes, kept to write the circuit back out
     std::vector<uint64_t> ic;     // (l+1) x 8
+    std::unique_ptr<SynthR1cs> synth;  // host R1CS of synthetic circuits (witness generation)
     size_t device_bytes() const;
 };
 
@@ -55,6 +57,12 @@ struct ProveSlot {
     uint64_t r[4], s[4];
     const Circuit* circ = nullptr;
     int window_w = 0, window_h = 0;  // 0 = default per size
+    // profiling (off by default): events on both streams, read after finish()
+    bool timed = false;
+    hipEvent_t ev[8] = {};
+    double stats[24] = {};
+    void enable_timing();
+    void collect_stats();
     ProveSlot();
     ~ProveSlot();
     ProveSlot(const ProveSlot&) = delete;
@@ -64,6 +72,8 @@ struct ProveSlot {
                  const uint64_t s[4]);
     // wait for the device work and run the host tail
     void finish(zk_proof* out);
+    // number of per-proof statistics in stats[] (layout in zk_abi.h: zk_prove_stats)
+    static constexpr int N_STATS = 24;
     size_t device_bytes() const;
 };
 
