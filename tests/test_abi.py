@@ -182,3 +182,16 @@ def test_wtns_roundtrip(tmp_path):
     p = str(tmp_path / "x.wtns")
     zk.write_wtns(p, w)
     assert np.array_equal(zk.read_wtns(p), w)
+
+
+def test_groth16_header_compiles(tmp_path):
+    """include/groth16.cuh (drop-in C++ API) compiles and links against libzkp.so."""
+    import subprocess
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "groth16.cuh"\n'
+                   'int main(int c, char** v) { Groth16Proof p; if (c > 3) groth16Prove(v[1], v[2], p);\n'
+                   '  char j[2048]; groth16ProofToSnarkjsJson(p, j, sizeof j); return 0; }\n')
+    out = tmp_path / "use"
+    r = subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(out),
+                        "-L", os.path.join(ROOT, "zyklop_amd"), "-lzkp"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

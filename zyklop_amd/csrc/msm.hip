@@ -9,7 +9,7 @@
 
 namespace zk {
 
-constexpr uint32_t L_MAX = 96;        // a bucket above this many entries takes the chunked path
+constexpr uint32_t L_MIN_LARGE = 64;  // floor of the oversized-bucket threshold
 constexpr uint32_t CHUNK = 256 * 16;  // entries per chunk of an oversized bucket
 constexpr int L1 = 8;                 // buckets per level-1 reduction thread
 constexpr int L2 = 4;                 // level-1 segments per level-2 thread
@@ -129,16 +129,20 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------- buckets
+// One thread per bucket, highest bucket index first: the top window holds
+// fewer scalar bits, so its buckets are the fullest and should not form the
+// tail of the launch. Buckets above lmax entries go to the chunked path.
 template <class T>
 __global__ void __launch_bounds__(256) k_msm_accum(const uint32_t* __restrict__ entries, const uint32_t* __restrict__ offsets,
-                                                   uint32_t nb, const void* __restrict__ points, void* __restrict__ buckets,
-                                                   uint32_t* __restrict__ counters, ChunkRec* __restrict__ chunks,
-                                                   LargeRec* __restrict__ large) {
-    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
+                                                   uint32_t nb, uint32_t lmax, const void* __restrict__ points,
+                                                   void* __restrict__ buckets, uint32_t* __restrict__ counters,
+                                                   ChunkRec* __restrict__ chunks, LargeRec* __restrict__ large) {
+    uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= nb) return;
+    uint32_t b = nb - 1 - tid;
     uint32_t st = offsets[b], en = offsets[b + 1];
     uint32_t cnt = en - st;
-    if (cnt > L_MAX) {
+    if (cnt > lmax) {
         uint32_t nch = (cnt + CHUNK - 1) / CHUNK;
         uint32_t first = atomicAdd(&counters[0], nch);
         for (uint32_t j = 0; j < nch; j++) {
@@ -315,8 +319,8 @@ void MsmScratch::ensure(const MsmGeom& g, size_t n_entries, bool g2) {
     t1.ensure((size_t)g.W * g.nseg * xb);
     r2.ensure((size_t)g.W * g.nblk * xb);
     t2.ensure((size_t)g.W * g.nblk * xb);
-    size_t max_chunks = n_entries / CHUNK + n_entries / L_MAX + 2;
-    size_t max_large = n_entries / L_MAX + 2;
+    size_t max_chunks = n_entries / CHUNK + n_entries / L_MIN_LARGE + 2;
+    size_t max_large = n_entries / L_MIN_LARGE + 2;
     chunks.ensure(max_chunks * sizeof(ChunkRec));
     large.ensure(max_large * sizeof(LargeRec));
     partials.ensure(max_chunks * xb);
@@ -331,7 +335,9 @@ static void launch_t(const MsmSort& sort, const void* d_points, MsmScratch& scr,
     const uint32_t nb = (uint32_t)((size_t)g.W * g.K);
     fill_u32(scr.counters.p, 0u, 16, st);
     if (scr.ev_acc0) ZK_HIP(hipEventRecord(scr.ev_acc0, st));
-    k_msm_accum<T><<<cdiv(nb, 128), 128, 0, st>>>(sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(), nb, d_points,
+    // threshold: 8x the mean bucket load (uniform scalars put n/K entries in a bucket)
+    const uint32_t lmax = std::max<uint32_t>(L_MIN_LARGE, 8u * (uint32_t)((sort.n + g.K - 1) / g.K));
+    k_msm_accum<T><<<cdiv(nb, 128), 128, 0, st>>>(sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(), nb, lmax, d_points,
                                                   scr.buckets.p, scr.counters.as<uint32_t>(), scr.chunks.as<ChunkRec>(),
                                                   scr.large.as<LargeRec>());
     ZK_HIP(hipGetLastError());

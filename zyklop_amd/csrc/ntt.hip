@@ -1,4 +1,7 @@
-// Fr NTT kernels (design in ntt.hpp).
+// Fr NTT kernels (design in ntt.hpp): register-radix passes of the four-step
+// recursion. Each thread owns one column of R = 2^lr points (lr <= 4), does
+// the whole radix-R DFT in registers, applies the inter-pass twiddle and
+// writes back; the last pass writes the digit-reversed (natural) order.
 #include <vector>
 
 #include "field.hpp"
@@ -7,7 +10,6 @@
 
 namespace zk {
 
-constexpr int NTT_TILE = 2048;  // elements resident in LDS per workgroup
 constexpr int NTT_THREADS = 256;
 constexpr int NTT_MAX_BATCH = 4;
 
@@ -16,126 +18,118 @@ struct NttPtrs {
     void* out[NTT_MAX_BATCH];
 };
 
-struct NttShape {
-    int np, j;
-    int lr[4];
+// w_R^i for i < R/2 (Montgomery), passed by value so they live in SGPRs
+struct RootsR {
+    Fr w[8];
 };
 
-// LDS tile: limb pairs, structure-of-arrays, so lanes touching consecutive
-// elements hit consecutive banks.
-struct LdsTile {
-    uint2 w[4][NTT_TILE];
-    ZK_DEV Fr ld(int i) const {
-        Fr r;
-#pragma unroll
-        for (int k = 0; k < 4; k++) { uint2 v = w[k][i]; r.v[2 * k] = v.x; r.v[2 * k + 1] = v.y; }
-        return r;
-    }
-    ZK_DEV void st(int i, const Fr& a) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) w[k][i] = make_uint2(a.v[2 * k], a.v[2 * k + 1]);
-    }
+struct NttDigits {
+    int np;
+    int lr[NTT_MAX_PASSES];
 };
 
-ZK_DEV uint32_t bitrev(uint32_t x, int bits) { return bits ? (__brev(x) >> (32 - bits)) : 0u; }
-
-// radix-2 DIT over T columns of R = 2^lr points held in LDS in bit-reversed order;
-// tw[i] = w_R^i for i < R/2. Leaves natural order.
-ZK_DEV void lds_dft(LdsTile& L, int lr, int T, const void* __restrict__ tw) {
-    const int R = 1 << lr, halfR = R >> 1;
-    const int nbf = T * halfR;
-    for (int s = 1; s <= lr; s++) {
-        const int half = 1 << (s - 1);
-        const int tstride = R >> s;  // w_{2^s}^pos = w_R^(pos * R / 2^s)
-        __syncthreads();
-        for (int q = threadIdx.x; q < nbf; q += NTT_THREADS) {
-            int ct = q >> (lr - 1);
-            int u = q & (halfR - 1);
-            int grp = u >> (s - 1), pos = u & (half - 1);
-            int i0 = ct * R + grp * 2 * half + pos, i1 = i0 + half;
-            Fr a = L.ld(i0), b = L.ld(i1);
-            if (pos) b = mul<FrCfg>(b, load<FrCfg>(tw, (size_t)pos * tstride));
-            L.st(i0, add<FrCfg>(a, b));
-            L.st(i1, sub<FrCfg>(a, b));
-        }
-    }
-    __syncthreads();
+constexpr int bitrev_c(int x, int bits) {
+    int r = 0;
+    for (int i = 0; i < bits; i++) r |= ((x >> i) & 1) << (bits - 1 - i);
+    return r;
 }
 
-// In-place pass j < np-1: block beta of size M = R*S, columns [c0, c0+T).
-__global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass(NttPtrs ptrs, NttShape sh, int log_n, const void* __restrict__ tw,
-                                                          const void* __restrict__ small, const void* __restrict__ pre) {
-    __shared__ LdsTile L;
-    void* data = ptrs.in[blockIdx.y];
-    int consumed = 0;
-    for (int q = 0; q < sh.j; q++) consumed += sh.lr[q];
-    const int lr = sh.lr[sh.j];
-    const int logM = log_n - consumed, logS = logM - lr;
-    const size_t S = (size_t)1 << logS;
-    const int R = 1 << lr;
-    const int T = (int)min((size_t)(NTT_TILE >> lr), S);
-    const size_t tiles_per_block = S / T;
-    const size_t beta = blockIdx.x / tiles_per_block, gamma = blockIdx.x - beta * tiles_per_block;
-    const size_t base = beta << logM;
-    const size_t c0 = gamma * T;
-    const int nel = T * R;
-    for (int q = threadIdx.x; q < nel; q += NTT_THREADS) {
-        int b = q / T, ct = q - b * T;
-        size_t g = base + ((size_t)b << logS) + c0 + ct;
-        Fr x = load<FrCfg>(data, g);
-        if (pre) x = mul<FrCfg>(x, load<FrCfg>(pre, g));
-        L.st(ct * R + (int)bitrev((uint32_t)b, lr), x);
+// one radix-2 DIT stage (span 2^(S-1)) of a 2^LR-point DFT held in registers;
+// every loop has a constant trip count so the whole DFT unrolls to registers
+template <int LR, int S>
+struct DftStage {
+    ZK_DEV static void run(Fr (&x)[1 << LR], const RootsR& rt) {
+        constexpr int R = 1 << LR, half = 1 << (S - 1);
+#pragma unroll
+        for (int q = 0; q < R / 2; q++) {
+            const int p = q & (half - 1);
+            const int i0 = (q >> (S - 1)) * 2 * half + p, i1 = i0 + half;
+            Fr a = x[i0], b = x[i1];
+            if (p) b = mul<FrCfg>(b, rt.w[p * (R >> S)]);
+            x[i0] = add<FrCfg>(a, b);
+            x[i1] = sub<FrCfg>(a, b);
+        }
+        DftStage<LR, S + 1>::run(x, rt);
     }
-    lds_dft(L, lr, T, small);
-    for (int q = threadIdx.x; q < nel; q += NTT_THREADS) {
-        int k = q / T, ct = q - k * T;
-        size_t c = c0 + ct;
-        Fr y = L.ld(ct * R + k);
+};
+template <int LR>
+struct DftStage<LR, LR + 1> {
+    ZK_DEV static void run(Fr (&)[1 << LR], const RootsR&) {}
+};
+
+// radix-2^LR DFT in registers, X_k = sum_b x_b w_R^(b k); the caller loads
+// input b into x[bitrev(b)] and gets natural-order output
+template <int LR>
+ZK_DEV void dft_reg(Fr (&x)[1 << LR], const RootsR& rt) {
+    DftStage<LR, 1>::run(x, rt);
+}
+
+// In-place pass j < np-1 over blocks of M = R*S elements: thread (beta, c)
+// transforms x[beta*M + b*S + c], b < R, then multiplies output k by w_M^(c k).
+template <int LR>
+__global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass(NttPtrs ptrs, int logM, size_t n, RootsR rt,
+                                                          const void* __restrict__ tw, const void* __restrict__ pre) {
+    constexpr int R = 1 << LR;
+    size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (n >> LR)) return;
+    void* data = ptrs.in[blockIdx.y];
+    const int logS = logM - LR;
+    const size_t c = tid & (((size_t)1 << logS) - 1);
+    const size_t base = (tid >> logS) << logM;
+    Fr x[R];
+#pragma unroll
+    for (int b = 0; b < R; b++) x[bitrev_c(b, LR)] = load<FrCfg>(data, base + ((size_t)b << logS) + c);
+    if (pre) {
+#pragma unroll
+        for (int b = 0; b < R; b++)
+            x[bitrev_c(b, LR)] = mul<FrCfg>(x[bitrev_c(b, LR)], load<FrCfg>(pre, base + ((size_t)b << logS) + c));
+    }
+    dft_reg<LR>(x, rt);
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        Fr y = x[k];
         if (k && c) y = mul<FrCfg>(y, load<FrCfg>(tw, ((size_t)k << logS) + c));
         store<FrCfg>(data, base + ((size_t)k << logS) + c, y);
     }
 }
 
-// Last pass: rows of R contiguous points; the workgroup takes T rows with
-// consecutive digit-reversed index and writes out[rev(row) + O*k].
-__global__ void __launch_bounds__(NTT_THREADS) k_ntt_final(NttPtrs ptrs, NttShape sh, int log_n, const void* __restrict__ small,
+// Last pass: thread = output column rv (digit-reversed row index); the row of
+// R contiguous points at unrev(rv)*R gives out[rv + O*k], k < R.
+template <int LR>
+__global__ void __launch_bounds__(NTT_THREADS) k_ntt_final(NttPtrs ptrs, NttDigits dg, size_t n, RootsR rt,
                                                            const void* __restrict__ pre, const void* __restrict__ post) {
-    __shared__ LdsTile L;
+    constexpr int R = 1 << LR;
+    const size_t O = n >> LR;
+    size_t rv = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (rv >= O) return;
     const void* in = ptrs.in[blockIdx.y];
     void* out = ptrs.out[blockIdx.y];
-    const int lr = sh.lr[sh.np - 1];
-    const int R = 1 << lr;
-    const int logO = log_n - lr;
-    const size_t O = (size_t)1 << logO;
-    const int T = (int)min((size_t)(NTT_TILE >> lr), O);
-    const size_t rev0 = (size_t)blockIdx.x * T;
-    const int nel = T * R;
-    for (int q = threadIdx.x; q < nel; q += NTT_THREADS) {
-        int ri = q >> lr, b = q & (R - 1);
-        // un-reverse: rev = d0 + R0*d1 + ... ; row = ((d0*R1 + d1)*R2 + ...)
-        size_t rv = rev0 + ri, row = 0;
-        for (int d = 0; d < sh.np - 1; d++) {
-            size_t digit = rv & (((size_t)1 << sh.lr[d]) - 1);
-            rv >>= sh.lr[d];
-            row = (row << sh.lr[d]) | digit;
-        }
-        size_t g = (row << lr) + b;
-        Fr x = load<FrCfg>(in, g);
-        if (pre && sh.np == 1) x = mul<FrCfg>(x, load<FrCfg>(pre, g));
-        L.st(ri * R + (int)bitrev((uint32_t)b, lr), x);
+    size_t r = rv, row = 0;
+    for (int d = 0; d < dg.np - 1; d++) {
+        size_t digit = r & (((size_t)1 << dg.lr[d]) - 1);
+        r >>= dg.lr[d];
+        row = (row << dg.lr[d]) | digit;
     }
-    lds_dft(L, lr, T, small);
-    for (int q = threadIdx.x; q < nel; q += NTT_THREADS) {
-        int k = q / T, ri = q - k * T;
-        size_t o = rev0 + ri + (size_t)k * O;
-        Fr y = L.ld(ri * R + k);
+    Fr x[R];
+#pragma unroll
+    for (int b = 0; b < R; b++) x[bitrev_c(b, LR)] = load<FrCfg>(in, (row << LR) + b);
+    if (pre && dg.np == 1) {
+#pragma unroll
+        for (int b = 0; b < R; b++)
+            x[bitrev_c(b, LR)] = mul<FrCfg>(x[bitrev_c(b, LR)], load<FrCfg>(pre, (row << LR) + b));
+    }
+    dft_reg<LR>(x, rt);
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        size_t o = rv + (size_t)k * O;
+        Fr y = x[k];
         if (post) y = mul<FrCfg>(y, load<FrCfg>(post, o));
         store<FrCfg>(out, o, y);
     }
 }
 
 // table[i] = lo[e & 4095] * hi[e >> 12] for e = exponent(i)
-//   mode 0: e = i * step (mod n_mod)
+//   mode 0: e = i * step (mod 2n)
 //   mode 1: pass twiddle, i = k*S + c -> e = c*k*step
 __global__ void k_pow_table(void* __restrict__ out, size_t count, const void* __restrict__ lo, const void* __restrict__ hi,
                             int mode, uint64_t step, int logS, uint64_t mask, const void* __restrict__ scale) {
@@ -179,11 +173,18 @@ void NttPlan::build(int ln, hipStream_t st) {
     ZK_REQUIRE(ln >= 1 && ln <= 27, ZK_E_ARG, "NTT size must be 2^1..2^27");
     log_n = ln;
     n = (size_t)1 << ln;
-    np = (ln + 7) / 8;
-    for (int j = 0; j < 4; j++) lr[j] = 0;
-    for (int j = 0; j < np; j++) lr[j] = ln / np + (j < ln % np ? 1 : 0);
+    // radix-8 passes; a radix-16 pass absorbs one leftover bit, radix-4/2 the rest
+    np = 0;
+    int rem = ln;
+    while (rem > 0) {
+        int take = rem >= 3 ? 3 : rem;
+        lr[np++] = take;
+        rem -= take;
+    }
+    if (np >= 2 && lr[np - 1] == 1) { lr[np - 2] += 1; np--; }  // ...3,1 -> ...4
+    for (int j = np; j < NTT_MAX_PASSES; j++) lr[j] = 0;
 
-    // base power tables for w_n, w_n^-1, w_2n, w_2n^-1 (the 2n-th root covers all)
+    // base power tables for w_2n and w_2n^-1 (cover every w_M, w_R and the coset factors)
     zkh::Fr w2n = root_of_unity(ln + 1), w2n_inv = w2n.inv();
     std::vector<zkh::Fr> lo_f, hi_f, lo_i, hi_i;
     host_pow_tables(w2n, ln + 1, lo_f, hi_f);
@@ -197,39 +198,36 @@ void NttPlan::build(int ln, hipStream_t st) {
     ZK_HIP(hipMemcpyAsync(d_scale.p, &ninv, 32, hipMemcpyHostToDevice, st));
     const uint64_t mask2n = (2 * (uint64_t)n) - 1;
 
-    // pass tables: w_M^(c k) with w_M = w_2n^(2n/M)
-    size_t tot = 0, stot = 0;
+    // radix roots w_R^i = w_2n^(i * 2n / R), forward and inverse, on the host
+    for (int j = 0; j < np; j++) {
+        int R = 1 << lr[j];
+        for (int i = 0; i < 8; i++) {
+            uint64_t e = i < R / 2 ? ((uint64_t)i * ((2 * (uint64_t)n) >> lr[j])) & mask2n : 0;
+            zkh::Fr f = lo_f[e & 4095] * hi_f[e >> 12], v = lo_i[e & 4095] * hi_i[e >> 12];
+            memcpy(roots_fwd[j] + 8 * i, f.v, 32);
+            memcpy(roots_inv[j] + 8 * i, v.v, 32);
+        }
+    }
+    // pass twiddle tables w_M^(c k), layout [k][c], for passes 0..np-2
+    size_t tot = 0;
     int consumed = 0;
     for (int j = 0; j < np; j++) {
-        int logM = ln - consumed;
         tw_off[j] = tot;
-        small_off[j] = stot;
-        tot += (j < np - 1) ? ((size_t)1 << logM) : 0;
-        stot += (size_t)1 << (lr[j] - 1);
+        if (j < np - 1) tot += (size_t)1 << (ln - consumed);
         consumed += lr[j];
     }
     tw_fwd.ensure(std::max<size_t>(tot, 1) * 32);
     tw_inv.ensure(std::max<size_t>(tot, 1) * 32);
-    small_fwd.ensure(stot * 32);
-    small_inv.ensure(stot * 32);
     consumed = 0;
-    for (int j = 0; j < np; j++) {
+    for (int j = 0; j < np - 1; j++) {
         int logM = ln - consumed;
         int logS = logM - lr[j];
         uint64_t step_M = (uint64_t)1 << (ln + 1 - logM);  // w_M = w_2n^(2n/M)
-        if (j < np - 1) {
-            size_t cnt = (size_t)1 << logM;
-            k_pow_table<<<cdiv(cnt, 256), 256, 0, st>>>((char*)tw_fwd.p + tw_off[j] * 32, cnt, d_lo_f.p, d_hi_f.p, 1, step_M,
-                                                       logS, mask2n, nullptr);
-            k_pow_table<<<cdiv(cnt, 256), 256, 0, st>>>((char*)tw_inv.p + tw_off[j] * 32, cnt, d_lo_i.p, d_hi_i.p, 1, step_M,
-                                                       logS, mask2n, nullptr);
-        }
-        size_t sc = (size_t)1 << (lr[j] - 1);
-        uint64_t step_R = (uint64_t)1 << (ln + 1 - lr[j]);  // w_R = w_2n^(2n/R)
-        k_pow_table<<<cdiv(sc, 256), 256, 0, st>>>((char*)small_fwd.p + small_off[j] * 32, sc, d_lo_f.p, d_hi_f.p, 0, step_R, 0,
-                                                  mask2n, nullptr);
-        k_pow_table<<<cdiv(sc, 256), 256, 0, st>>>((char*)small_inv.p + small_off[j] * 32, sc, d_lo_i.p, d_hi_i.p, 0, step_R, 0,
-                                                  mask2n, nullptr);
+        size_t cnt = (size_t)1 << logM;
+        k_pow_table<<<cdiv(cnt, 256), 256, 0, st>>>((char*)tw_fwd.p + tw_off[j] * 32, cnt, d_lo_f.p, d_hi_f.p, 1, step_M,
+                                                   logS, mask2n, nullptr);
+        k_pow_table<<<cdiv(cnt, 256), 256, 0, st>>>((char*)tw_inv.p + tw_off[j] * 32, cnt, d_lo_i.p, d_hi_i.p, 1, step_M,
+                                                   logS, mask2n, nullptr);
         consumed += lr[j];
     }
     // element-wise tables
@@ -245,6 +243,20 @@ void NttPlan::build(int ln, hipStream_t st) {
     ZK_HIP(hipStreamSynchronize(st));  // the temporaries above are freed on return
 }
 
+template <int LR>
+static void launch_pass(const NttPtrs& ptrs, int batch, int logM, size_t n, const RootsR& rt, const void* tw,
+                        const void* pre, hipStream_t st) {
+    unsigned grid = cdiv(n >> LR, NTT_THREADS);
+    k_ntt_pass<LR><<<dim3(grid, batch), NTT_THREADS, 0, st>>>(ptrs, logM, n, rt, tw, pre);
+}
+
+template <int LR>
+static void launch_final(const NttPtrs& ptrs, int batch, const NttDigits& dg, size_t n, const RootsR& rt, const void* pre,
+                         const void* post, hipStream_t st) {
+    unsigned grid = cdiv(n >> LR, NTT_THREADS);
+    k_ntt_final<LR><<<dim3(grid, batch), NTT_THREADS, 0, st>>>(ptrs, dg, n, rt, pre, post);
+}
+
 void ntt_run(const NttPlan& p, void* const* in, void* const* out, int batch, bool inverse, const void* pre, const void* post,
              hipStream_t st) {
     ZK_REQUIRE(batch >= 1 && batch <= NTT_MAX_BATCH, ZK_E_ARG, "NTT batch must be 1..4");
@@ -254,30 +266,36 @@ void ntt_run(const NttPlan& p, void* const* in, void* const* out, int batch, boo
         ptrs.out[b] = b < batch ? out[b] : nullptr;
         if (b < batch && p.np > 1) ZK_REQUIRE(in[b] != out[b], ZK_E_ARG, "multi-pass NTT needs distinct in/out");
     }
-    NttShape sh;
-    sh.np = p.np;
-    for (int j = 0; j < 4; j++) sh.lr[j] = p.lr[j];
+    NttDigits dg;
+    dg.np = p.np;
+    for (int j = 0; j < NTT_MAX_PASSES; j++) dg.lr[j] = p.lr[j];
     const char* tw = (const char*)(inverse ? p.tw_inv.p : p.tw_fwd.p);
-    const char* sm = (const char*)(inverse ? p.small_inv.p : p.small_fwd.p);
     int consumed = 0;
-    for (int j = 0; j < p.np - 1; j++) {
-        sh.j = j;
-        int logM = p.log_n - consumed;
-        size_t S = (size_t)1 << (logM - p.lr[j]);
-        size_t T = std::min<size_t>((size_t)NTT_TILE >> p.lr[j], S);
-        unsigned grid = (unsigned)(p.n / (((size_t)1 << p.lr[j]) * T));
-        consumed += p.lr[j];
-        k_ntt_pass<<<dim3(grid, batch), NTT_THREADS, 0, st>>>(ptrs, sh, p.log_n, tw + p.tw_off[j] * 32,
-                                                              sm + p.small_off[j] * 32, j == 0 ? pre : nullptr);
+    for (int j = 0; j < p.np; j++) {
+        RootsR rt;
+        memcpy(&rt, inverse ? p.roots_inv[j] : p.roots_fwd[j], sizeof rt);
+        const int lrj = p.lr[j];
+        if (j < p.np - 1) {
+            const int logM = p.log_n - consumed;
+            const void* pj = j == 0 ? pre : nullptr;
+            const void* twj = tw + p.tw_off[j] * 32;
+            switch (lrj) {
+                case 1: launch_pass<1>(ptrs, batch, logM, p.n, rt, twj, pj, st); break;
+                case 2: launch_pass<2>(ptrs, batch, logM, p.n, rt, twj, pj, st); break;
+                case 3: launch_pass<3>(ptrs, batch, logM, p.n, rt, twj, pj, st); break;
+                default: launch_pass<4>(ptrs, batch, logM, p.n, rt, twj, pj, st); break;
+            }
+        } else {
+            switch (lrj) {
+                case 1: launch_final<1>(ptrs, batch, dg, p.n, rt, pre, post, st); break;
+                case 2: launch_final<2>(ptrs, batch, dg, p.n, rt, pre, post, st); break;
+                case 3: launch_final<3>(ptrs, batch, dg, p.n, rt, pre, post, st); break;
+                default: launch_final<4>(ptrs, batch, dg, p.n, rt, pre, post, st); break;
+            }
+        }
         ZK_HIP(hipGetLastError());
+        consumed += lrj;
     }
-    sh.j = p.np - 1;
-    int lr_last = p.lr[p.np - 1];
-    size_t O = p.n >> lr_last;
-    size_t T = std::min<size_t>((size_t)NTT_TILE >> lr_last, O);
-    unsigned grid = (unsigned)(O / T);
-    k_ntt_final<<<dim3(grid, batch), NTT_THREADS, 0, st>>>(ptrs, sh, p.log_n, sm + p.small_off[p.np - 1] * 32, pre, post);
-    ZK_HIP(hipGetLastError());
 }
 
 }  // namespace zk

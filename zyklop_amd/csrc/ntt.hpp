@@ -1,15 +1,18 @@
 // Fr NTT on gfx950: natural order in and out, w_n = 5^((r-1)/n) (the
 // reference's convention, include/ntt_bn254.cuh:190-261,300-464).
 //
-// Design: four-step recursion n = R_0 * R_1 * ... * R_{p-1} with every R_j <=
-// 256. Each pass is one kernel over LDS tiles of 2048 elements (T columns of
-// R_j points) doing a radix-2 DIT DFT in LDS, then the inter-pass twiddle
-// w_{M_j}^{c k} from a precomputed [k][c] table (coalesced). Passes 0..p-2
-// run in place; the last pass writes the digit-reversed result out of place
-// with consecutive lanes on consecutive output indices. An optional
-// element-wise table multiplies the input (first pass) or the output (last
-// pass): that is how the coset shift and n^-1 are fused instead of running
-// separate element-wise passes (src/groth16.cu:914-921,1066-1076).
+// Design: four-step recursion n = R_0 * R_1 * ... * R_{p-1} with R_j = 8
+// (one radix-16 or radix-4 pass absorbs the leftover bits). Each pass is one
+// kernel in which every thread owns one column of R_j points: it loads them
+// (consecutive lanes on consecutive columns, so every load is coalesced),
+// runs the radix-R_j DFT in registers, multiplies by the inter-pass twiddle
+// w_{M_j}^{c k} from a precomputed [k][c] table (also coalesced) and stores.
+// Passes 0..p-2 run in place; the last pass writes the digit-reversed
+// (natural) order out of place with lanes on consecutive output indices. No
+// LDS and no barriers: a pass is one coalesced read and write of the array.
+// An optional element-wise table multiplies the input (first pass) or the
+// output (last pass): that is how the coset shift and n^-1 are fused instead
+// of separate element-wise passes (src/groth16.cu:914-921,1066-1076).
 #pragma once
 #include <stdint.h>
 
@@ -17,15 +20,19 @@
 
 namespace zk {
 
+constexpr int NTT_MAX_PASSES = 10;
+
 struct NttPlan {
     int log_n = -1;
     int np = 0;
-    int lr[4] = {0, 0, 0, 0};
+    int lr[NTT_MAX_PASSES] = {};
     size_t n = 0;
     // per-pass twiddle tables (forward / inverse), concatenated; offsets per pass
-    DevBuf tw_fwd, tw_inv, small_fwd, small_inv;
-    size_t tw_off[4] = {0, 0, 0, 0};
-    size_t small_off[4] = {0, 0, 0, 0};
+    DevBuf tw_fwd, tw_inv;
+    size_t tw_off[NTT_MAX_PASSES] = {};
+    // radix roots w_R^i (i < R/2, Montgomery) per pass, forward and inverse
+    uint64_t roots_fwd[NTT_MAX_PASSES][32] = {};
+    uint64_t roots_inv[NTT_MAX_PASSES][32] = {};
     // element-wise tables (Montgomery Fr, n entries each)
     DevBuf coset_fwd;   // w_2n^i           (coset forward prologue)
     DevBuf inv_coset;   // n^-1 * w_2n^i    (iNTT then coset shift: H pipeline)

@@ -2,6 +2,7 @@
 
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 
@@ -225,8 +226,7 @@ void circuit_write_zkey(const Circuit& c, const std::string& path) {
 
 // ------------------------------------------------------------------ proving
 ProveSlot::ProveSlot() {
-    ZK_HIP(hipStreamCreateWithFlags(&s0, hipStreamNonBlocking));
-    ZK_HIP(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
+    for (hipStream_t* p : {&s0, &s1, &s2, &s3, &s4}) ZK_HIP(hipStreamCreateWithFlags(p, hipStreamNonBlocking));
     ZK_HIP(hipEventCreateWithFlags(&ev_scal, hipEventDisableTiming));
     ZK_HIP(hipEventCreateWithFlags(&ev_done1, hipEventDisableTiming));
 }
@@ -241,8 +241,8 @@ void ProveSlot::enable_timing() {
 ProveSlot::~ProveSlot() {
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
-    if (s0) { (void)hipStreamSynchronize(s0); (void)hipStreamDestroy(s0); }
-    if (s1) { (void)hipStreamSynchronize(s1); (void)hipStreamDestroy(s1); }
+    for (hipStream_t p : {s0, s1, s2, s3, s4})
+        if (p) { (void)hipStreamSynchronize(p); (void)hipStreamDestroy(p); }
     if (ev_scal) (void)hipEventDestroy(ev_scal);
     if (ev_done1) (void)hipEventDestroy(ev_done1);
 }
@@ -296,11 +296,15 @@ void ProveSlot::enqueue(Context& ctx, const Circuit& c, const uint64_t* witness,
     ZK_HIP(hipStreamWaitEvent(s1, ev_scal, 0));
     if (timed) ZK_HIP(hipEventRecord(ev[4], s1));
     sortW.run(scal.p, nS, window_w, s1);
+    ZK_HIP(hipEventRecord(ev_done1, s1));
     if (timed) ZK_HIP(hipEventRecord(ev[5], s1));
+    // the four witness MSMs read the same sorted entries on their own streams:
+    // one MSM's latency-bound reduction overlaps another's accumulation
+    for (hipStream_t p : {s2, s3, s4}) ZK_HIP(hipStreamWaitEvent(p, ev_done1, 0));
+    msm_launch(sortW, c.B2.p, true, scrB2, pB2, s4);
     msm_launch(sortW, c.A.p, false, scrA, pA, s1);
-    msm_launch(sortW, c.B1.p, false, scrB1, pB1, s1);
-    msm_launch(sortW, c.C.p, false, scrC, pC, s1);
-    msm_launch(sortW, c.B2.p, true, scrB2, pB2, s1);
+    msm_launch(sortW, c.B1.p, false, scrB1, pB1, s2);
+    msm_launch(sortW, c.C.p, false, scrC, pC, s3);
 }
 
 template <class T>
@@ -314,7 +318,7 @@ static void put_aff(const zkh::Xyzz<T>& p, uint64_t* dst) {
 
 void ProveSlot::finish(zk_proof* out) {
     using clk = std::chrono::steady_clock;
-    ZK_HIP(hipStreamSynchronize(s1));
+    for (hipStream_t p : {s1, s2, s3, s4}) ZK_HIP(hipStreamSynchronize(p));
     auto h0 = clk::now();
     auto A = msm_finish_g1(pA);
     auto B1 = msm_finish_g1(pB1);
@@ -365,7 +369,8 @@ void ProveSlot::collect_stats() {
     stats[19] = sortW.g.W;
     stats[20] = sortH.g.W;
     stats[21] = el(ev[0], scrH.ev_end);   // stream 0 span
-    stats[22] = el(ev[4], scrB2.ev_end);  // stream 1 span
+    stats[22] = 0;                         // witness-MSM span (latest of the four streams)
+    for (MsmScratch* q : {&scrA, &scrB1, &scrC, &scrB2}) stats[22] = std::max(stats[22], el(ev[4], q->ev_end));
 }
 
 // ------------------------------------------------------------------ serialization

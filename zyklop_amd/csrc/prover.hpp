@@ -48,7 +48,8 @@ struct Circuit {
 };
 
 struct ProveSlot {
-    hipStream_t s0 = nullptr, s1 = nullptr;
+    // s0: H pipeline + H MSM; s1: witness sort + A; s2: B1; s3: C; s4: B2
+    hipStream_t s0 = nullptr, s1 = nullptr, s2 = nullptr, s3 = nullptr, s4 = nullptr;
     hipEvent_t ev_scal = nullptr, ev_done1 = nullptr;
     DevBuf scal, aw, bw, cw, t0, t1, t2, h, wtmp;
     MsmSort sortW, sortH;
