@@ -35,6 +35,7 @@ extern "C" {
 
 typedef struct zk_ctx zk_ctx;
 typedef struct zk_circuit zk_circuit;
+typedef struct zk_server zk_server;
 
 /* Same layout as Groth16Proof (include/groth16.cuh:69-73). */
 typedef struct zk_proof {
@@ -109,6 +110,27 @@ int zk_set_batch_slots(zk_ctx* ctx, int n);
  *  [21] stream-0 span ms  [22] stream-1 span ms  [23] host tail ms */
 int zk_set_profiling(zk_ctx* ctx, int on);
 int zk_prove_stats(zk_ctx* ctx, double* out, int n);
+
+/* ---- serving (no reference counterpart: the reference proves one file pair
+ * per process, src/main.cu:115-171) ----
+ * A worker thread admits submitted requests into workspace slots while other
+ * proofs run (continuous batching), creating or growing a slot only if free
+ * HBM stays above reserve_bytes. Circuits must outlive their requests.
+ * zk_server_destroy finishes every submitted request first. */
+int zk_server_create(zk_ctx* ctx, int max_slots, uint64_t reserve_bytes, zk_server** out);
+void zk_server_destroy(zk_server* s);
+/* witness: host, canonical, copied at submit; r, s NULL = 0 */
+int zk_server_submit(zk_server* s, const zk_circuit* c, const uint64_t* witness, uint32_t n_witness, const uint64_t r[4],
+                     const uint64_t sb[4], uint64_t* ticket);
+/* *done = 1 when finished (then the ticket is released); times (ms):
+ * [0] queued, [1] on the device + host tail, [2] total latency. A failed
+ * request returns its status here without affecting other requests. */
+int zk_server_poll(zk_server* s, uint64_t ticket, int* done, zk_proof* out, double times[3]);
+int zk_server_wait(zk_server* s, uint64_t ticket, zk_proof* out, double times[3]);
+/* removes a request that has not started; *cancelled = 1 if it was removed */
+int zk_server_cancel(zk_server* s, uint64_t ticket, int* cancelled);
+/* [0] completed [1] failed [2] rejected [3] cancelled [4] queued [5] in flight [6] slots [7] peak in flight */
+int zk_server_stats(zk_server* s, double out[8]);
 
 /* ---- serialization (groth16ProofToSnarkjsJson / groth16ProofToCalldata,
  * include/groth16.cuh:149-167; public.json: src/main.cu:28-110) ---- */

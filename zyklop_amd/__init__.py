@@ -69,6 +69,14 @@ ABI = {
     "zk_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
     "zk_set_batch_slots": (ctypes.c_int, [_vp, ctypes.c_int]),
     "zk_prove_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
+    "zk_server_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    "zk_server_destroy": (None, [_vp]),
+    "zk_server_submit": (ctypes.c_int, [_vp, _vp, _u64p, ctypes.c_uint32, _u64p, _u64p, ctypes.POINTER(ctypes.c_uint64)]),
+    "zk_server_poll": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(Proof),
+                                      ctypes.POINTER(ctypes.c_double)]),
+    "zk_server_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(Proof), ctypes.POINTER(ctypes.c_double)]),
+    "zk_server_cancel": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
+    "zk_server_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
     "zk_proof_to_json": (ctypes.c_int, [ctypes.POINTER(Proof), ctypes.c_char_p, ctypes.c_size_t]),
     "zk_proof_to_calldata": (ctypes.c_int, [ctypes.POINTER(Proof), _u8p, ctypes.c_size_t]),
     "zk_public_json": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t]),
@@ -315,6 +323,58 @@ class Context:
     def prove_stats(self):
         buf = (ctypes.c_double * 24)()
         _check(lib().zk_prove_stats(self.h, buf, 24))
+        return dict(zip(self.STAT_NAMES, [float(x) for x in buf]))
+
+
+class Server:
+    """Continuous-batching proof server (zk_server_*) on a context."""
+
+    STAT_NAMES = ["completed", "failed", "rejected", "cancelled", "queued", "in_flight", "slots", "peak_in_flight"]
+
+    def __init__(self, ctx, max_slots=3, reserve_bytes=8 << 30):
+        h = _vp()
+        _check(lib().zk_server_create(ctx.h, max_slots, reserve_bytes, ctypes.byref(h)))
+        self.h = h
+        self.ctx = ctx
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().zk_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, circuit, witness, r=None, s=None):
+        w = np.ascontiguousarray(witness, dtype=np.uint64)
+        r4, s4 = _opt4(r), _opt4(s)
+        t = ctypes.c_uint64(0)
+        _check(lib().zk_server_submit(self.h, circuit.h, u64p(w), w.size // 4, u64p(r4) if r4 is not None else None,
+                                      u64p(s4) if s4 is not None else None, ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket):
+        """None while pending, else (Proof, (queued_ms, service_ms, total_ms))."""
+        done = ctypes.c_int(0)
+        pr = Proof()
+        tm = (ctypes.c_double * 3)()
+        _check(lib().zk_server_poll(self.h, ticket, ctypes.byref(done), ctypes.byref(pr), tm))
+        return (pr, tuple(tm)) if done.value else None
+
+    def wait(self, ticket):
+        pr = Proof()
+        tm = (ctypes.c_double * 3)()
+        _check(lib().zk_server_wait(self.h, ticket, ctypes.byref(pr), tm))
+        return pr, tuple(tm)
+
+    def cancel(self, ticket):
+        c = ctypes.c_int(0)
+        _check(lib().zk_server_cancel(self.h, ticket, ctypes.byref(c)))
+        return bool(c.value)
+
+    def stats(self):
+        buf = (ctypes.c_double * 8)()
+        _check(lib().zk_server_stats(self.h, buf))
         return dict(zip(self.STAT_NAMES, [float(x) for x in buf]))
 
 

@@ -9,6 +9,7 @@
 #include "ntt.hpp"
 #include "prover.hpp"
 #include "qap.hpp"
+#include "server.hpp"
 #include "synth.hpp"
 #include "zkey.hpp"
 
@@ -34,6 +35,10 @@ struct zk_ctx {
 
 struct zk_circuit {
     std::unique_ptr<zk::Circuit> impl;
+};
+
+struct zk_server {
+    std::unique_ptr<zk::Server> impl;
 };
 
 namespace {
@@ -303,6 +308,69 @@ int zk_prove_stats(zk_ctx* ctx, double* out, int n) {
         zk::ProveSlot& sl = ctx_slot(ctx);
         need(sl.timed, "profiling is off (zk_set_profiling)");
         for (int i = 0; i < n && i < zk::ProveSlot::N_STATS; i++) out[i] = sl.stats[i];
+    });
+}
+
+int zk_server_create(zk_ctx* ctx, int max_slots, uint64_t reserve_bytes, zk_server** out) {
+    return guard([&] {
+        need(ctx && out, "NULL argument");
+        auto s = new zk_server;
+        try {
+            s->impl = std::make_unique<zk::Server>(ctx->impl, max_slots, (size_t)reserve_bytes);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+void zk_server_destroy(zk_server* s) { delete s; }
+
+int zk_server_submit(zk_server* s, const zk_circuit* c, const uint64_t* witness, uint32_t n_witness, const uint64_t r[4],
+                     const uint64_t sb[4], uint64_t* ticket) {
+    return guard([&] {
+        need(s && c && witness && ticket, "NULL argument");
+        *ticket = s->impl->submit(*c->impl, witness, n_witness, r, sb);
+    });
+}
+
+static int server_result(int status, const std::string& err) {
+    if (status != ZK_OK) throw zk::Error(status, err);
+    return ZK_OK;
+}
+
+int zk_server_poll(zk_server* s, uint64_t ticket, int* done, zk_proof* out, double times[3]) {
+    return guard([&] {
+        need(s && done, "NULL argument");
+        int st = ZK_OK;
+        std::string err;
+        *done = s->impl->poll(ticket, out, times, &st, &err) ? 1 : 0;
+        if (*done) server_result(st, err);
+    });
+}
+
+int zk_server_wait(zk_server* s, uint64_t ticket, zk_proof* out, double times[3]) {
+    return guard([&] {
+        need(s != nullptr, "NULL server");
+        int st = ZK_OK;
+        std::string err;
+        s->impl->wait(ticket, out, times, &st, &err);
+        server_result(st, err);
+    });
+}
+
+int zk_server_cancel(zk_server* s, uint64_t ticket, int* cancelled) {
+    return guard([&] {
+        need(s && cancelled, "NULL argument");
+        *cancelled = s->impl->cancel(ticket) ? 1 : 0;
+    });
+}
+
+int zk_server_stats(zk_server* s, double out[8]) {
+    return guard([&] {
+        need(s && out, "NULL argument");
+        s->impl->stats(out);
     });
 }
 

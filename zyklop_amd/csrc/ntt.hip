@@ -204,8 +204,8 @@ void NttPlan::build(int ln, hipStream_t st) {
         for (int i = 0; i < 8; i++) {
             uint64_t e = i < R / 2 ? ((uint64_t)i * ((2 * (uint64_t)n) >> lr[j])) & mask2n : 0;
             zkh::Fr f = lo_f[e & 4095] * hi_f[e >> 12], v = lo_i[e & 4095] * hi_i[e >> 12];
-            memcpy(roots_fwd[j] + 8 * i, f.v, 32);
-            memcpy(roots_inv[j] + 8 * i, v.v, 32);
+            memcpy(roots_fwd[j] + 4 * i, f.v, 32);  // 4 words per element
+            memcpy(roots_inv[j] + 4 * i, v.v, 32);
         }
     }
     // pass twiddle tables w_M^(c k), layout [k][c], for passes 0..np-2

@@ -270,8 +270,15 @@ void ProveSlot::enqueue(Context& ctx, const Circuit& c, const uint64_t* witness,
     for (DevBuf* b : {&aw, &bw, &cw, &t0, &t1, &t2, &h}) b->ensure(n * 32);
     if (timed) ZK_HIP(hipEventRecord(ev[0], s0));
     // scalars: witness, then [1, r, s, -r*s]
-    if (on_dev) ZK_HIP(hipMemcpyAsync(scal.p, witness, nv * 32, hipMemcpyDeviceToDevice, s0));
-    else ZK_HIP(hipMemcpyAsync(scal.p, witness, nv * 32, hipMemcpyHostToDevice, s0));
+    if (on_dev) {
+        ZK_HIP(hipMemcpyAsync(scal.p, witness, nv * 32, hipMemcpyDeviceToDevice, s0));
+    } else {
+        // pinned staging makes the upload asynchronous; the previous proof of
+        // this slot has finished (finish() ran), so the staging buffer is free
+        staging.ensure(nv * 32);
+        memcpy(staging.p, witness, nv * 32);
+        ZK_HIP(hipMemcpyAsync(scal.p, staging.p, nv * 32, hipMemcpyHostToDevice, s0));
+    }
     uint64_t ex[16] = {1, 0, 0, 0};
     memcpy(ex + 4, r, 32);
     memcpy(ex + 8, s, 32);
@@ -305,6 +312,28 @@ void ProveSlot::enqueue(Context& ctx, const Circuit& c, const uint64_t* witness,
     msm_launch(sortW, c.A.p, false, scrA, pA, s1);
     msm_launch(sortW, c.B1.p, false, scrB1, pB1, s2);
     msm_launch(sortW, c.C.p, false, scrC, pC, s3);
+}
+
+bool ProveSlot::device_done() const {
+    for (hipStream_t p : {s0, s1, s2, s3, s4}) {
+        hipError_t e = hipStreamQuery(p);
+        if (e == hipErrorNotReady) return false;
+        if (e != hipSuccess) ZK_HIP(e);
+    }
+    return true;
+}
+
+size_t ProveSlot::bytes_needed(const Circuit& c) {
+    const size_t nS = (size_t)c.n_vars + 4, n = c.n;
+    MsmGeom gw = msm_geom(nS, 0, false), gh = msm_geom(n, 0, false);
+    size_t b = nS * 32 + 7 * n * 32;                                    // scalars + Fr arrays
+    b += (size_t)gw.W * nS * 4 + (size_t)gh.W * n * 4;                  // sorted entries
+    b += 3 * 4 * ((size_t)gw.W * gw.K + (size_t)gh.W * gh.K);           // counts / offsets
+    size_t bw = (size_t)gw.W * gw.K, bh = (size_t)gh.W * gh.K;
+    b += 3 * bw * 128 + bw * 256 + bh * 128;                            // buckets A, B1, C, B2, H
+    b += (3 * bw + bw * 2 + bh) * 128 / 4;                              // reduction partials (upper bound)
+    b += ((size_t)gw.W * nS / 64) * (16 + 128) * 4 + ((size_t)gh.W * n / 64) * (16 + 128);  // large-bucket lists
+    return b;
 }
 
 template <class T>

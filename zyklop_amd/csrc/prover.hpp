@@ -52,6 +52,7 @@ struct ProveSlot {
     hipStream_t s0 = nullptr, s1 = nullptr, s2 = nullptr, s3 = nullptr, s4 = nullptr;
     hipEvent_t ev_scal = nullptr, ev_done1 = nullptr;
     DevBuf scal, aw, bw, cw, t0, t1, t2, h, wtmp;
+    HostBuf staging;                 // pinned copy of a host witness (async upload)
     MsmSort sortW, sortH;
     MsmScratch scrA, scrB1, scrC, scrB2, scrH;
     MsmPartials pA, pB1, pC, pB2, pH;
@@ -73,6 +74,10 @@ struct ProveSlot {
                  const uint64_t s[4]);
     // wait for the device work and run the host tail
     void finish(zk_proof* out);
+    // true when every stream of the slot has drained (non-blocking)
+    bool device_done() const;
+    // device bytes a proof of c needs in a slot (workspace estimate for admission)
+    static size_t bytes_needed(const Circuit& c);
     // number of per-proof statistics in stats[] (layout in zk_abi.h: zk_prove_stats)
     static constexpr int N_STATS = 24;
     size_t device_bytes() const;

@@ -1,0 +1,238 @@
+#include "server.hpp"
+
+#include <string.h>
+
+#include <algorithm>
+
+namespace zk {
+
+using clk = std::chrono::steady_clock;
+
+static double ms_between(clk::time_point a, clk::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+Server::Server(Context& ctx, int max_slots, size_t reserve_bytes)
+    : ctx_(ctx), max_slots_(max_slots), reserve_(reserve_bytes) {
+    ZK_REQUIRE(max_slots >= 1 && max_slots <= 16, ZK_E_ARG, "max_slots must be 1..16");
+    worker_ = std::thread([this] { loop(); });
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_work_.notify_all();
+    if (worker_.joinable()) worker_.join();
+}
+
+uint64_t Server::submit(const Circuit& c, const uint64_t* w, uint32_t nw, const uint64_t* r, const uint64_t* s) {
+    ZK_REQUIRE(w != nullptr, ZK_E_ARG, "NULL witness");
+    ZK_REQUIRE(nw >= c.n_vars, ZK_E_ARG, "witness shorter than nVars");
+    for (uint32_t i = 0; i < c.n_vars; i++)
+        ZK_REQUIRE(zkh::Fr::is_canonical(w + 4ull * i), ZK_E_ARG, "witness value " + std::to_string(i) + " is not canonical");
+    auto req = std::make_shared<ServerRequest>();
+    if (r) memcpy(req->r, r, 32);
+    if (s) memcpy(req->s, s, 32);
+    ZK_REQUIRE(zkh::Fr::is_canonical(req->r) && zkh::Fr::is_canonical(req->s), ZK_E_ARG, "blinding must be canonical");
+    size_t free_b = 0, total_b = 0;
+    ZK_HIP(hipMemGetInfo(&free_b, &total_b));
+    if (ProveSlot::bytes_needed(c) + reserve_ > total_b) {
+        std::lock_guard<std::mutex> lk(mu_);
+        n_rejected_++;
+        throw Error(ZK_E_NOMEM, "request needs more device memory than the GPU has");
+    }
+    req->circ = &c;
+    req->witness.assign(w, w + 4ull * c.n_vars);
+    req->t_submit = clk::now();
+    uint64_t t;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        ZK_REQUIRE(!stop_, ZK_E_STATE, "server is stopping");
+        t = req->ticket = next_ticket_++;
+        reqs_[t] = req;
+        queue_.push_back(req);
+    }
+    cv_work_.notify_one();
+    return t;
+}
+
+static void copy_out(const ServerRequest& q, zk_proof* out, double times[3], int* status, std::string* err) {
+    if (out) *out = q.proof;
+    if (times) {
+        times[0] = q.t_start.time_since_epoch().count() ? ms_between(q.t_submit, q.t_start) : 0;
+        times[1] = q.t_start.time_since_epoch().count() ? ms_between(q.t_start, q.t_done) : 0;
+        times[2] = ms_between(q.t_submit, q.t_done);
+    }
+    if (status) *status = q.status;
+    if (err) *err = q.error;
+}
+
+bool Server::poll(uint64_t ticket, zk_proof* out, double times[3], int* status, std::string* err) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = reqs_.find(ticket);
+    ZK_REQUIRE(it != reqs_.end(), ZK_E_ARG, "unknown ticket " + std::to_string(ticket));
+    if (!it->second->done) return false;
+    copy_out(*it->second, out, times, status, err);
+    reqs_.erase(it);
+    return true;
+}
+
+void Server::wait(uint64_t ticket, zk_proof* out, double times[3], int* status, std::string* err) {
+    std::unique_lock<std::mutex> lk(mu_);
+    auto it = reqs_.find(ticket);
+    ZK_REQUIRE(it != reqs_.end(), ZK_E_ARG, "unknown ticket " + std::to_string(ticket));
+    auto req = it->second;
+    cv_done_.wait(lk, [&] { return req->done; });
+    copy_out(*req, out, times, status, err);
+    reqs_.erase(ticket);
+}
+
+bool Server::cancel(uint64_t ticket) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = std::find_if(queue_.begin(), queue_.end(), [&](const std::shared_ptr<ServerRequest>& q) { return q->ticket == ticket; });
+    if (it == queue_.end()) return false;
+    auto req = *it;
+    queue_.erase(it);
+    req->status = ZK_E_STATE;
+    req->error = "cancelled";
+    req->t_done = clk::now();
+    req->done = true;
+    n_cancelled_++;
+    cv_done_.notify_all();
+    return true;
+}
+
+void Server::stats(double out[8]) {
+    std::lock_guard<std::mutex> lk(mu_);
+    double inflight = 0;
+    for (auto& r : running_) inflight += r ? 1 : 0;
+    out[0] = n_completed_;
+    out[1] = n_failed_;
+    out[2] = n_rejected_;
+    out[3] = n_cancelled_;
+    out[4] = (double)queue_.size();
+    out[5] = inflight;
+    out[6] = (double)slots_.size();
+    out[7] = peak_inflight_;
+}
+
+// Admit queued requests into slots, oldest first. A free slot whose workspace
+// already fits is used as is; otherwise a free slot may grow, or a new slot be
+// created, only if free HBM stays above the reserve afterwards.
+bool Server::try_admit(std::unique_lock<std::mutex>& lk) {
+    bool any = false;
+    while (!queue_.empty()) {
+        auto req = queue_.front();
+        const size_t need = ProveSlot::bytes_needed(*req->circ);
+        int pick = -1;
+        for (size_t i = 0; i < slots_.size() && pick < 0; i++)
+            if (!running_[i] && cap_[i] >= need) pick = (int)i;
+        if (pick < 0) {
+            size_t free_b = 0, total_b = 0;
+            ZK_HIP(hipMemGetInfo(&free_b, &total_b));
+            for (size_t i = 0; i < slots_.size() && pick < 0; i++)
+                if (!running_[i] && free_b >= need - std::min(need, cap_[i]) + reserve_) pick = (int)i;
+            if (pick < 0 && (int)slots_.size() < max_slots_ && free_b >= need + reserve_) {
+                slots_.push_back(std::make_unique<ProveSlot>());
+                running_.push_back(nullptr);
+                cap_.push_back(0);
+                pick = (int)slots_.size() - 1;
+            }
+        }
+        if (pick < 0) break;  // wait for a slot or for memory
+        queue_.pop_front();
+        running_[pick] = req;
+        cap_[pick] = std::max(cap_[pick], need);
+        double inflight = 0;
+        for (auto& r : running_) inflight += r ? 1 : 0;
+        peak_inflight_ = std::max(peak_inflight_, inflight);
+        req->t_start = clk::now();
+        lk.unlock();
+        int st = ZK_OK;
+        std::string err;
+        try {
+            std::lock_guard<std::mutex> g(ctx_.mu);
+            slots_[pick]->enqueue(ctx_, *req->circ, req->witness.data(), false, req->r, req->s);
+        } catch (const Error& e) {
+            st = e.code;
+            err = e.what();
+        } catch (const std::exception& e) {
+            st = ZK_E_INTERNAL;
+            err = e.what();
+        }
+        if (st != ZK_OK) {
+            // drop whatever part of the proof was queued before the failure
+            for (hipStream_t p : {slots_[pick]->s0, slots_[pick]->s1, slots_[pick]->s2, slots_[pick]->s3, slots_[pick]->s4})
+                (void)hipStreamSynchronize(p);
+        }
+        lk.lock();
+        if (st != ZK_OK) {
+            req->status = st;
+            req->error = err;
+            req->t_done = clk::now();
+            req->done = true;
+            running_[pick] = nullptr;
+            n_failed_++;
+            cv_done_.notify_all();
+        }
+        any = true;
+    }
+    return any;
+}
+
+void Server::loop() {
+    (void)hipSetDevice(ctx_.device);
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+        bool progress = false;
+        for (size_t i = 0; i < slots_.size(); i++) {
+            auto req = running_[i];
+            if (!req) continue;
+            lk.unlock();
+            bool done = false;
+            int st = ZK_OK;
+            std::string err;
+            try {
+                if (slots_[i]->device_done()) {
+                    std::lock_guard<std::mutex> g(ctx_.mu);
+                    slots_[i]->finish(&req->proof);
+                    done = true;
+                }
+            } catch (const Error& e) {
+                done = true;
+                st = e.code;
+                err = e.what();
+            } catch (const std::exception& e) {
+                done = true;
+                st = ZK_E_INTERNAL;
+                err = e.what();
+            }
+            lk.lock();
+            if (done) {
+                req->status = st;
+                req->error = err;
+                req->t_done = clk::now();
+                req->done = true;
+                running_[i] = nullptr;
+                (st == ZK_OK ? n_completed_ : n_failed_) += 1;
+                progress = true;
+                cv_done_.notify_all();
+            }
+        }
+        try {
+            if (try_admit(lk)) progress = true;
+        } catch (const std::exception&) {
+            // hipMemGetInfo failed: retry on the next iteration
+        }
+        bool busy = std::any_of(running_.begin(), running_.end(), [](const std::shared_ptr<ServerRequest>& r) { return r != nullptr; });
+        if (stop_ && queue_.empty() && !busy) break;
+        if (!progress) {
+            if (busy) cv_work_.wait_for(lk, std::chrono::microseconds(50));
+            else cv_work_.wait(lk, [&] { return stop_ || !queue_.empty(); });
+        }
+    }
+}
+
+}  // namespace zk
