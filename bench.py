@@ -1,18 +1,31 @@
 """Benchmark: Groth16/BN254 proofs/s and p50/p99 latency at 2^20 constraints,
 batched vs plain, on one MI355X.
 
-Workload: the seeded synthetic R1CS with m = 2^20 constraints and 4096
-public inputs (3 terms per A/B row, C_i = aux_i; domain n = 2^21), its
-trapdoor CRS resident in HBM, fresh satisfying witnesses (fresh public inputs)
-and fresh blinding r, s per proof, all uploaded to HBM before timing.
+Sections (all inputs resident in HBM before any timed region; every proof has
+fresh blinding r, s; batched/served proofs are compared bit for bit with plain
+proofs of the same inputs):
 
-  plain   : one proof at a time through zk_prove_device (stream-synchronised)
-  batched : the same proofs through zk_prove_batch_device (several workspace
-            slots overlapped on the device)
+  headline (2^20): the seeded synthetic R1CS with m = 2^20 constraints and 4096
+      public inputs (3 terms per A/B row, C_i = aux_i; domain n = 2^21),
+      trapdoor CRS resident in HBM.
+        plain   : one proof at a time (zk_prove_device, stream-synchronised)
+        batched : the same proofs with several in flight, as streams
+                  (zk_prove_batch_device) and as one replayed HIP graph
+                  (zk_batch_graph_*); `value` is the faster of the two.
+      Kernel probes (alone on the device): the G1 and G2 witness MSMs over the
+      resident bases (roofline of the bucket-accumulation kernel) and NTTs of
+      2^22 (forward, inverse, coset).
+  mixed: 64 proofs, m log-uniform in [2^14, 2^20], 1024 public inputs; half
+      over 4 shared circuits (shared CRS), half over 32 circuits of their own;
+      windows from the autotuner; plain vs one captured graph vs streams.
+  served: Poisson arrivals against 4 resident circuits (m = 2^18, 2^20, 2^22,
+      2^24, 1024 public inputs), continuous batching (zk_server_*), at three
+      arrival rates set from the measured plain capacity; proofs/s, p50/p99.
 
-Prints one JSON line. `value` is the batched whole-job proofs/s.
+Prints one JSON line.
 """
 import argparse
+import concurrent.futures as cf
 import json
 import math
 import os
@@ -24,10 +37,11 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_MAC_TS = 256 * 64 * 2.4e9 / 1e12  # v_mad_u64_u32: 256 CUs x 64 lanes/clk x 2.4 GHz (full rate, measured)
+PEAK_MAC_TS = 256 * 64 * 2.4e9 / 1e12  # VALU lane-ops: 256 CUs x 64 lanes/clk x 2.4 GHz; v_mad_u64_u32 issues at full rate
 PEAK_HBM_GBS = 8000.0
-MACS_PER_FPMUL = 136                    # 8x8 products + 8x8 reduction + 8 quotient words (32-bit CIOS)
-MADD_FPMUL = 11                         # mixed add, Fp-mul equivalents (algorithmic figure)
+MACS_PER_FPMUL = 136                    # 8x8 products + 8x8 reduction + 8 quotient words (32-bit Montgomery)
+MADD_FPMUL = 10                         # XYZZ mixed add (madd-2008-s): 8M + 2S
+MADD_FPMUL_G2 = 30                      # the same in Fp2: 10 Fp2 products x 3 Fp products (Karatsuba)
 
 
 def log(msg):
@@ -58,6 +72,10 @@ def pct(xs, p):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
+def r3(x):
+    return None if x is None else round(float(x), 3)
+
+
 def cpu_baseline(m_cpu, n_pub_cpu, m, n_pub):
     """Oracle (GMP, single thread) prove of a bounded sample, scaled to the metric."""
     import oracle as O
@@ -82,6 +100,342 @@ def cpu_baseline(m_cpu, n_pub_cpu, m, n_pub):
     }
 
 
+def upload_all(ctx, arrays):
+    out = []
+    for a in arrays:
+        p = ctx.dev_alloc(a.nbytes)
+        ctx.upload(p, a)
+        out.append(p)
+    return out
+
+
+# ---------------------------------------------------------------- headline section
+def section_headline(args, zk, ctx):
+    m, n_pub, K, W = 1 << args.log_m, args.n_pub, args.steps, args.warmup
+    t0 = time.time()
+    circ = ctx.synth_circuit(args.seed, m, n_pub)
+    log(f"[bench] circuit m=2^{args.log_m} n_pub={n_pub} n_vars={circ.n_vars} domain=2^{circ.log_n} "
+        f"resident {circ.device_bytes / 2**30:.2f} GiB, setup {time.time() - t0:.1f} s")
+    tuned = None
+    if not args.no_tune:
+        t0 = time.time()
+        tuned = ctx.autotune(circ)
+        log(f"[bench] autotuned windows (witness, h) = {tuned} in {time.time() - t0:.1f} s")
+
+    # satisfying witnesses (distinct public inputs) resident in HBM, cycled over
+    # the proofs; every proof gets fresh blinding r, s
+    total = W + K
+    t0 = time.time()
+    d_uniq = upload_all(ctx, [circ.witness(args.seed * 1000 + j + 1) for j in range(args.witnesses)])
+    d_w = [d_uniq[i % len(d_uniq)] for i in range(total)]
+    bl = [zk.synth_blinding(args.seed * 1000 + i + 1) for i in range(total)]
+    R = np.stack([b[0] for b in bl])
+    S = np.stack([b[1] for b in bl])
+    log(f"[bench] {args.witnesses} witnesses generated and uploaded in {time.time() - t0:.1f} s")
+
+    # ---- plain: one proof at a time, profiled ----
+    ctx.set_profiling(True)
+    for i in range(W):
+        ctx.prove_device(circ, d_w[i], R[i], S[i])
+    lat, stats, plain_proofs = [], [], []
+    t_start = time.perf_counter()
+    for i in range(W, total):
+        t0 = time.perf_counter()
+        pr = ctx.prove_device(circ, d_w[i], R[i], S[i])
+        lat.append((time.perf_counter() - t0) * 1e3)
+        stats.append(ctx.prove_stats())
+        plain_proofs.append(pr.words())
+    plain_ps = K / (time.perf_counter() - t_start)
+    ctx.set_profiling(False)
+    log(f"[bench] plain: {plain_ps:.2f} proofs/s, p50 {pct(lat, 50):.1f} ms, p99 {pct(lat, 99):.1f} ms")
+
+    # ---- batched as streams: all K proofs through the slot pipeline ----
+    ctx.prove_batch_device([circ] * min(W, K), d_w[:min(W, K)], R[:min(W, K)], S[:min(W, K)])
+    t0 = time.perf_counter()
+    out = ctx.prove_batch_device([circ] * K, d_w[W:], R[W:], S[W:])
+    streams_ps = K / (time.perf_counter() - t0)
+    same_streams = all(np.array_equal(out[j].words(), plain_proofs[j]) for j in range(K))
+    log(f"[bench] batched (streams, {args.slots} slots): {streams_ps:.2f} proofs/s; identical to plain: {same_streams}")
+
+    # ---- batched as one captured graph, replayed ----
+    t0 = time.perf_counter()
+    g = ctx.batch_graph([circ] * K, d_w[W:], slots=args.slots)
+    build_s = time.perf_counter() - t0
+    g.run(R[:K][::-1], S[:K][::-1])                      # warm replay, other blinding
+    t0 = time.perf_counter()
+    out = g.run(R[W:], S[W:])
+    graph_ps = K / (time.perf_counter() - t0)
+    same_graph = all(np.array_equal(out[j].words(), plain_proofs[j]) for j in range(K))
+    log(f"[bench] batched (graph, {g.n_nodes} nodes, built in {build_s:.2f} s): {graph_ps:.2f} proofs/s; "
+        f"identical to plain: {same_graph}")
+    n_nodes = g.n_nodes
+    g.close()
+
+    mean = {k: float(np.mean([s[k] for s in stats])) for k in stats[0]}
+    best_mode = "graph" if graph_ps >= streams_ps else "streams"
+    batch_ps = max(graph_ps, streams_ps)
+    return {
+        "circ": circ, "d_uniq": d_uniq, "c_w": int(mean["c_w"]), "c_h": int(mean["c_h"]),
+        "plain": {"proofs_per_s": round(plain_ps, 4), "p50_ms": r3(pct(lat, 50)), "p99_ms": r3(pct(lat, 99))},
+        "batched": {"proofs_per_s": round(batch_ps, 4), "mode": best_mode,
+                    "streams_proofs_per_s": round(streams_ps, 4), "graph_proofs_per_s": round(graph_ps, 4),
+                    "identical_to_plain": bool(same_streams and same_graph), "slots": args.slots,
+                    "graph_nodes": int(n_nodes), "graph_build_s": round(build_s, 3),
+                    "latency_note": "all proofs submitted at t=0; per-proof latency = batch position x period"},
+        "speedup_batched_vs_plain": round(batch_ps / plain_ps, 4),
+        "phases_ms": {k: round(v, 4) for k, v in mean.items() if k.endswith("_ms")},
+        "tuned_windows": tuned,
+    }
+
+
+# ---------------------------------------------------------------- kernel probes
+def probe_msm(args, zk, ctx, circ, which, c_w, g2):
+    """Witness MSM over resident bases, alone on the device: (total, sort, accumulation ms, entries)."""
+    d_bases, n_ext = circ.bases(which)
+    w0 = circ.witness(args.seed * 1000 + 1)
+    ext = np.ascontiguousarray(np.concatenate([w0, zk.synth_uniform(args.seed + 7, 4)]), dtype=np.uint64)
+    assert ext.shape[0] == n_ext
+    d_sc = ctx.dev_alloc(ext.nbytes)
+    ctx.upload(d_sc, ext)
+    rows = []
+    fn = ctx.msm_g2_device if g2 else ctx.msm_g1_device
+    for i in range(2 + args.probe_reps):
+        fn(d_bases, d_sc, n_ext, c_w)
+        if i >= 2:
+            rows.append(ctx.last_timing())
+    ctx.dev_free(d_sc)
+    return n_ext, np.array(rows).mean(axis=0)
+
+
+def probe_ntt(args, zk, ctx, log_n):
+    n = 1 << log_n
+    x = zk.synth_uniform(args.seed + 11, n)
+    a, b = ctx.dev_alloc(x.nbytes), ctx.dev_alloc(x.nbytes)
+    res = {}
+    for kind, name in ((0, "forward"), (1, "inverse"), (2, "coset")):
+        ts = []
+        for i in range(2 + args.probe_reps):
+            ctx.upload(a, x)
+            ctx.ntt_device([a], [b], log_n, kind)
+            if i >= 2:
+                ts.append(ctx.last_timing()[0])
+        res[name] = float(np.mean(ts))
+    ctx.dev_free(a)
+    ctx.dev_free(b)
+    return res
+
+
+def section_probes(args, zk, ctx, head):
+    circ, c_w = head["circ"], head["c_w"]
+    n_ext, t1 = probe_msm(args, zk, ctx, circ, 0, c_w, False)
+    _, t2 = probe_msm(args, zk, ctx, circ, 2, c_w, True)
+    t_launch = t1[2] / 1e3
+    units = float(t1[3])
+    achieved = units * MADD_FPMUL * MACS_PER_FPMUL / t_launch / 1e12
+    g2_achieved = float(t2[3]) * MADD_FPMUL_G2 * MACS_PER_FPMUL / (t2[2] / 1e3) / 1e12
+    log(f"[bench] isolated G1 witness MSM n={n_ext} c={c_w}: total {t1[0]:.3f} ms, sort {t1[1]:.3f} ms, "
+        f"accumulation {t1[2]:.3f} ms over {int(units)} entries -> {achieved:.2f} TMAC/s")
+    log(f"[bench] isolated G2 witness MSM: total {t2[0]:.3f} ms, accumulation {t2[2]:.3f} ms -> {g2_achieved:.2f} TMAC/s")
+    roofline = {
+        "bound": "valu", "kernel": "k_msm_accum<Fp> (G1 bucket accumulation)",
+        "achieved": round(achieved, 3), "peak": round(PEAK_MAC_TS, 2), "unit": "TMAC/s",
+        "frac": round(achieved / PEAK_MAC_TS, 4), "traffic": None,
+        "avg_launch_ms": round(t_launch * 1e3, 4), "units_per_launch": int(units),
+        "unit_def": "one signed-digit (point, bucket) entry = one XYZZ mixed add = 10 Fp-mul x 136 32x32 MACs",
+        "probe": (f"G1 MSM over the resident A' bases, n={n_ext}, c={c_w}, alone on the device, "
+                  f"{args.probe_reps} launches, HIP events on the MSM stream"),
+        "msm_total_ms": r3(t1[0]), "sort_ms": r3(t1[1]),
+        "g2": {"kernel": "k_msm_accum<Fq2>", "achieved": round(g2_achieved, 3),
+               "frac": round(g2_achieved / PEAK_MAC_TS, 4), "avg_launch_ms": r3(t2[2]),
+               "msm_total_ms": r3(t2[0]), "unit_def": "entry = Fp2 XYZZ mixed add = 30 Fp-mul x 136 MACs"},
+    }
+    ntt = probe_ntt(args, zk, ctx, 22)
+    n = 1 << 22
+    gbs = {k: 64 * n / (v / 1e3) / 1e9 for k, v in ntt.items()}
+    log(f"[bench] NTT 2^22 ms {ntt} -> GB/s {({k: round(v) for k, v in gbs.items()})}")
+    roofline_ntt = {
+        "bound": "hbm", "kernel": "NTT 2^22 (k_ntt_pass + k_ntt_final), forward",
+        "achieved": round(gbs["forward"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+        "frac": round(gbs["forward"] / PEAK_HBM_GBS, 4), "traffic": None,
+        "avg_launch_ms": r3(ntt["forward"]), "algorithmic_bytes": 64 * n,
+        "ms": {k: r3(v) for k, v in ntt.items()},
+        "GBps": {k: round(v, 1) for k, v in gbs.items()},
+    }
+    return roofline, roofline_ntt
+
+
+# ---------------------------------------------------------------- mixed batch
+def section_mixed(args, zk, ctx):
+    rng = np.random.default_rng(args.seed + 3)
+    n_pub = 1024
+    lo, hi = 14, 20
+    n_distinct = args.mixed_proofs // 2
+    shared_m = [int(round(2 ** rng.uniform(lo, hi))) for _ in range(4)]
+    distinct_m = [int(round(2 ** rng.uniform(lo, hi))) for _ in range(n_distinct)]
+    t0 = time.time()
+    shared = [ctx.synth_circuit(30000 + i, m, n_pub) for i, m in enumerate(shared_m)]
+    distinct = [ctx.synth_circuit(40000 + j, m, n_pub) for j, m in enumerate(distinct_m)]
+    setup_s = time.time() - t0
+    plan = [shared[int(rng.integers(0, 4))] for _ in range(args.mixed_proofs - n_distinct)] + distinct
+    plan = [plan[i] for i in rng.permutation(len(plan))]
+    t0 = time.time()
+    if not args.no_tune:
+        for c in shared + distinct:
+            ctx.autotune(c)
+    tune_s = time.time() - t0
+    d_ws = upload_all(ctx, [c.witness(50000 + k) for k, c in enumerate(plan)])
+    N = len(plan)
+    bl = [zk.synth_blinding(60000 + k) for k in range(N)]
+    R = np.stack([b[0] for b in bl])
+    S = np.stack([b[1] for b in bl])
+    log(f"[bench] mixed: {N} proofs, n_vars in [{min(c.n_vars for c in plan)}, {max(c.n_vars for c in plan)}], "
+        f"{len(shared) + len(distinct)} circuits (setup {setup_s:.1f} s, tuning {tune_s:.1f} s)")
+    # plain (one untimed proof per circuit first)
+    seen = set()
+    for k, c in enumerate(plan):
+        if id(c) not in seen:
+            seen.add(id(c))
+            ctx.prove_device(c, d_ws[k], R[k], S[k])
+    t0 = time.perf_counter()
+    plain = [ctx.prove_device(c, d_ws[k], R[k], S[k]).words() for k, c in enumerate(plan)]
+    plain_s = time.perf_counter() - t0
+    # one captured graph over the whole batch
+    t0 = time.perf_counter()
+    g = ctx.batch_graph(plan, d_ws, slots=args.mixed_slots)
+    build_s = time.perf_counter() - t0
+    g.run(R[::-1], S[::-1])
+    t0 = time.perf_counter()
+    out = g.run(R, S)
+    graph_s = time.perf_counter() - t0
+    same_graph = all(np.array_equal(out[k].words(), plain[k]) for k in range(N))
+    nodes = g.n_nodes
+    g.close()
+    # streams
+    ctx.set_batch_slots(args.mixed_slots)
+    ctx.prove_batch_device(plan[:4], d_ws[:4], R[:4], S[:4])
+    t0 = time.perf_counter()
+    out = ctx.prove_batch_device(plan, d_ws, R, S)
+    streams_s = time.perf_counter() - t0
+    ctx.set_batch_slots(args.slots)
+    same_streams = all(np.array_equal(out[k].words(), plain[k]) for k in range(N))
+    log(f"[bench] mixed: plain {N / plain_s:.2f} proofs/s, graph {N / graph_s:.2f} ({nodes} nodes, "
+        f"built {build_s:.2f} s), streams {N / streams_s:.2f}; identical: graph {same_graph}, streams {same_streams}")
+    windows = sorted({(c.n_vars + 4, ctx.tuned_window(0, c.n_vars + 4), c.n, ctx.tuned_window(1, c.n))
+                      for c in shared + distinct})
+    res = {
+        "proofs": N, "n_public": n_pub, "m_range_log2": [lo, hi],
+        "circuits": {"shared": len(shared), "shared_proofs": N - n_distinct, "distinct": n_distinct},
+        "plain_proofs_per_s": round(N / plain_s, 4),
+        "graph_proofs_per_s": round(N / graph_s, 4),
+        "streams_proofs_per_s": round(N / streams_s, 4),
+        "speedup_graph_vs_plain": round(plain_s / graph_s, 4),
+        "speedup_streams_vs_plain": round(plain_s / streams_s, 4),
+        "identical_to_plain": bool(same_graph and same_streams),
+        "graph_nodes": int(nodes), "graph_build_s": round(build_s, 3), "slots": args.mixed_slots,
+        "windows_by_size": [{"n_witness": a, "c_w": b, "n_h": c, "c_h": d} for a, b, c, d in windows],
+    }
+    for p in d_ws:
+        ctx.dev_free(p)
+    for c in shared + distinct:
+        c.free()
+    return res
+
+
+# ---------------------------------------------------------------- served
+def section_served(args, zk, ctx, pool_futures):
+    n_pub = 1024
+    sizes = args.serve_log_m
+    circs = {}
+    for lg in sizes:
+        t0 = time.time()
+        circs[lg] = ctx.synth_circuit(70000 + lg, 1 << lg, n_pub)
+        log(f"[bench] served: circuit 2^{lg} resident {circs[lg].device_bytes / 2**30:.2f} GiB, "
+            f"setup {time.time() - t0:.1f} s")
+        if not args.no_tune:
+            t0 = time.time()
+            tw = ctx.autotune(circs[lg])
+            log(f"[bench] served: 2^{lg} windows {tw} tuned in {time.time() - t0:.1f} s")
+    pools = {lg: upload_all(ctx, pool_futures[lg].result()) for lg in sizes}
+    # plain capacity: one proof at a time per circuit
+    svc = {}
+    for lg in sizes:
+        r, s = zk.synth_blinding(80000 + lg)
+        ctx.prove_device(circs[lg], pools[lg][0], r, s)
+        ts = []
+        for i in range(args.serve_svc_reps):
+            t0 = time.perf_counter()
+            ctx.prove_device(circs[lg], pools[lg][i % len(pools[lg])], r, s)
+            ts.append(time.perf_counter() - t0)
+        svc[lg] = float(np.mean(ts))
+    capacity = len(sizes) / sum(svc.values())          # requests/s, uniform circuit choice, one at a time
+    log(f"[bench] served: plain service times (ms) {({k: round(v * 1e3, 1) for k, v in svc.items()})}, "
+        f"capacity {capacity:.2f} req/s")
+    srv = zk.Server(ctx, max_slots=args.serve_slots, reserve_bytes=args.serve_reserve_gib << 30)
+    rng = np.random.default_rng(args.seed + 5)
+    rates = []
+    last = None
+    for frac in args.serve_loads:
+        rate = frac * capacity
+        N = args.serve_requests
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, N))
+        which = rng.integers(0, len(sizes), N)
+        picks = [(sizes[int(q)], int(rng.integers(0, 1 << 30))) for q in which]
+        tickets = []
+        t_base = time.perf_counter()
+        for k in range(N):
+            while True:
+                dt = t_base + arrivals[k] - time.perf_counter()
+                if dt <= 0:
+                    break
+                time.sleep(min(dt, 0.002))
+            lg, seed = picks[k]
+            r, s = zk.synth_blinding(seed)
+            d = pools[lg][seed % len(pools[lg])]
+            tickets.append((srv.submit_device(circs[lg], d, r, s), lg, d, r, s))
+        lat, queued, by, results = [], [], {lg: [] for lg in sizes}, []
+        for t, lg, d, r, s in tickets:
+            pr, tm = srv.wait(t)
+            lat.append(tm[2])
+            queued.append(tm[0])
+            by[lg].append(tm[2])
+            results.append((pr, lg, d, r, s))
+        wall = time.perf_counter() - t_base
+        st = srv.stats()
+        rates.append({
+            "load": frac, "offered_req_per_s": round(rate, 3), "requests": N,
+            "proofs_per_s": round(N / wall, 3),
+            "p50_ms": r3(pct(lat, 50)), "p99_ms": r3(pct(lat, 99)), "mean_queued_ms": r3(np.mean(queued)),
+            "p50_ms_by_log_m": {str(lg): r3(pct(v, 50)) for lg, v in by.items() if v},
+            "p99_ms_by_log_m": {str(lg): r3(pct(v, 99)) for lg, v in by.items() if v},
+            "peak_in_flight": int(st["peak_in_flight"]), "slots": int(st["slots"]),
+        })
+        log(f"[bench] served load {frac:.2f} ({rate:.2f} req/s): {N / wall:.2f} proofs/s, p50 {pct(lat, 50):.1f} ms, "
+            f"p99 {pct(lat, 99):.1f} ms, peak in flight {int(st['peak_in_flight'])}")
+        last = results
+    failed = int(srv.stats()["failed"])
+    srv.close()
+    # served results against plain proofs of the same inputs (one per circuit size)
+    same, done = True, set()
+    for pr, lg, d, r, s in last:
+        if lg in done:
+            continue
+        done.add(lg)
+        same &= bool(np.array_equal(pr.words(), ctx.prove_device(circs[lg], d, r, s).words()))
+    for lg in sizes:
+        for p in pools[lg]:
+            ctx.dev_free(p)
+        circs[lg].free()
+    return {
+        "circuits_log_m": sizes, "n_public": n_pub,
+        "witness_pool_per_circuit": {str(lg): len(pool_futures[lg].result()) for lg in sizes},
+        "plain_service_ms": {str(lg): r3(v * 1e3) for lg, v in svc.items()},
+        "plain_capacity_req_per_s": round(capacity, 3),
+        "rates": rates, "identical_to_plain": same, "failed": failed, "max_slots": args.serve_slots,
+        "reserve_gib": args.serve_reserve_gib,
+        "arrivals": "Poisson, circuit uniform over the four sizes, fresh r, s per request",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +448,18 @@ def main():
     ap.add_argument("--witnesses", type=int, default=4)
     ap.add_argument("--cpu-log-m", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--probe-reps", type=int, default=5)
+    ap.add_argument("--sections", default="headline,mixed,served",
+                    help="comma list of: headline (always run), mixed, served")
+    ap.add_argument("--mixed-proofs", type=int, default=64)
+    ap.add_argument("--mixed-slots", type=int, default=4)
+    ap.add_argument("--serve-log-m", type=lambda s: [int(x) for x in s.split(",")], default=[18, 20, 22, 24])
+    ap.add_argument("--serve-loads", type=lambda s: [float(x) for x in s.split(",")], default=[0.4, 0.7, 1.0])
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-reserve-gib", type=int, default=8)
+    ap.add_argument("--serve-svc-reps", type=int, default=2)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -103,115 +469,50 @@ def main():
 
     import zyklop_amd as zk
 
-    m, n_pub, K, W = 1 << args.log_m, args.n_pub, args.steps, args.warmup
+    sections = set(args.sections.split(","))
+    # witnesses of the served circuits are generated on host threads while the GPU sections run
+    pool_futures, ex = {}, None
+    if "served" in sections:
+        ex = cf.ThreadPoolExecutor(max_workers=4)
+        for lg in args.serve_log_m:
+            k = 2 if lg <= 20 else 1
+            pool_futures[lg] = ex.submit(
+                lambda lg=lg, k=k: [zk.synth_witness(70000 + lg, 1 << lg, 1024, 71000 + lg * 10 + j) for j in range(k)])
+
     ctx = zk.Context(0)
     ctx.set_batch_slots(args.slots)
-    t0 = time.time()
-    circ = ctx.synth_circuit(args.seed, m, n_pub)
-    log(f"[bench] circuit m=2^{args.log_m} n_pub={n_pub} n_vars={circ.n_vars} domain=2^{circ.log_n} "
-        f"resident {circ.device_bytes / 2**30:.2f} GiB, setup {time.time() - t0:.1f} s")
-
-    # satisfying witnesses (distinct public inputs) resident in HBM, cycled over
-    # the proofs; every proof gets fresh blinding r, s
-    total = W + K
-    t0 = time.time()
-    d_uniq = []
-    for j in range(args.witnesses):
-        w = circ.witness(args.seed * 1000 + j + 1)
-        p = ctx.dev_alloc(w.nbytes)
-        ctx.upload(p, w)
-        d_uniq.append(p)
-    d_w, rs, ss = [], [], []
-    for i in range(total):
-        d_w.append(d_uniq[i % len(d_uniq)])
-        r, s = zk.synth_blinding(args.seed * 1000 + i + 1)
-        rs.append(r)
-        ss.append(s)
-    log(f"[bench] {args.witnesses} witnesses generated and uploaded in {time.time() - t0:.1f} s")
-
-    # ---- plain: one proof at a time, profiled ----
-    ctx.set_profiling(True)
-    for i in range(W):
-        ctx.prove_device(circ, d_w[i], rs[i], ss[i])
-    lat, stats = [], []
-    plain_proofs = []
-    t_start = time.perf_counter()
-    for i in range(W, total):
-        t0 = time.perf_counter()
-        pr = ctx.prove_device(circ, d_w[i], rs[i], ss[i])
-        lat.append((time.perf_counter() - t0) * 1e3)
-        stats.append(ctx.prove_stats())
-        plain_proofs.append(pr.words())
-    plain_wall = time.perf_counter() - t_start
-    plain_ps = K / plain_wall
-    log(f"[bench] plain: {plain_ps:.2f} proofs/s, p50 {pct(lat, 50):.1f} ms, p99 {pct(lat, 99):.1f} ms")
-
-    # ---- batched: all K proofs in flight through the slot pipeline ----
-    circs = [circ] * K
-    ctx.prove_batch_device([circ] * min(W, K), d_w[:min(W, K)], np.stack(rs[:min(W, K)]), np.stack(ss[:min(W, K)]))
-    t0 = time.perf_counter()
-    out = ctx.prove_batch_device(circs, d_w[W:], np.stack(rs[W:]), np.stack(ss[W:]))
-    batch_wall = time.perf_counter() - t0
-    batch_ps = K / batch_wall
-    same = all(np.array_equal(out[j].words(), plain_proofs[j]) for j in range(K))
-    log(f"[bench] batched: {batch_ps:.2f} proofs/s ({batch_wall * 1e3:.1f} ms for {K}); "
-        f"identical to plain: {same}")
-
-    # ---- roofline of the dominant kernel: G1 bucket accumulation ----
-    mean = {k: float(np.mean([s[k] for s in stats])) for k in stats[0]}
-    g1_launch_ms, g1_units = [], []
-    for s in stats:
-        for name, ent in (("acc_A_ms", "entries_w"), ("acc_B1_ms", "entries_w"), ("acc_C_ms", "entries_w"),
-                          ("acc_H_ms", "entries_h")):
-            g1_launch_ms.append(s[name])
-            g1_units.append(s[ent])
-    t_launch = float(np.mean(g1_launch_ms)) / 1e3
-    units = float(np.mean(g1_units))
-    achieved = units * MADD_FPMUL * MACS_PER_FPMUL / t_launch / 1e12
-    roofline = {
-        "bound": "valu", "kernel": "k_msm_accum<Fp> (G1 bucket accumulation)",
-        "achieved": round(achieved, 3), "peak": round(PEAK_MAC_TS, 2), "unit": "TMAC/s",
-        "frac": round(achieved / PEAK_MAC_TS, 4), "traffic": None,
-        "avg_launch_ms": round(t_launch * 1e3, 4), "units_per_launch": int(units),
-        "unit_def": "one signed-digit (point, bucket) entry = one mixed add = 11 Fp-mul x 136 32x32 MACs",
-    }
-    n = circ.n
-    ntt_ms = mean["ntt3_ms"]
-    ntt_gbs = 3 * 64 * n / (ntt_ms / 1e3) / 1e9
-    roofline_ntt = {
-        "bound": "hbm", "kernel": "3 x forward NTT 2^%d (k_ntt_pass + k_ntt_final)" % circ.log_n,
-        "achieved": round(ntt_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-        "frac": round(ntt_gbs / PEAK_HBM_GBS, 4), "traffic": None,
-        "avg_launch_ms": round(ntt_ms, 4), "algorithmic_bytes": 3 * 64 * n,
-    }
-
+    head = section_headline(args, zk, ctx)
+    roofline, roofline_ntt = section_probes(args, zk, ctx, head)
+    m, n_pub, K = 1 << args.log_m, args.n_pub, args.steps
+    circ = head.pop("circ")
+    for p in head.pop("d_uniq"):
+        ctx.dev_free(p)
     result = {
         "metric": "Groth16 proofs/s + p50/p99 latency at 2^20 constraints, batched vs plain",
-        "value": round(batch_ps, 4),
+        "value": head["batched"]["proofs_per_s"],
         "unit": "proofs/s",
         "n_gpus": 1,
         "steps": K,
-        "warmup": W,
-        "ms_per_step": round(1e3 / batch_ps, 3),
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 / head["batched"]["proofs_per_s"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
         "config": {"workload": f"groth16_bn254_prove_m2^{args.log_m}_pub{n_pub}", "constraints": m,
-                   "n_public": n_pub, "n_vars": circ.n_vars, "domain": n, "batched_slots": args.slots,
-                   "window_witness": int(mean["c_w"]), "window_h": int(mean["c_h"]), "blinding": "fresh r,s"},
-        "plain": {"proofs_per_s": round(plain_ps, 4), "p50_ms": round(pct(lat, 50), 3),
-                  "p99_ms": round(pct(lat, 99), 3)},
-        "batched": {"proofs_per_s": round(batch_ps, 4), "identical_to_plain": same,
-                    "latency_note": "all proofs submitted at t=0; per-proof latency = batch position x period"},
-        "speedup_batched_vs_plain": round(batch_ps / plain_ps, 4),
-        "phases_ms": {k: round(v, 4) for k, v in mean.items() if k.endswith("_ms")},
-        "roofline": roofline,
-        "roofline_ntt": roofline_ntt,
+                   "n_public": n_pub, "n_vars": circ.n_vars, "domain": circ.n, "batched_slots": args.slots,
+                   "window_witness": head.pop("c_w"), "window_h": head.pop("c_h"), "blinding": "fresh r,s"},
     }
-    for p in d_uniq:
-        ctx.dev_free(p)
+    result.update(head)
+    result["roofline"] = roofline
+    result["roofline_ntt"] = roofline_ntt
+    circ.free()
+    if "mixed" in sections:
+        result["mixed_batch"] = section_mixed(args, zk, ctx)
+    if "served" in sections:
+        result["served"] = section_served(args, zk, ctx, pool_futures)
+        ex.shutdown()
     if not args.no_cpu:
         log(f"[bench] cpu baseline: oracle prove at m=2^{args.cpu_log_m} ...")
         result["cpu_baseline"] = cpu_baseline(1 << args.cpu_log_m, 1024, m, n_pub)

@@ -36,6 +36,7 @@ extern "C" {
 typedef struct zk_ctx zk_ctx;
 typedef struct zk_circuit zk_circuit;
 typedef struct zk_server zk_server;
+typedef struct zk_batch zk_batch;
 
 /* Same layout as Groth16Proof (include/groth16.cuh:69-73). */
 typedef struct zk_proof {
@@ -82,8 +83,42 @@ int zk_prove(zk_ctx* ctx, const zk_circuit* c, const uint64_t* witness, uint32_t
 /* same, witness already in device memory (n_vars x 32 bytes) */
 int zk_prove_device(zk_ctx* ctx, const zk_circuit* c, const void* d_witness, const uint64_t r[4], const uint64_t s[4],
                     zk_proof* out);
-/* MSM window override for the next proofs on ctx (0 = per-size default) */
+/* MSM window override for the next proofs on ctx (0 = tuned table, else the
+ * per-size default). The reference fixes the window at compile time
+ * (include/msm_g1.cuh:11, MSM_G1_C_BITS). */
 int zk_set_windows(zk_ctx* ctx, int window_witness, int window_h);
+
+/* MSM window autotuner (no reference counterpart: the reference's window is a
+ * compile-time constant, include/msm_g1.cuh:11). For circuit c it measures on
+ * the device, over c's resident bases and uniform scalars, the digit sort plus
+ * the MSMs that share it for every window in [11, 18] (clipped to at most
+ * ceil(log2 n) + 2): kind 0 = the witness sort over n_vars + 4 scalars and the
+ * A', B1', C' (G1) and B2' (G2) MSMs; kind 1 = the sort of h and the H MSM
+ * (n scalars). The fastest window is recorded per (kind, ceil log2 n) in the
+ * context's table, which every later proof on ctx uses unless zk_set_windows
+ * forces one. force = 0 keeps size classes already in the table.
+ * windows (may be NULL) receives the (witness, h) windows now in effect. */
+int zk_autotune(zk_ctx* ctx, const zk_circuit* c, int force, int windows[2]);
+/* window a proof on ctx would use for an MSM of kind (0 witness, 1 h) and size n */
+int zk_tune_get(zk_ctx* ctx, int kind, uint64_t n, int* window);
+/* merge / write the tuning table as text ("<arch> <kind> <ceil log2 n> <c>"
+ * per line); entries for other GPU architectures are skipped on load */
+int zk_tune_load(zk_ctx* ctx, const char* path, int* n_entries);
+int zk_tune_save(zk_ctx* ctx, const char* path);
+
+/* Captured batch (no reference counterpart: the reference proves one pair per
+ * process, src/main.cu:115-171). A fixed sequence of count proofs over
+ * circuits[i] and the device witnesses at d_witnesses[i] (n_vars x 32 bytes,
+ * canonical) is recorded once as one HIP graph over `slots` workspace slots
+ * (proof i on slot i % slots). zk_batch_graph_run replays it: the witnesses
+ * are read from the same device addresses (the caller may change their
+ * contents between runs), r and s (count x 4 words, NULL = zeros) are new per
+ * run. Results are identical to zk_prove_device on the same inputs. */
+int zk_batch_graph_create(zk_ctx* ctx, int count, const zk_circuit* const* circuits, const void* const* d_witnesses,
+                          int slots, zk_batch** out);
+int zk_batch_graph_run(zk_batch* b, const uint64_t* r, const uint64_t* s, zk_proof* out);
+int zk_batch_graph_info(const zk_batch* b, uint64_t* n_nodes, int* n_slots);
+void zk_batch_graph_destroy(zk_batch* b);
 
 /* Batched proving: count independent proofs, possibly over different
  * circuits, overlapped on the device (several workspace slots and streams).
@@ -107,7 +142,8 @@ int zk_set_batch_slots(zk_ctx* ctx, int n);
  *  [10..14] whole MSM (accumulation .. partials on host) ms, same order
  *  [15] digit entries of the witness sort  [16] of the h sort
  *  [17] window c (witness) [18] c (h) [19] windows W (witness) [20] W (h)
- *  [21] stream-0 span ms  [22] stream-1 span ms  [23] host tail ms */
+ *  [21] stream-0 span ms  [22] witness-MSM span ms (shared sort start to
+ *  the last of the four MSMs)  [23] host tail ms */
 int zk_set_profiling(zk_ctx* ctx, int on);
 int zk_prove_stats(zk_ctx* ctx, double* out, int n);
 
@@ -122,6 +158,11 @@ void zk_server_destroy(zk_server* s);
 /* witness: host, canonical, copied at submit; r, s NULL = 0 */
 int zk_server_submit(zk_server* s, const zk_circuit* c, const uint64_t* witness, uint32_t n_witness, const uint64_t r[4],
                      const uint64_t sb[4], uint64_t* ticket);
+/* witness already in device memory (n_vars x 32 bytes, canonical, as for
+ * zk_prove_device); it is read when the request runs and must stay valid and
+ * unchanged until the request completes */
+int zk_server_submit_device(zk_server* s, const zk_circuit* c, const void* d_witness, const uint64_t r[4],
+                            const uint64_t sb[4], uint64_t* ticket);
 /* *done = 1 when finished (then the ticket is released); times (ms):
  * [0] queued, [1] on the device + host tail, [2] total latency. A failed
  * request returns its status here without affecting other requests. */
@@ -179,9 +220,15 @@ int zk_msm_g1_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, s
                      int* is_inf);
 int zk_msm_g2_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t n, int window, uint64_t out[16],
                      int* is_inf);
-/* stream-ordered device time of the last kernel-level call, milliseconds
- * ([0] total, [1] bucket phase (MSM) / last pass (NTT)) */
-int zk_last_timing(zk_ctx* ctx, float ms[2]);
+/* device time of the last kernel-level call on ctx (HIP events on its stream):
+ * MSM: [0] total ms [1] digit sort ms [2] bucket-accumulation kernel ms
+ *      [3] number of (point, bucket) entries accumulated;
+ * NTT: [0] total ms, others 0 */
+int zk_last_timing(zk_ctx* ctx, float ms[4]);
+/* device pointer and point count of a resident base array: which = 0 A',
+ * 1 B1', 2 B2' (G2), 3 C', 4 H; the primed arrays carry the 4 extra slots
+ * described in the prover (alpha/beta/delta terms) after the n_vars bases */
+int zk_circuit_bases(const zk_circuit* c, int which, void** d_ptr, uint64_t* count);
 
 /* ---- synthetic workload (seeded; see the generator's recipe) ---- */
 /* builds the R1CS + trapdoor setup for (seed, m, n_pub) and makes it resident */

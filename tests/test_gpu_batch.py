@@ -1,0 +1,110 @@
+"""Batched proving on the MI355X: a fixed batch captured as one HIP graph and
+replayed, and the MSM window autotuner. Both must leave every proof identical
+(tolerance 0: the arithmetic is exact) to one-at-a-time proofs of the same
+inputs, and one proof per test is checked against the oracle and verified."""
+import numpy as np
+import pytest
+
+import oracle as O
+import zyklop_amd as zk
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup():
+    ctx = zk.Context(0)
+    specs = [(81, 300, 3), (82, 1500, 8), (83, 700, 5)]
+    circs = [ctx.synth_circuit(seed, m, l) for seed, m, l in specs]
+    yield ctx, specs, circs
+    for c in circs:
+        c.free()
+    ctx.close()
+
+
+def _upload(ctx, arrays):
+    ptrs = []
+    for a in arrays:
+        p = ctx.dev_alloc(a.nbytes)
+        ctx.upload(p, a)
+        ptrs.append(p)
+    return ptrs
+
+
+def test_batch_graph_matches_plain(setup):
+    ctx, specs, circs = setup
+    order = [0, 1, 2, 1, 0, 2, 1]            # mixed sizes, circuits shared across proofs
+    ws = [circs[ci].witness(500 + k) for k, ci in enumerate(order)]
+    d_ws = _upload(ctx, ws)
+    g = ctx.batch_graph([circs[ci] for ci in order], d_ws, slots=3)
+    assert g.n_nodes > 0 and g.slots == 3
+    for rnd in range(2):
+        if rnd == 1:                          # new witness contents at the same addresses
+            ws = [circs[ci].witness(900 + k) for k, ci in enumerate(order)]
+            for p, w in zip(d_ws, ws):
+                ctx.upload(p, w)
+        bl = [zk.synth_blinding(100 * rnd + k) for k in range(len(order))]
+        R = np.stack([b[0] for b in bl])
+        S = np.stack([b[1] for b in bl])
+        out = g.run(R, S)
+        for k, ci in enumerate(order):
+            plain = ctx.prove_device(circs[ci], d_ws[k], R[k], S[k]).words()
+            assert np.array_equal(out[k].words(), plain), (rnd, k)
+    # zero blinding = reference mode
+    out0 = g.run()
+    assert np.array_equal(out0[1].words(), ctx.prove_device(circs[order[1]], d_ws[1]).words())
+    # one proof of the last round against the oracle's closed form, verified
+    seed, m, l = specs[order[3]]
+    ref = O.Circuit(seed, m, l, with_crs=False)
+    assert np.array_equal(out[3].words(), ref.prove_trapdoor(R[3], S[3], w=ws[3]))
+    assert ref.verify(out[3].words(), R[3], S[3], w=ws[3])
+    g.close()
+    for p in d_ws:
+        ctx.dev_free(p)
+
+
+def test_batch_graph_rejects_bad_blinding(setup):
+    ctx, specs, circs = setup
+    w = circs[0].witness(1)
+    (d,) = _upload(ctx, [w])
+    g = ctx.batch_graph([circs[0]], [d], slots=1)
+    bad = np.array([[0xFFFFFFFFFFFFFFFF] * 4], np.uint64)
+    with pytest.raises(zk.ZkError) as e:
+        g.run(bad, None)
+    assert e.value.code == 1
+    g.close()
+    ctx.dev_free(d)
+
+
+def test_autotune_table(tmp_path):
+    ctx = zk.Context(0)                       # fresh context: empty table
+    c = ctx.synth_circuit(84, 2000, 8)
+    w = c.witness(5)
+    r, s = zk.synth_blinding(3)
+    n_w = c.n_vars + 4
+    before = ctx.prove(c, w, r, s).words()
+    cw, ch = ctx.autotune(c)
+    assert 11 <= cw <= 18 and 11 <= ch <= 18
+    assert (ctx.tuned_window(0, n_w), ctx.tuned_window(1, c.n)) == (cw, ch)
+    assert ctx.autotune(c) == (cw, ch)        # cached size classes are kept
+    after = ctx.prove(c, w, r, s).words()
+    assert np.array_equal(before, after)      # the window never changes the result
+    path = tmp_path / "windows.txt"
+    ctx.tune_save(path)
+    ctx2 = zk.Context(0)
+    assert ctx2.tune_load(path) == 2
+    assert (ctx2.tuned_window(0, n_w), ctx2.tuned_window(1, c.n)) == (cw, ch)
+    ctx.set_windows(13, 12)                   # a forced window wins over the table
+    assert (ctx.tuned_window(0, n_w), ctx.tuned_window(1, c.n)) == (13, 12)
+    assert np.array_equal(ctx.prove(c, w, r, s).words(), before)
+    bad = tmp_path / "bad.txt"
+    bad.write_text("gfx950 0 twenty 16\n")
+    with pytest.raises(zk.ZkError) as e:
+        ctx2.tune_load(bad)
+    assert e.value.code == 4
+    with pytest.raises(zk.ZkError) as e:
+        ctx2.tune_load(tmp_path / "missing.txt")
+    assert e.value.code == 3
+    c.free()
+    ctx2.close()
+    ctx.close()

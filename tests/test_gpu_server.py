@@ -53,6 +53,35 @@ def test_mix_equals_alone(setup):
     assert ref.verify(alone[0], r, s, w=w)
 
 
+def test_device_witnesses_in_mix(setup):
+    ctx, specs, circs = setup
+    reqs = []
+    for k in range(8):
+        ci = k % len(circs)
+        w = circs[ci].witness(specs[ci][0] * 7 + k)
+        r, s = zk.synth_blinding(2000 + k)
+        reqs.append((ci, w, r, s))
+    d_ws = []
+    for _, w, _, _ in reqs:
+        p = ctx.dev_alloc(w.nbytes)
+        ctx.upload(p, w)
+        d_ws.append(p)
+    srv = zk.Server(ctx, max_slots=3, reserve_bytes=1 << 30)
+    tickets = []
+    for k, (ci, w, r, s) in enumerate(reqs):     # device and host witnesses interleaved
+        if k % 2:
+            tickets.append(srv.submit(circs[ci], w, r, s))
+        else:
+            tickets.append(srv.submit_device(circs[ci], d_ws[k], r, s))
+    for k, t in enumerate(tickets):
+        pr, _ = srv.wait(t)
+        ci, w, r, s = reqs[k]
+        assert np.array_equal(pr.words(), ctx.prove_device(circs[ci], d_ws[k], r, s).words()), k
+    srv.close()
+    for p in d_ws:
+        ctx.dev_free(p)
+
+
 def test_bad_request_is_isolated(setup):
     ctx, specs, circs = setup
     srv = zk.Server(ctx, max_slots=2, reserve_bytes=1 << 30)

@@ -62,6 +62,15 @@ ABI = {
     "zk_prove": (ctypes.c_int, [_vp, _vp, _u64p, ctypes.c_uint32, _u64p, _u64p, ctypes.POINTER(Proof)]),
     "zk_prove_device": (ctypes.c_int, [_vp, _vp, _vp, _u64p, _u64p, ctypes.POINTER(Proof)]),
     "zk_set_windows": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "zk_autotune": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "zk_tune_get": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
+    "zk_tune_load": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
+    "zk_tune_save": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "zk_batch_graph_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int,
+                                             ctypes.POINTER(_vp)]),
+    "zk_batch_graph_run": (ctypes.c_int, [_vp, _u64p, _u64p, ctypes.POINTER(Proof)]),
+    "zk_batch_graph_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int)]),
+    "zk_batch_graph_destroy": (None, [_vp]),
     "zk_prove_batch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_u64p), _u64p, _u64p,
                                       ctypes.POINTER(Proof)]),
     "zk_prove_batch_device": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _u64p, _u64p,
@@ -72,6 +81,7 @@ ABI = {
     "zk_server_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "zk_server_destroy": (None, [_vp]),
     "zk_server_submit": (ctypes.c_int, [_vp, _vp, _u64p, ctypes.c_uint32, _u64p, _u64p, ctypes.POINTER(ctypes.c_uint64)]),
+    "zk_server_submit_device": (ctypes.c_int, [_vp, _vp, _vp, _u64p, _u64p, ctypes.POINTER(ctypes.c_uint64)]),
     "zk_server_poll": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(Proof),
                                       ctypes.POINTER(ctypes.c_double)]),
     "zk_server_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(Proof), ctypes.POINTER(ctypes.c_double)]),
@@ -101,6 +111,7 @@ ABI = {
     "zk_msm_g2_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_int, _u64p,
                                         ctypes.POINTER(ctypes.c_int)]),
     "zk_last_timing": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
+    "zk_circuit_bases": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_uint64)]),
     "zk_synth_circuit": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_vp)]),
     "zk_synth_witness": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, _u64p]),
     "zk_circuit_witness": (ctypes.c_int, [_vp, ctypes.c_uint64, _u64p]),
@@ -208,9 +219,10 @@ class Context:
         return h
 
     def last_timing(self):
-        ms = (ctypes.c_float * 2)()
+        """(total_ms, sort_ms, accumulation_kernel_ms, entries) of the last kernel-level call."""
+        ms = (ctypes.c_float * 4)()
         _check(lib().zk_last_timing(self.h, ms))
-        return float(ms[0]), float(ms[1])
+        return float(ms[0]), float(ms[1]), float(ms[2]), float(ms[3])
 
     def set_windows(self, w_witness=0, w_h=0):
         _check(lib().zk_set_windows(self.h, w_witness, w_h))
@@ -311,6 +323,30 @@ class Context:
     def set_batch_slots(self, n):
         _check(lib().zk_set_batch_slots(self.h, n))
 
+    def autotune(self, circuit, force=False):
+        """Measure and record the fastest MSM windows for circuit's sizes; returns (c_witness, c_h)."""
+        w = (ctypes.c_int * 2)()
+        _check(lib().zk_autotune(self.h, circuit.h, 1 if force else 0, w))
+        return int(w[0]), int(w[1])
+
+    def tuned_window(self, kind, n):
+        """Window a proof would use for an MSM of kind (0 witness, 1 h) and size n."""
+        c = ctypes.c_int(0)
+        _check(lib().zk_tune_get(self.h, kind, n, ctypes.byref(c)))
+        return c.value
+
+    def tune_load(self, path):
+        k = ctypes.c_int(0)
+        _check(lib().zk_tune_load(self.h, str(path).encode(), ctypes.byref(k)))
+        return k.value
+
+    def tune_save(self, path):
+        _check(lib().zk_tune_save(self.h, str(path).encode()))
+
+    def batch_graph(self, circuits, d_witnesses, slots=3):
+        """Record a fixed batch of proofs as one HIP graph (BatchGraph)."""
+        return BatchGraph(self, circuits, d_witnesses, slots)
+
     def set_profiling(self, on=True):
         _check(lib().zk_set_profiling(self.h, 1 if on else 0))
 
@@ -324,6 +360,38 @@ class Context:
         buf = (ctypes.c_double * 24)()
         _check(lib().zk_prove_stats(self.h, buf, 24))
         return dict(zip(self.STAT_NAMES, [float(x) for x in buf]))
+
+
+class BatchGraph:
+    """A fixed sequence of proofs captured once as a HIP graph (zk_batch_graph_*)."""
+
+    def __init__(self, ctx, circuits, d_witnesses, slots=3):
+        n = len(circuits)
+        ch = (_vp * n)(*[c.h for c in circuits])
+        wp = (_vp * n)(*d_witnesses)
+        h = _vp()
+        _check(lib().zk_batch_graph_create(ctx.h, n, ch, wp, slots, ctypes.byref(h)))
+        self.h, self.ctx, self.count = h, ctx, n
+        self._keep = (list(circuits), list(d_witnesses))
+        nodes, ns = ctypes.c_uint64(0), ctypes.c_int(0)
+        _check(lib().zk_batch_graph_info(h, ctypes.byref(nodes), ctypes.byref(ns)))
+        self.n_nodes, self.slots = nodes.value, ns.value
+
+    def run(self, r=None, s=None):
+        rr = np.ascontiguousarray(r, dtype=np.uint64).reshape(-1) if r is not None else None
+        ss = np.ascontiguousarray(s, dtype=np.uint64).reshape(-1) if s is not None else None
+        out = (Proof * self.count)()
+        _check(lib().zk_batch_graph_run(self.h, u64p(rr) if rr is not None else None,
+                                        u64p(ss) if ss is not None else None, out))
+        return list(out)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().zk_batch_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Server:
@@ -351,6 +419,14 @@ class Server:
         t = ctypes.c_uint64(0)
         _check(lib().zk_server_submit(self.h, circuit.h, u64p(w), w.size // 4, u64p(r4) if r4 is not None else None,
                                       u64p(s4) if s4 is not None else None, ctypes.byref(t)))
+        return t.value
+
+    def submit_device(self, circuit, d_witness, r=None, s=None):
+        """Queue a proof whose witness is already in device memory (kept valid until done)."""
+        r4, s4 = _opt4(r), _opt4(s)
+        t = ctypes.c_uint64(0)
+        _check(lib().zk_server_submit_device(self.h, circuit.h, d_witness, u64p(r4) if r4 is not None else None,
+                                             u64p(s4) if s4 is not None else None, ctypes.byref(t)))
         return t.value
 
     def poll(self, ticket):
@@ -399,6 +475,13 @@ class Circuit:
 
     def write_zkey(self, path):
         _check(lib().zk_circuit_write_zkey(self.h, path.encode()))
+
+    def bases(self, which):
+        """(device pointer, point count) of a resident base array: 0 A', 1 B1', 2 B2', 3 C', 4 H."""
+        p = _vp()
+        n = ctypes.c_uint64(0)
+        _check(lib().zk_circuit_bases(self.h, which, ctypes.byref(p), ctypes.byref(n)))
+        return p, n.value
 
     def witness(self, pub_seed):
         """Satisfying witness of a synthetic circuit with public inputs from pub_seed."""

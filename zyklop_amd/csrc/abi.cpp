@@ -17,7 +17,7 @@ struct zk_ctx {
     zk::Context impl;
     std::vector<std::unique_ptr<zk::ProveSlot>> batch_slots;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
-    float last_ms[2] = {0, 0};
+    float last_ms[4] = {0, 0, 0, 0};
     bool profiling = false;
     int batch_slots_n = 3;
     explicit zk_ctx(int dev) : impl(dev) {
@@ -39,6 +39,11 @@ struct zk_circuit {
 
 struct zk_server {
     std::unique_ptr<zk::Server> impl;
+};
+
+struct zk_batch {
+    zk_ctx* ctx = nullptr;
+    std::unique_ptr<zk::BatchGraph> impl;
 };
 
 namespace {
@@ -85,13 +90,7 @@ zk::ProveSlot& ctx_slot(zk_ctx* ctx) {
 void run_batch(zk_ctx* ctx, int count, const zk_circuit* const* circuits, const uint64_t* const* witnesses, bool on_dev,
                const uint64_t* r, const uint64_t* s, zk_proof* out) {
     const int nslots = ctx->batch_slots_n;
-    while ((int)ctx->batch_slots.size() < nslots) {
-        ctx->batch_slots.push_back(std::make_unique<zk::ProveSlot>());
-        zk::ProveSlot& b = *ctx->batch_slots.back();
-        zk::ProveSlot& d = ctx_slot(ctx);
-        b.window_w = d.window_w;
-        b.window_h = d.window_h;
-    }
+    while ((int)ctx->batch_slots.size() < nslots) ctx->batch_slots.push_back(std::make_unique<zk::ProveSlot>());
     for (int i = 0; i < count; i++) need(circuits[i] && witnesses[i], "NULL circuit or witness");
     int next = 0;
     const int inflight = std::min(count, nslots);
@@ -126,6 +125,7 @@ void msm_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t
     zk::MsmSort sort;
     zk::MsmScratch scr;
     zk::MsmPartials part;
+    scr.enable_timing();
     ZK_HIP(hipEventRecord(ctx->ev0, st));
     sort.run(d_scalars, n, window, st);
     ZK_HIP(hipEventRecord(ctx->ev1, st));
@@ -133,7 +133,11 @@ void msm_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t
     ZK_HIP(hipEventRecord(ctx->ev2, st));
     ZK_HIP(hipStreamSynchronize(st));
     ZK_HIP(hipEventElapsedTime(&ctx->last_ms[0], ctx->ev0, ctx->ev2));
-    ZK_HIP(hipEventElapsedTime(&ctx->last_ms[1], ctx->ev1, ctx->ev2));
+    ZK_HIP(hipEventElapsedTime(&ctx->last_ms[1], ctx->ev0, ctx->ev1));
+    ZK_HIP(hipEventElapsedTime(&ctx->last_ms[2], scr.ev_acc0, scr.ev_acc1));
+    uint32_t entries = 0;
+    ZK_HIP(hipMemcpy(&entries, sort.offsets.as<uint32_t>() + (size_t)sort.g.W * sort.g.K, 4, hipMemcpyDeviceToHost));
+    ctx->last_ms[3] = (float)entries;
     if (g2) put_result(zk::msm_finish_g2(part), out, is_inf);
     else put_result(zk::msm_finish_g1(part), out, is_inf);
 }
@@ -149,7 +153,7 @@ void ntt_device(zk_ctx* ctx, void* const* in, void* const* out, int batch, int l
     ZK_HIP(hipEventRecord(ctx->ev2, C.stream));
     ZK_HIP(hipStreamSynchronize(C.stream));
     ZK_HIP(hipEventElapsedTime(&ctx->last_ms[0], ctx->ev0, ctx->ev2));
-    ctx->last_ms[1] = ctx->last_ms[0];
+    ctx->last_ms[1] = ctx->last_ms[2] = ctx->last_ms[3] = 0;
 }
 }  // namespace
 
@@ -233,11 +237,90 @@ int zk_set_windows(zk_ctx* ctx, int window_witness, int window_h) {
         need(window_witness == 0 || (window_witness >= 4 && window_witness <= 22), "window must be 0 or 4..22");
         need(window_h == 0 || (window_h >= 4 && window_h <= 22), "window must be 0 or 4..22");
         std::lock_guard<std::mutex> lk(ctx->impl.mu);
-        zk::ProveSlot& sl = ctx_slot(ctx);
-        sl.window_w = window_witness;
-        sl.window_h = window_h;
-        for (auto& b : ctx->batch_slots) { b->window_w = window_witness; b->window_h = window_h; }
+        ctx->impl.force_w = window_witness;
+        ctx->impl.force_h = window_h;
     });
+}
+
+int zk_autotune(zk_ctx* ctx, const zk_circuit* c, int force, int windows[2]) {
+    return guard([&] {
+        need(ctx && c, "NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        int w[2] = {0, 0};
+        ctx->impl.autotune(*c->impl, force != 0, 2, w);
+        if (windows) { windows[0] = w[0]; windows[1] = w[1]; }
+    });
+}
+
+int zk_tune_get(zk_ctx* ctx, int kind, uint64_t n, int* window) {
+    return guard([&] {
+        need(ctx && window, "NULL argument");
+        need(kind == 0 || kind == 1, "kind must be 0 (witness) or 1 (h)");
+        need(n >= 1, "n must be >= 1");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        *window = ctx->impl.window_for(kind, n);
+    });
+}
+
+int zk_tune_load(zk_ctx* ctx, const char* path, int* n_entries) {
+    return guard([&] {
+        need(ctx && path, "NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        int k = ctx->impl.tune_load(path);
+        if (n_entries) *n_entries = k;
+    });
+}
+
+int zk_tune_save(zk_ctx* ctx, const char* path) {
+    return guard([&] {
+        need(ctx && path, "NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ctx->impl.tune_save(path);
+    });
+}
+
+int zk_batch_graph_create(zk_ctx* ctx, int count, const zk_circuit* const* circuits, const void* const* d_witnesses,
+                          int slots, zk_batch** out) {
+    return guard([&] {
+        need(ctx && circuits && d_witnesses && out, "NULL argument");
+        need(count >= 1, "count must be >= 1");
+        *out = nullptr;
+        std::vector<const zk::Circuit*> cs(count);
+        for (int i = 0; i < count; i++) {
+            need(circuits[i] != nullptr, "NULL circuit");
+            cs[i] = circuits[i]->impl.get();
+        }
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        auto b = std::make_unique<zk_batch>();
+        b->ctx = ctx;
+        b->impl = std::make_unique<zk::BatchGraph>(ctx->impl, count, cs.data(), d_witnesses, slots);
+        *out = b.release();
+    });
+}
+
+int zk_batch_graph_run(zk_batch* b, const uint64_t* r, const uint64_t* s, zk_proof* out) {
+    return guard([&] {
+        need(b && out, "NULL argument");
+        std::lock_guard<std::mutex> lk(b->ctx->impl.mu);
+        ZK_HIP(hipSetDevice(b->ctx->impl.device));
+        b->impl->run(r, s, out);
+    });
+}
+
+int zk_batch_graph_info(const zk_batch* b, uint64_t* n_nodes, int* n_slots) {
+    return guard([&] {
+        need(b != nullptr, "NULL argument");
+        if (n_nodes) *n_nodes = b->impl->n_nodes;
+        if (n_slots) *n_slots = (int)b->impl->slots.size();
+    });
+}
+
+void zk_batch_graph_destroy(zk_batch* b) {
+    if (!b) return;
+    std::lock_guard<std::mutex> lk(b->ctx->impl.mu);
+    (void)hipSetDevice(b->ctx->impl.device);
+    delete b;
 }
 
 int zk_prove_files(zk_ctx* ctx, const char* zkey_path, const char* wtns_path, const uint64_t r[4], const uint64_t s[4],
@@ -332,6 +415,14 @@ int zk_server_submit(zk_server* s, const zk_circuit* c, const uint64_t* witness,
     return guard([&] {
         need(s && c && witness && ticket, "NULL argument");
         *ticket = s->impl->submit(*c->impl, witness, n_witness, r, sb);
+    });
+}
+
+int zk_server_submit_device(zk_server* s, const zk_circuit* c, const void* d_witness, const uint64_t r[4],
+                            const uint64_t sb[4], uint64_t* ticket) {
+    return guard([&] {
+        need(s && c && d_witness && ticket, "NULL argument");
+        *ticket = s->impl->submit_device(*c->impl, d_witness, r, sb);
     });
 }
 
@@ -612,11 +703,21 @@ int zk_dev_download(zk_ctx* ctx, void* dst, const void* src, size_t bytes) {
     });
 }
 
-int zk_last_timing(zk_ctx* ctx, float ms[2]) {
+int zk_last_timing(zk_ctx* ctx, float ms[4]) {
     return guard([&] {
         need(ctx && ms, "NULL argument");
-        ms[0] = ctx->last_ms[0];
-        ms[1] = ctx->last_ms[1];
+        for (int i = 0; i < 4; i++) ms[i] = ctx->last_ms[i];
+    });
+}
+
+int zk_circuit_bases(const zk_circuit* c, int which, void** d_ptr, uint64_t* count) {
+    return guard([&] {
+        need(c && d_ptr && count, "NULL argument");
+        const zk::Circuit& k = *c->impl;
+        const zk::DevBuf* b[5] = {&k.A, &k.B1, &k.B2, &k.C, &k.H};
+        need(which >= 0 && which <= 4, "which must be 0..4 (A, B1, B2, C, H)");
+        *d_ptr = b[which]->p;
+        *count = which == 4 ? k.n : (uint64_t)k.n_vars + 4;
     });
 }
 

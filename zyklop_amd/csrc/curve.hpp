@@ -23,14 +23,14 @@ ZK_DEV Fq2 sub(const Fq2& a, const Fq2& b) { return {sub<FpCfg>(a.c0, b.c0), sub
 ZK_DEV Fq2 neg(const Fq2& a) { return {neg<FpCfg>(a.c0), neg<FpCfg>(a.c1)}; }
 // Karatsuba: 3 Fp products (include/fp2_bn254.cuh:130-142 uses the same count).
 // Kept out of line: G2 formulas call it a dozen times per point operation.
-ZK_NOINL Fq2 mul(const Fq2& a, const Fq2& b) {
+ZK_DEV Fq2 mul(const Fq2& a, const Fq2& b) {
     Fp t0 = mul<FpCfg>(a.c0, b.c0);
     Fp t1 = mul<FpCfg>(a.c1, b.c1);
     Fp t2 = mul<FpCfg>(add<FpCfg>(a.c0, a.c1), add<FpCfg>(b.c0, b.c1));
     return {sub<FpCfg>(t0, t1), sub<FpCfg>(sub<FpCfg>(t2, t0), t1)};
 }
 // (a0 + a1 u)^2 = (a0+a1)(a0-a1) + 2 a0 a1 u : 2 Fp products
-ZK_NOINL Fq2 sqr(const Fq2& a) {
+ZK_DEV Fq2 sqr(const Fq2& a) {
     Fp t = mul<FpCfg>(a.c0, a.c1);
     return {mul<FpCfg>(add<FpCfg>(a.c0, a.c1), sub<FpCfg>(a.c0, a.c1)), add<FpCfg>(t, t)};
 }

@@ -104,6 +104,45 @@ __global__ void k_msm_scatter(const uint32_t* __restrict__ sc, size_t n, int c, 
     }
 }
 
+// accumulation-order key of bucket b: 0 = oversized, then descending load,
+// nkey-1 = empty
+__device__ inline uint32_t load_key(const uint32_t* __restrict__ off, uint32_t b, uint32_t lmax, int qs, uint32_t nkey) {
+    uint32_t len = off[b + 1] - off[b];
+    return len > lmax ? 0u : (nkey - 1) - (len >> qs);
+}
+
+__global__ void __launch_bounds__(1024) k_load_hist(const uint32_t* __restrict__ off, uint32_t nb, uint32_t lmax, int qs,
+                                                    uint32_t nkey, uint32_t* __restrict__ hist) {
+    extern __shared__ uint32_t lh[];
+    for (uint32_t k = threadIdx.x; k < nkey; k += blockDim.x) lh[k] = 0;
+    __syncthreads();
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nb) atomicAdd(&lh[load_key(off, b, lmax, qs, nkey)], 1u);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nkey; k += blockDim.x)
+        if (lh[k]) atomicAdd(&hist[k], lh[k]);
+}
+
+// each block reserves one range per key, then ranks its buckets inside it in LDS
+__global__ void __launch_bounds__(1024) k_load_scatter(const uint32_t* __restrict__ off, uint32_t nb, uint32_t lmax, int qs,
+                                                       uint32_t nkey, uint32_t* __restrict__ cursor,
+                                                       uint32_t* __restrict__ order) {
+    extern __shared__ uint32_t lh[];
+    for (uint32_t k = threadIdx.x; k < nkey; k += blockDim.x) lh[k] = 0;
+    __syncthreads();
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t key = 0, rank = 0;
+    if (b < nb) {
+        key = load_key(off, b, lmax, qs, nkey);
+        rank = atomicAdd(&lh[key], 1u);
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nkey; k += blockDim.x)
+        if (lh[k]) lh[k] = atomicAdd(&cursor[k], lh[k]);
+    __syncthreads();
+    if (b < nb) order[lh[key] + rank] = b;
+}
+
 void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
     ZK_REQUIRE(n_ < 0x80000000ull, ZK_E_ARG, "MSM size must be < 2^31");
     n = n_;
@@ -126,20 +165,38 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
                                                    entries.as<uint32_t>());
         ZK_HIP(hipGetLastError());
     }
+    // accumulation order. Oversized threshold: 8x the mean bucket load
+    // (uniform scalars put about n/K entries in a bucket).
+    lmax = std::max<uint32_t>(L_MIN_LARGE, 8u * (uint32_t)((n + g.K - 1) / g.K));
+    qs = 0;
+    while ((lmax >> qs) + 2 > 4096) qs++;
+    nkey = (lmax >> qs) + 2;
+    order.ensure(nb * 4);
+    lkeys.ensure((size_t)4 * (2 * nkey + cdiv(nkey, SCAN_TILE) + 1));
+    uint32_t* hist = lkeys.as<uint32_t>();
+    uint32_t* cur = hist + nkey;
+    fill_u32(hist, 0u, nkey, st);
+    const unsigned nblk_o = (unsigned)cdiv(nb, 1024);
+    k_load_hist<<<nblk_o, 1024, nkey * 4, st>>>(off, (uint32_t)nb, lmax, qs, nkey, hist);
+    ZK_HIP(hipGetLastError());
+    exclusive_scan(hist, cur, cur + nkey, nkey, st);
+    k_load_scatter<<<nblk_o, 1024, nkey * 4, st>>>(off, (uint32_t)nb, lmax, qs, nkey, cur, order.as<uint32_t>());
+    ZK_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------- buckets
-// One thread per bucket, highest bucket index first: the top window holds
-// fewer scalar bits, so its buckets are the fullest and should not form the
-// tail of the launch. Buckets above lmax entries go to the chunked path.
+// One thread per bucket, in the sort's accumulation order (longest first, so
+// a wave's lanes carry near-equal loads and the launch has no long tail).
+// Buckets above lmax entries go to the chunked path.
 template <class T>
 __global__ void __launch_bounds__(256) k_msm_accum(const uint32_t* __restrict__ entries, const uint32_t* __restrict__ offsets,
+                                                   const uint32_t* __restrict__ order,
                                                    uint32_t nb, uint32_t lmax, const void* __restrict__ points,
                                                    void* __restrict__ buckets, uint32_t* __restrict__ counters,
                                                    ChunkRec* __restrict__ chunks, LargeRec* __restrict__ large) {
     uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= nb) return;
-    uint32_t b = nb - 1 - tid;
+    uint32_t b = order[tid];
     uint32_t st = offsets[b], en = offsets[b + 1];
     uint32_t cnt = en - st;
     if (cnt > lmax) {
@@ -335,9 +392,8 @@ static void launch_t(const MsmSort& sort, const void* d_points, MsmScratch& scr,
     const uint32_t nb = (uint32_t)((size_t)g.W * g.K);
     fill_u32(scr.counters.p, 0u, 16, st);
     if (scr.ev_acc0) ZK_HIP(hipEventRecord(scr.ev_acc0, st));
-    // threshold: 8x the mean bucket load (uniform scalars put n/K entries in a bucket)
-    const uint32_t lmax = std::max<uint32_t>(L_MIN_LARGE, 8u * (uint32_t)((sort.n + g.K - 1) / g.K));
-    k_msm_accum<T><<<cdiv(nb, 128), 128, 0, st>>>(sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(), nb, lmax, d_points,
+    k_msm_accum<T><<<cdiv(nb, 128), 128, 0, st>>>(sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(),
+                                                  sort.order.as<uint32_t>(), nb, sort.lmax, d_points,
                                                   scr.buckets.p, scr.counters.as<uint32_t>(), scr.chunks.as<ChunkRec>(),
                                                   scr.large.as<LargeRec>());
     ZK_HIP(hipGetLastError());

@@ -34,10 +34,16 @@ int msm_default_c(size_t n);
 MsmGeom msm_geom(size_t n, int c, bool g2);
 
 // Per-scalar-vector digit sort (shared by the MSMs over the same scalars).
+// Besides the entries in bucket order it produces the accumulation order: all
+// W*K bucket ids sorted by descending load (quantised by 2^qs), so that the
+// lanes of one wave walk buckets of nearly equal length. Buckets above lmax
+// entries take the chunked path and come first.
 struct MsmSort {
     MsmGeom g{};
     size_t n = 0;
-    DevBuf counts, offsets, entries;
+    uint32_t lmax = 0, nkey = 0;
+    int qs = 0;
+    DevBuf counts, offsets, entries, order, lkeys;
     // sorts canonical scalars (n x 32 B, device) into bucket order
     void run(const void* d_scalars, size_t n, int c, hipStream_t st);
 };

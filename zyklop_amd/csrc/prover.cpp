@@ -1,5 +1,7 @@
 #include "prover.hpp"
 
+#include <thread>
+
 #include <string.h>
 
 #include <algorithm>
@@ -229,6 +231,7 @@ ProveSlot::ProveSlot() {
     for (hipStream_t* p : {&s0, &s1, &s2, &s3, &s4}) ZK_HIP(hipStreamCreateWithFlags(p, hipStreamNonBlocking));
     ZK_HIP(hipEventCreateWithFlags(&ev_scal, hipEventDisableTiming));
     ZK_HIP(hipEventCreateWithFlags(&ev_done1, hipEventDisableTiming));
+    for (auto& e : ev_join) ZK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
 void ProveSlot::enable_timing() {
@@ -245,6 +248,8 @@ ProveSlot::~ProveSlot() {
         if (p) { (void)hipStreamSynchronize(p); (void)hipStreamDestroy(p); }
     if (ev_scal) (void)hipEventDestroy(ev_scal);
     if (ev_done1) (void)hipEventDestroy(ev_done1);
+    for (auto& e : ev_join)
+        if (e) (void)hipEventDestroy(e);
 }
 
 size_t ProveSlot::device_bytes() const {
@@ -259,15 +264,30 @@ size_t ProveSlot::device_bytes() const {
 
 static bool is_zero4(const uint64_t* v) { return !(v[0] | v[1] | v[2] | v[3]); }
 
+// the four scalars after the witness: [1, r, s, -r*s], canonical
+static void blinding_tail(const uint64_t r[4], const uint64_t s[4], uint64_t ex[16]) {
+    memset(ex, 0, 16 * 8);
+    ex[0] = 1;
+    memcpy(ex + 4, r, 32);
+    memcpy(ex + 8, s, 32);
+    Fr rm = Fr::to_mont(Fr::from_raw(r)), sm = Fr::to_mont(Fr::from_raw(s));
+    (-(rm * sm)).canonical(ex + 12);
+}
+
+void ProveSlot::prepare(Context& ctx, const Circuit& c) {
+    const size_t nS = (size_t)c.n_vars + 4, n = c.n;
+    ctx.ntt_plan(c.log_n);
+    scal.ensure(nS * 32);
+    for (DevBuf* b : {&aw, &bw, &cw, &t0, &t1, &t2, &h}) b->ensure(n * 32);
+}
+
 void ProveSlot::enqueue(Context& ctx, const Circuit& c, const uint64_t* witness, bool on_dev, const uint64_t r_[4],
                         const uint64_t s_[4]) {
     circ = &c;
     for (int i = 0; i < 4; i++) { r[i] = r_ ? r_[i] : 0; s[i] = s_ ? s_[i] : 0; }
     ZK_REQUIRE(Fr::is_canonical(r) && Fr::is_canonical(s), ZK_E_ARG, "blinding scalars must be canonical (< r)");
-    const size_t nv = c.n_vars, nS = nv + 4, n = c.n;
-    NttPlan& plan = ctx.ntt_plan(c.log_n);
-    scal.ensure(nS * 32);
-    for (DevBuf* b : {&aw, &bw, &cw, &t0, &t1, &t2, &h}) b->ensure(n * 32);
+    const size_t nv = c.n_vars;
+    prepare(ctx, c);
     if (timed) ZK_HIP(hipEventRecord(ev[0], s0));
     // scalars: witness, then [1, r, s, -r*s]
     if (on_dev) {
@@ -279,12 +299,17 @@ void ProveSlot::enqueue(Context& ctx, const Circuit& c, const uint64_t* witness,
         memcpy(staging.p, witness, nv * 32);
         ZK_HIP(hipMemcpyAsync(scal.p, staging.p, nv * 32, hipMemcpyHostToDevice, s0));
     }
-    uint64_t ex[16] = {1, 0, 0, 0};
-    memcpy(ex + 4, r, 32);
-    memcpy(ex + 8, s, 32);
-    Fr rm = Fr::to_mont(Fr::from_raw(r)), sm = Fr::to_mont(Fr::from_raw(s));
-    (-(rm * sm)).canonical(ex + 12);
+    uint64_t ex[16];
+    blinding_tail(r, s, ex);
     write_scalars4((char*)scal.p + nv * 32, ex, s0);
+    body(ctx, c, parts);
+}
+
+void ProveSlot::body(Context& ctx, const Circuit& c, ProofParts& out) {
+    const size_t nS = (size_t)c.n_vars + 4, n = c.n;
+    NttPlan& plan = ctx.ntt_plan(c.log_n);
+    window_w = ctx.window_for(0, nS);
+    window_h = ctx.window_for(1, n);
     ZK_HIP(hipEventRecord(ev_scal, s0));
     // H pipeline on s0
     spmv_abc(c.rowA.as<uint32_t>(), c.colA.as<uint32_t>(), c.valA.p, c.n_rowsA, c.rowB.as<uint32_t>(),
@@ -298,7 +323,7 @@ void ProveSlot::enqueue(Context& ctx, const Circuit& c, const uint64_t* witness,
     if (timed) ZK_HIP(hipEventRecord(ev[3], s0));
     h_final(aw.p, bw.p, cw.p, h.p, n, s0);
     sortH.run(h.p, n, window_h, s0);
-    msm_launch(sortH, c.H.p, false, scrH, pH, s0);
+    msm_launch(sortH, c.H.p, false, scrH, out.h, s0);
     // witness MSMs on s1 (one shared sort)
     ZK_HIP(hipStreamWaitEvent(s1, ev_scal, 0));
     if (timed) ZK_HIP(hipEventRecord(ev[4], s1));
@@ -308,10 +333,15 @@ void ProveSlot::enqueue(Context& ctx, const Circuit& c, const uint64_t* witness,
     // the four witness MSMs read the same sorted entries on their own streams:
     // one MSM's latency-bound reduction overlaps another's accumulation
     for (hipStream_t p : {s2, s3, s4}) ZK_HIP(hipStreamWaitEvent(p, ev_done1, 0));
-    msm_launch(sortW, c.B2.p, true, scrB2, pB2, s4);
-    msm_launch(sortW, c.A.p, false, scrA, pA, s1);
-    msm_launch(sortW, c.B1.p, false, scrB1, pB1, s2);
-    msm_launch(sortW, c.C.p, false, scrC, pC, s3);
+    msm_launch(sortW, c.B2.p, true, scrB2, out.b2, s4);
+    msm_launch(sortW, c.A.p, false, scrA, out.a, s1);
+    msm_launch(sortW, c.B1.p, false, scrB1, out.b1, s2);
+    msm_launch(sortW, c.C.p, false, scrC, out.c, s3);
+    hipStream_t br[4] = {s1, s2, s3, s4};
+    for (int k = 0; k < 4; k++) {
+        ZK_HIP(hipEventRecord(ev_join[k], br[k]));
+        ZK_HIP(hipStreamWaitEvent(s0, ev_join[k], 0));
+    }
 }
 
 bool ProveSlot::device_done() const {
@@ -328,7 +358,7 @@ size_t ProveSlot::bytes_needed(const Circuit& c) {
     MsmGeom gw = msm_geom(nS, 0, false), gh = msm_geom(n, 0, false);
     size_t b = nS * 32 + 7 * n * 32;                                    // scalars + Fr arrays
     b += (size_t)gw.W * nS * 4 + (size_t)gh.W * n * 4;                  // sorted entries
-    b += 3 * 4 * ((size_t)gw.W * gw.K + (size_t)gh.W * gh.K);           // counts / offsets
+    b += 4 * 4 * ((size_t)gw.W * gw.K + (size_t)gh.W * gh.K);           // counts / offsets / order
     size_t bw = (size_t)gw.W * gw.K, bh = (size_t)gh.W * gh.K;
     b += 3 * bw * 128 + bw * 256 + bh * 128;                            // buckets A, B1, C, B2, H
     b += (3 * bw + bw * 2 + bh) * 128 / 4;                              // reduction partials (upper bound)
@@ -345,25 +375,26 @@ static void put_aff(const zkh::Xyzz<T>& p, uint64_t* dst) {
     memcpy(dst + lw, &a.y, 8 * lw);
 }
 
-void ProveSlot::finish(zk_proof* out) {
-    using clk = std::chrono::steady_clock;
-    for (hipStream_t p : {s1, s2, s3, s4}) ZK_HIP(hipStreamSynchronize(p));
-    auto h0 = clk::now();
-    auto A = msm_finish_g1(pA);
-    auto B1 = msm_finish_g1(pB1);
-    auto Cw = msm_finish_g1(pC);
-    auto B2 = msm_finish_g2(pB2);
-    double host_ms = std::chrono::duration<double, std::milli>(clk::now() - h0).count();
-    ZK_HIP(hipStreamSynchronize(s0));
-    h0 = clk::now();
-    auto Hp = msm_finish_g1(pH);
+void assemble_proof(const ProofParts& p, const uint64_t r[4], const uint64_t s[4], zk_proof* out) {
+    auto A = msm_finish_g1(p.a);
+    auto B1 = msm_finish_g1(p.b1);
+    auto Cw = msm_finish_g1(p.c);
+    auto B2 = msm_finish_g2(p.b2);
+    auto Hp = msm_finish_g1(p.h);
     auto C = zkh::xyzz_add(Cw, Hp);
     if (!is_zero4(s)) C = zkh::xyzz_add(C, zkh::xyzz_mul(A, s));
     if (!is_zero4(r)) C = zkh::xyzz_add(C, zkh::xyzz_mul(B1, r));
     put_aff(A, out->pi_a);
     put_aff(B2, out->pi_b);
     put_aff(C, out->pi_c);
-    host_ms += std::chrono::duration<double, std::milli>(clk::now() - h0).count();
+}
+
+void ProveSlot::finish(zk_proof* out) {
+    using clk = std::chrono::steady_clock;
+    ZK_HIP(hipStreamSynchronize(s0));   // s1..s4 are joined into s0
+    auto h0 = clk::now();
+    assemble_proof(parts, r, s, out);
+    double host_ms = std::chrono::duration<double, std::milli>(clk::now() - h0).count();
     if (timed) {
         collect_stats();
         stats[23] = host_ms;
@@ -400,6 +431,115 @@ void ProveSlot::collect_stats() {
     stats[21] = el(ev[0], scrH.ev_end);   // stream 0 span
     stats[22] = 0;                         // witness-MSM span (latest of the four streams)
     for (MsmScratch* q : {&scrA, &scrB1, &scrC, &scrB2}) stats[22] = std::max(stats[22], el(ev[4], q->ev_end));
+}
+
+// ------------------------------------------------------------------ batch graph
+BatchGraph::BatchGraph(Context& c, int count, const Circuit* const* cs, const void* const* d_wits, int nslots)
+    : ctx(c) {
+    ZK_REQUIRE(count >= 1, ZK_E_ARG, "empty batch");
+    ZK_REQUIRE(nslots >= 1 && nslots <= 16, ZK_E_ARG, "slots must be 1..16");
+    for (int i = 0; i < count; i++) {
+        ZK_REQUIRE(cs[i] && d_wits[i], ZK_E_ARG, "NULL circuit or witness in batch");
+        circs.push_back(cs[i]);
+        wits.push_back(d_wits[i]);
+    }
+    const int S = std::min(nslots, count);
+    for (int k = 0; k < S; k++) slots.push_back(std::make_unique<ProveSlot>());
+    parts.reset(new ProofParts[count]);
+    params.ensure((size_t)count * 128);
+    ZK_HIP(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking));
+    ZK_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    ev_end.resize(S);
+    for (auto& e : ev_end) ZK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // one eager pass sizes every buffer (slots, sorts, scratch, partials, NTT
+    // plans) so that the captured pass allocates nothing
+    fill_params(nullptr, nullptr);
+    record(false);
+    ZK_HIP(hipStreamSynchronize(origin));
+    ZK_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal));
+    try {
+        record(true);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(origin, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    ZK_HIP(hipStreamEndCapture(origin, &graph));
+    ZK_HIP(hipGraphGetNodes(graph, nullptr, &n_nodes));
+    ZK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+}
+
+BatchGraph::~BatchGraph() {
+    if (origin) (void)hipStreamSynchronize(origin);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto& e : ev_end)
+        if (e) (void)hipEventDestroy(e);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    slots.clear();
+    if (origin) (void)hipStreamDestroy(origin);
+}
+
+void BatchGraph::fill_params(const uint64_t* r, const uint64_t* s) {
+    static const uint64_t zero[4] = {0, 0, 0, 0};
+    for (size_t i = 0; i < circs.size(); i++) {
+        const uint64_t* ri = r ? r + 4 * i : zero;
+        const uint64_t* si = s ? s + 4 * i : zero;
+        ZK_REQUIRE(Fr::is_canonical(ri) && Fr::is_canonical(si), ZK_E_ARG, "blinding scalars must be canonical (< r)");
+        blinding_tail(ri, si, params.as<uint64_t>() + 16 * i);
+    }
+}
+
+// queue the whole batch: proof i on slot i % S; a slot's proofs are ordered by
+// its s0 (each proof's end joins s1..s4 into s0)
+void BatchGraph::record(bool capture) {
+    const size_t S = slots.size();
+    ZK_HIP(hipEventRecord(ev_fork, origin));
+    for (auto& sl : slots) ZK_HIP(hipStreamWaitEvent(sl->s0, ev_fork, 0));
+    for (size_t i = 0; i < circs.size(); i++) {
+        ProveSlot& sl = *slots[i % S];
+        const Circuit& c = *circs[i];
+        if (!capture) {
+            // sizing pass: buffers may grow (free + allocate), so the slot's
+            // previous proof must have drained first
+            if (i >= S) ZK_HIP(hipStreamSynchronize(sl.s0));
+            sl.prepare(ctx, c);
+        }
+        const size_t nv = c.n_vars;
+        ZK_HIP(hipMemcpyAsync(sl.scal.p, wits[i], nv * 32, hipMemcpyDeviceToDevice, sl.s0));
+        ZK_HIP(hipMemcpyAsync((char*)sl.scal.p + nv * 32, params.as<uint64_t>() + 16 * i, 128, hipMemcpyHostToDevice,
+                              sl.s0));
+        sl.body(ctx, c, parts[i]);
+    }
+    for (size_t k = 0; k < S; k++) {
+        ZK_HIP(hipEventRecord(ev_end[k], slots[k]->s0));
+        ZK_HIP(hipStreamWaitEvent(origin, ev_end[k], 0));
+    }
+}
+
+void BatchGraph::run(const uint64_t* r, const uint64_t* s, zk_proof* out) {
+    fill_params(r, s);
+    ZK_HIP(hipGraphLaunch(exec, origin));
+    ZK_HIP(hipStreamSynchronize(origin));
+    // host tails in parallel (each proof's partials are its own)
+    const size_t count = circs.size();
+    unsigned nt = std::max(1u, std::min<unsigned>(8u, std::thread::hardware_concurrency()));
+    nt = (unsigned)std::min<size_t>(nt, count);
+    std::vector<std::thread> th;
+    std::vector<std::string> errs(nt);
+    for (unsigned t = 0; t < nt; t++)
+        th.emplace_back([&, t] {
+            try {
+                for (size_t i = t; i < count; i += nt)
+                    assemble_proof(parts[i], params.as<uint64_t>() + 16 * i + 4, params.as<uint64_t>() + 16 * i + 8,
+                                   &out[i]);
+            } catch (const std::exception& e) {
+                errs[t] = e.what();
+            }
+        });
+    for (auto& x : th) x.join();
+    for (auto& e : errs) ZK_REQUIRE(e.empty(), ZK_E_INTERNAL, e);
 }
 
 // ------------------------------------------------------------------ serialization

@@ -47,18 +47,28 @@ struct Circuit {
     size_t device_bytes() const;
 };
 
+// Device results of one proof's five MSMs (level-2 partials, host memory).
+struct ProofParts {
+    MsmPartials a, b1, c, b2, h;
+};
+
+// Host tail of a proof: finish the MSMs and assemble pi_A, pi_B,
+// pi_C = C' + H + s*pi_A + r*B1 (Montgomery affine).
+void assemble_proof(const ProofParts& p, const uint64_t r[4], const uint64_t s[4], zk_proof* out);
+
 struct ProveSlot {
-    // s0: H pipeline + H MSM; s1: witness sort + A; s2: B1; s3: C; s4: B2
+    // s0: H pipeline + H MSM; s1: witness sort + A; s2: B1; s3: C; s4: B2.
+    // Each proof forks s1..s4 from s0 and joins them back into s0 at its end.
     hipStream_t s0 = nullptr, s1 = nullptr, s2 = nullptr, s3 = nullptr, s4 = nullptr;
-    hipEvent_t ev_scal = nullptr, ev_done1 = nullptr;
+    hipEvent_t ev_scal = nullptr, ev_done1 = nullptr, ev_join[4] = {};
     DevBuf scal, aw, bw, cw, t0, t1, t2, h, wtmp;
     HostBuf staging;                 // pinned copy of a host witness (async upload)
     MsmSort sortW, sortH;
     MsmScratch scrA, scrB1, scrC, scrB2, scrH;
-    MsmPartials pA, pB1, pC, pB2, pH;
+    ProofParts parts;
     uint64_t r[4], s[4];
     const Circuit* circ = nullptr;
-    int window_w = 0, window_h = 0;  // 0 = default per size
+    int window_w = 0, window_h = 0;  // windows of the last proof
     // profiling (off by default): events on both streams, read after finish()
     bool timed = false;
     hipEvent_t ev[8] = {};
@@ -72,6 +82,12 @@ struct ProveSlot {
     // queue all device work for one proof; witness is host (canonical) or device
     void enqueue(Context& ctx, const Circuit& c, const uint64_t* witness, bool witness_on_device, const uint64_t r[4],
                  const uint64_t s[4]);
+    // size every device buffer a proof of c needs in this slot
+    void prepare(Context& ctx, const Circuit& c);
+    // the device work of one proof once scal holds [w | 1, r, s, -rs] (queued
+    // on s0 before the call): everything up to the partials in out, ending
+    // with s1..s4 joined into s0. Stream-capturable after prepare().
+    void body(Context& ctx, const Circuit& c, ProofParts& out);
     // wait for the device work and run the host tail
     void finish(zk_proof* out);
     // true when every stream of the slot has drained (non-blocking)
@@ -83,6 +99,9 @@ struct ProveSlot {
     size_t device_bytes() const;
 };
 
+constexpr int TUNE_C_MIN = 11, TUNE_C_MAX = 18;  // autotuner window range
+int size_class(size_t n);                         // ceil(log2 n)
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -90,10 +109,46 @@ struct Context {
     std::map<int, std::unique_ptr<NttPlan>> ntt_plans;
     std::unique_ptr<ProveSlot> slot;   // slot used by the one-shot prove entry point
     DevBuf comb_g1, comb_g2;           // fixed-base tables for synthetic setup
+    // MSM windows: forced (nonzero) or tuned per (kind, ceil log2 n), else the
+    // size default. kind 0 = witness sort (A', B1', C', B2'), 1 = H.
+    int force_w = 0, force_h = 0;
+    std::map<std::pair<int, int>, int> tuned;
     explicit Context(int dev);
     ~Context();
     NttPlan& ntt_plan(int log_n);
     void ensure_comb();
+    int window_for(int kind, size_t n) const;
+    std::string arch() const;
+    void autotune(const Circuit& c, bool force, int reps, int chosen[2]);
+    int tune_load(const std::string& path);
+    void tune_save(const std::string& path) const;
+};
+
+// A fixed sequence of proofs captured once as one HIP graph: every proof's
+// device work on a few slots (round-robin), witnesses read from caller device
+// buffers whose addresses stay fixed, blinding read from pinned memory that
+// run() fills, so the graph is replayed for new witness contents and new r, s.
+struct BatchGraph {
+    Context& ctx;
+    std::vector<const Circuit*> circs;
+    std::vector<const void*> wits;
+    std::vector<std::unique_ptr<ProveSlot>> slots;
+    std::unique_ptr<ProofParts[]> parts;  // one per proof
+    HostBuf params;                     // count x [1, r, s, -rs] canonical
+    hipStream_t origin = nullptr;
+    hipEvent_t ev_fork = nullptr;
+    std::vector<hipEvent_t> ev_end;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    size_t n_nodes = 0;
+    BatchGraph(Context& ctx, int count, const Circuit* const* circs, const void* const* d_wits, int nslots);
+    ~BatchGraph();
+    // r, s: count x 4 words each (NULL = zero blinding); out: count proofs
+    void run(const uint64_t* r, const uint64_t* s, zk_proof* out);
+
+  private:
+    void fill_params(const uint64_t* r, const uint64_t* s);
+    void record(bool capture);
 };
 
 std::unique_ptr<Circuit> circuit_from_zkey(Context& ctx, const std::string& path);

@@ -33,6 +33,19 @@ uint64_t Server::submit(const Circuit& c, const uint64_t* w, uint32_t nw, const 
     for (uint32_t i = 0; i < c.n_vars; i++)
         ZK_REQUIRE(zkh::Fr::is_canonical(w + 4ull * i), ZK_E_ARG, "witness value " + std::to_string(i) + " is not canonical");
     auto req = std::make_shared<ServerRequest>();
+    req->witness.assign(w, w + 4ull * c.n_vars);
+    return admit_request(std::move(req), c, r, s);
+}
+
+uint64_t Server::submit_device(const Circuit& c, const void* d_w, const uint64_t* r, const uint64_t* s) {
+    ZK_REQUIRE(d_w != nullptr, ZK_E_ARG, "NULL device witness");
+    auto req = std::make_shared<ServerRequest>();
+    req->d_witness = d_w;
+    return admit_request(std::move(req), c, r, s);
+}
+
+uint64_t Server::admit_request(std::shared_ptr<ServerRequest> req, const Circuit& c, const uint64_t* r,
+                               const uint64_t* s) {
     if (r) memcpy(req->r, r, 32);
     if (s) memcpy(req->s, s, 32);
     ZK_REQUIRE(zkh::Fr::is_canonical(req->r) && zkh::Fr::is_canonical(req->s), ZK_E_ARG, "blinding must be canonical");
@@ -44,7 +57,6 @@ uint64_t Server::submit(const Circuit& c, const uint64_t* w, uint32_t nw, const 
         throw Error(ZK_E_NOMEM, "request needs more device memory than the GPU has");
     }
     req->circ = &c;
-    req->witness.assign(w, w + 4ull * c.n_vars);
     req->t_submit = clk::now();
     uint64_t t;
     {
@@ -154,7 +166,10 @@ bool Server::try_admit(std::unique_lock<std::mutex>& lk) {
         std::string err;
         try {
             std::lock_guard<std::mutex> g(ctx_.mu);
-            slots_[pick]->enqueue(ctx_, *req->circ, req->witness.data(), false, req->r, req->s);
+            if (req->d_witness)
+                slots_[pick]->enqueue(ctx_, *req->circ, (const uint64_t*)req->d_witness, true, req->r, req->s);
+            else
+                slots_[pick]->enqueue(ctx_, *req->circ, req->witness.data(), false, req->r, req->s);
         } catch (const Error& e) {
             st = e.code;
             err = e.what();

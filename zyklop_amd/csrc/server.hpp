@@ -30,7 +30,8 @@ namespace zk {
 struct ServerRequest {
     uint64_t ticket = 0;
     const Circuit* circ = nullptr;
-    std::vector<uint64_t> witness;
+    std::vector<uint64_t> witness;      // host copy (empty for a device witness)
+    const void* d_witness = nullptr;    // caller's device witness
     uint64_t r[4] = {0, 0, 0, 0}, s[4] = {0, 0, 0, 0};
     std::chrono::steady_clock::time_point t_submit, t_start, t_done;
     zk_proof proof{};
@@ -44,6 +45,9 @@ class Server {
     Server(Context& ctx, int max_slots, size_t reserve_bytes);
     ~Server();
     uint64_t submit(const Circuit& c, const uint64_t* w, uint32_t nw, const uint64_t* r, const uint64_t* s);
+    // witness already in device memory (n_vars x 32 B, canonical), read when
+    // the request runs: it must stay valid and unchanged until completion
+    uint64_t submit_device(const Circuit& c, const void* d_w, const uint64_t* r, const uint64_t* s);
     // done: copies the result out and forgets the request; returns false while pending
     bool poll(uint64_t ticket, zk_proof* out, double times[3], int* status, std::string* err);
     void wait(uint64_t ticket, zk_proof* out, double times[3], int* status, std::string* err);
@@ -51,6 +55,7 @@ class Server {
     void stats(double out[8]);
 
   private:
+    uint64_t admit_request(std::shared_ptr<ServerRequest> req, const Circuit& c, const uint64_t* r, const uint64_t* s);
     void loop();
     bool try_admit(std::unique_lock<std::mutex>& lk);
     Context& ctx_;
