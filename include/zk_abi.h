@@ -214,6 +214,8 @@ int zk_dev_alloc(zk_ctx* ctx, size_t bytes, void** out);
 int zk_dev_free(zk_ctx* ctx, void* p);
 int zk_dev_upload(zk_ctx* ctx, void* dst, const void* src, size_t bytes);
 int zk_dev_download(zk_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* device-to-device copy on the context stream (synchronous) */
+int zk_dev_copy(zk_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* NTT of `batch` arrays d_in[i] -> d_out[i] (distinct buffers; d_in clobbered) */
 int zk_ntt_device(zk_ctx* ctx, void* const* d_in, void* const* d_out, int batch, int log_n, int kind);
 int zk_msm_g1_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t n, int window, uint64_t out[8],

@@ -2,6 +2,8 @@
 replayed, and the MSM window autotuner. Both must leave every proof identical
 (tolerance 0: the arithmetic is exact) to one-at-a-time proofs of the same
 inputs, and one proof per test is checked against the oracle and verified."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -31,6 +33,18 @@ def _upload(ctx, arrays):
     return ptrs
 
 
+GRAPH_OFF = ("graph creation ended in a host segmentation fault on its first GPU run; the cause is not "
+             "found, so zk_batch_graph_create is switched off and raises ZK_E_STATE")
+
+
+def test_batch_graph_disabled(setup):
+    ctx, specs, circs = setup
+    with pytest.raises(zk.ZkError) as e:
+        ctx.batch_graph([circs[0]], [ctypes.c_void_p(1)], slots=1)   # rejected before any device work
+    assert e.value.code == 6
+
+
+@pytest.mark.skip(reason=GRAPH_OFF)
 def test_batch_graph_matches_plain(setup):
     ctx, specs, circs = setup
     order = [0, 1, 2, 1, 0, 2, 1]            # mixed sizes, circuits shared across proofs
@@ -63,6 +77,7 @@ def test_batch_graph_matches_plain(setup):
         ctx.dev_free(p)
 
 
+@pytest.mark.skip(reason=GRAPH_OFF)
 def test_batch_graph_rejects_bad_blinding(setup):
     ctx, specs, circs = setup
     w = circs[0].witness(1)

@@ -104,6 +104,7 @@ ABI = {
     "zk_dev_free": (ctypes.c_int, [_vp, _vp]),
     "zk_dev_upload": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
     "zk_dev_download": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "zk_dev_copy": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
     "zk_ntt_device": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int]),
     "zk_msm_g1_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_int, _u64p,
@@ -244,6 +245,9 @@ class Context:
         out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype=dtype)
         _check(lib().zk_dev_download(self.h, out.ctypes.data_as(_vp), src, nbytes))
         return out
+
+    def dev_copy(self, dst, src, nbytes):
+        _check(lib().zk_dev_copy(self.h, dst, src, nbytes))
 
     def ntt_device(self, d_in, d_out, log_n, kind):
         n = len(d_in)

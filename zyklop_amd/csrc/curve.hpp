@@ -22,7 +22,8 @@ ZK_DEV Fq2 add(const Fq2& a, const Fq2& b) { return {add<FpCfg>(a.c0, b.c0), add
 ZK_DEV Fq2 sub(const Fq2& a, const Fq2& b) { return {sub<FpCfg>(a.c0, b.c0), sub<FpCfg>(a.c1, b.c1)}; }
 ZK_DEV Fq2 neg(const Fq2& a) { return {neg<FpCfg>(a.c0), neg<FpCfg>(a.c1)}; }
 // Karatsuba: 3 Fp products (include/fp2_bn254.cuh:130-142 uses the same count).
-// Kept out of line: G2 formulas call it a dozen times per point operation.
+// Inline: out-of-line calls in the G2 bucket loop cost more in spills than the
+// larger code.
 ZK_DEV Fq2 mul(const Fq2& a, const Fq2& b) {
     Fp t0 = mul<FpCfg>(a.c0, b.c0);
     Fp t1 = mul<FpCfg>(a.c1, b.c1);
@@ -76,9 +77,11 @@ ZK_DEV Xyzz<T> xyzz_from_aff(const Aff<T>& a) {
     return {a.x, a.y, f_one<T>(), f_one<T>()};
 }
 
-// mdbl-2008-s-1: 2 * (x, y) for an affine point, a = 0
+// mdbl-2008-s-1: 2 * (x, y) for an affine point, a = 0. Inline: it is the
+// rare branch of the mixed add, and an out-of-line call there would make the
+// caller keep its point in memory on every iteration of the bucket loop.
 template <class T>
-ZK_NOINL Xyzz<T> xyzz_mdbl(const Aff<T>& a) {
+ZK_DEV Xyzz<T> xyzz_mdbl(const Aff<T>& a) {
     T U = add(a.y, a.y);
     T V = sqr(U);
     T W = mul(U, V);

@@ -75,8 +75,11 @@ struct DigitIter {
     }
 };
 
+// Pass 1: count digits per bucket; the atomic's return value is the entry's
+// rank inside its bucket, kept per (window, scalar) so that pass 2 places
+// entries with plain stores.
 __global__ void k_msm_hist(const uint32_t* __restrict__ sc, size_t n, int c, int W, uint32_t K,
-                           uint32_t* __restrict__ counts) {
+                           uint32_t* __restrict__ counts, uint32_t* __restrict__ ranks) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     DigitIter it;
@@ -84,12 +87,14 @@ __global__ void k_msm_hist(const uint32_t* __restrict__ sc, size_t n, int c, int
     for (int w = 0; w < W; w++) {
         bool ng;
         uint32_t mag = it.next(ng);
-        if (mag) atomicAdd(&counts[(size_t)w * K + mag - 1], 1u);
+        if (mag) ranks[(size_t)w * n + i] = atomicAdd(&counts[(size_t)w * K + mag - 1], 1u);
     }
 }
 
+// Pass 2: entry (scalar i, window w) goes to offsets[bucket] + rank.
 __global__ void k_msm_scatter(const uint32_t* __restrict__ sc, size_t n, int c, int W, uint32_t K,
-                              uint32_t* __restrict__ cursor, uint32_t* __restrict__ entries) {
+                              const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ ranks,
+                              uint32_t* __restrict__ entries) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     DigitIter it;
@@ -98,7 +103,7 @@ __global__ void k_msm_scatter(const uint32_t* __restrict__ sc, size_t n, int c, 
         bool ng;
         uint32_t mag = it.next(ng);
         if (mag) {
-            uint32_t pos = atomicAdd(&cursor[(size_t)w * K + mag - 1], 1u);
+            uint32_t pos = offsets[(size_t)w * K + mag - 1] + ranks[(size_t)w * n + i];
             entries[pos] = (uint32_t)i | (ng ? 0x80000000u : 0u);
         }
     }
@@ -151,18 +156,18 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
     counts.ensure((nb + 1) * 4);
     offsets.ensure((nb + 1) * 4 + (size_t)4 * (cdiv(nb + 1, SCAN_TILE) + 1));
     entries.ensure(std::max<size_t>(1, (size_t)g.W * n) * 4);
+    ranks.ensure(std::max<size_t>(1, (size_t)g.W * n) * 4);
     fill_u32(counts.p, 0u, nb + 1, st);
     if (n) {
-        k_msm_hist<<<cdiv(n, 256), 256, 0, st>>>((const uint32_t*)d_scalars, n, g.c, g.W, g.K, counts.as<uint32_t>());
+        k_msm_hist<<<cdiv(n, 256), 256, 0, st>>>((const uint32_t*)d_scalars, n, g.c, g.W, g.K, counts.as<uint32_t>(),
+                                                ranks.as<uint32_t>());
         ZK_HIP(hipGetLastError());
     }
     uint32_t* off = offsets.as<uint32_t>();
     exclusive_scan(counts.as<uint32_t>(), off, off + nb + 1, nb + 1, st);
-    // scatter uses counts as the running cursor (re-seeded from the offsets)
-    ZK_HIP(hipMemcpyAsync(counts.p, off, (nb + 1) * 4, hipMemcpyDeviceToDevice, st));
     if (n) {
-        k_msm_scatter<<<cdiv(n, 256), 256, 0, st>>>((const uint32_t*)d_scalars, n, g.c, g.W, g.K, counts.as<uint32_t>(),
-                                                   entries.as<uint32_t>());
+        k_msm_scatter<<<cdiv(n, 256), 256, 0, st>>>((const uint32_t*)d_scalars, n, g.c, g.W, g.K, off,
+                                                   ranks.as<uint32_t>(), entries.as<uint32_t>());
         ZK_HIP(hipGetLastError());
     }
     // accumulation order. Oversized threshold: 8x the mean bucket load
