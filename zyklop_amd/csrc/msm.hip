@@ -75,37 +75,94 @@ struct DigitIter {
     }
 };
 
-// Pass 1: count digits per bucket; the atomic's return value is the entry's
-// rank inside its bucket, kept per (window, scalar) so that pass 2 places
-// entries with plain stores.
-__global__ void k_msm_hist(const uint32_t* __restrict__ sc, size_t n, int c, int W, uint32_t K,
-                           uint32_t* __restrict__ counts, uint32_t* __restrict__ ranks) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    DigitIter it;
-    it.init(sc, i, c);
-    for (int w = 0; w < W; w++) {
-        bool ng;
-        uint32_t mag = it.next(ng);
-        if (mag) ranks[(size_t)w * n + i] = atomicAdd(&counts[(size_t)w * K + mag - 1], 1u);
+// ---------------------------------------------------------------- digit sort
+// Two-pass LDS radix sort of the flat bucket ids b = w*K + |digit| - 1 of all
+// nonzero digits (no global atomics). Buckets are grouped by b >> s.
+// Pass A, per tile of scalars: an LDS histogram of groups goes to a count
+// matrix [group][tile]; a device scan of the matrix gives every tile its run
+// inside every group; the tile then writes (entry, b) pairs at run + LDS rank.
+// Pass B, per group: LDS counts of its 2^s buckets give the bucket offsets,
+// then the entries go to offset + LDS rank.
+__global__ void __launch_bounds__(1024) k_sort_tile_hist(const uint32_t* __restrict__ sc, size_t n, size_t tile, int c,
+                                                         int W, uint32_t K, int s, uint32_t G, uint32_t nt,
+                                                         uint32_t* __restrict__ matrix) {
+    extern __shared__ uint32_t lh[];
+    for (uint32_t k = threadIdx.x; k < G; k += blockDim.x) lh[k] = 0;
+    __syncthreads();
+    const size_t i0 = (size_t)blockIdx.x * tile, i1 = i0 + tile < n ? i0 + tile : n;
+    for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        DigitIter it;
+        it.init(sc, i, c);
+        for (int w = 0; w < W; w++) {
+            bool ng;
+            uint32_t mag = it.next(ng);
+            if (mag) atomicAdd(&lh[((uint32_t)w * K + mag - 1) >> s], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < G; k += blockDim.x) matrix[(size_t)k * nt + blockIdx.x] = lh[k];
+}
+
+__global__ void __launch_bounds__(1024) k_sort_tile_scatter(const uint32_t* __restrict__ sc, size_t n, size_t tile, int c,
+                                                            int W, uint32_t K, int s, uint32_t G, uint32_t nt,
+                                                            const uint32_t* __restrict__ mscan, uint2* __restrict__ pairs) {
+    extern __shared__ uint32_t lds[];
+    uint32_t* base = lds;       // this tile's run start in each group
+    uint32_t* rank = lds + G;   // next rank in each group
+    for (uint32_t k = threadIdx.x; k < G; k += blockDim.x) {
+        base[k] = mscan[(size_t)k * nt + blockIdx.x];
+        rank[k] = 0;
+    }
+    __syncthreads();
+    const size_t i0 = (size_t)blockIdx.x * tile, i1 = i0 + tile < n ? i0 + tile : n;
+    for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        DigitIter it;
+        it.init(sc, i, c);
+        for (int w = 0; w < W; w++) {
+            bool ng;
+            uint32_t mag = it.next(ng);
+            if (mag) {
+                const uint32_t b = (uint32_t)w * K + mag - 1, g = b >> s;
+                const uint32_t pos = base[g] + atomicAdd(&rank[g], 1u);
+                pairs[pos] = make_uint2((uint32_t)i | (ng ? 0x80000000u : 0u), b);
+            }
+        }
     }
 }
 
-// Pass 2: entry (scalar i, window w) goes to offsets[bucket] + rank.
-__global__ void k_msm_scatter(const uint32_t* __restrict__ sc, size_t n, int c, int W, uint32_t K,
-                              const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ ranks,
-                              uint32_t* __restrict__ entries) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    DigitIter it;
-    it.init(sc, i, c);
-    for (int w = 0; w < W; w++) {
-        bool ng;
-        uint32_t mag = it.next(ng);
-        if (mag) {
-            uint32_t pos = offsets[(size_t)w * K + mag - 1] + ranks[(size_t)w * n + i];
-            entries[pos] = (uint32_t)i | (ng ? 0x80000000u : 0u);
-        }
+__global__ void __launch_bounds__(256) k_sort_group(const uint2* __restrict__ pairs, const uint32_t* __restrict__ mscan,
+                                                    uint32_t nt, int s, uint32_t nb, uint32_t* __restrict__ offsets,
+                                                    uint32_t* __restrict__ entries) {
+    extern __shared__ uint32_t cnt[];   // 2^s counters
+    __shared__ uint32_t wsum[16];
+    const uint32_t g = blockIdx.x, F = 1u << s, mask = F - 1;
+    const uint32_t start = mscan[(size_t)g * nt], end = mscan[(size_t)(g + 1) * nt];
+    for (uint32_t f = threadIdx.x; f < F; f += blockDim.x) cnt[f] = 0;
+    __syncthreads();
+    for (uint32_t e = start + threadIdx.x; e < end; e += blockDim.x) atomicAdd(&cnt[pairs[e].y & mask], 1u);
+    __syncthreads();
+    // exclusive scan of the counters: per-thread chunks, then the block scan
+    const uint32_t q = (F + blockDim.x - 1) / blockDim.x, f0 = threadIdx.x * q;
+    const uint32_t f1 = f0 + q < F ? f0 + q : F;
+    uint32_t sum = 0;
+    for (uint32_t f = f0; f < f1; f++) sum += cnt[f];
+    uint32_t total;
+    uint32_t run = start + block_exclusive_scan(sum, wsum, &total);
+    for (uint32_t f = f0; f < f1; f++) {
+        const uint32_t v = cnt[f];
+        cnt[f] = run;
+        run += v;
+    }
+    __syncthreads();
+    for (uint32_t f = threadIdx.x; f < F; f += blockDim.x) {
+        const uint32_t b = g * F + f;
+        if (b < nb) offsets[b] = cnt[f];
+    }
+    if (g == gridDim.x - 1 && threadIdx.x == 0) offsets[nb] = end;
+    __syncthreads();
+    for (uint32_t e = start + threadIdx.x; e < end; e += blockDim.x) {
+        const uint2 p = pairs[e];
+        entries[atomicAdd(&cnt[p.y & mask], 1u)] = p.x;
     }
 }
 
@@ -153,23 +210,39 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
     n = n_;
     g = msm_geom(n, c, false);
     size_t nb = (size_t)g.W * g.K;
-    counts.ensure((nb + 1) * 4);
-    offsets.ensure((nb + 1) * 4 + (size_t)4 * (cdiv(nb + 1, SCAN_TILE) + 1));
-    entries.ensure(std::max<size_t>(1, (size_t)g.W * n) * 4);
-    ranks.ensure(std::max<size_t>(1, (size_t)g.W * n) * 4);
-    fill_u32(counts.p, 0u, nb + 1, st);
+    // group size 2^s: at most 4096 groups (LDS histograms of pass A)
+    int sh = 7;
+    while (((nb + (1ull << sh) - 1) >> sh) > 4096) sh++;
+    const uint32_t G = (uint32_t)((nb + (1ull << sh) - 1) >> sh);
+    // tiles of pass A: multiples of 1024 scalars, at most 2048 tiles
+    const size_t tile = (size_t)1024 * std::max<size_t>(1, (n + 1024 * 2048 - 1) / (1024 * 2048));
+    const uint32_t nt = (uint32_t)std::max<size_t>(1, (n + tile - 1) / tile);
+    const size_t mlen = (size_t)G * nt + 1;
+    const size_t ne = std::max<size_t>(1, (size_t)g.W * n);
+    matrix.ensure(mlen * 4);
+    mscan.ensure(mlen * 4 + (size_t)4 * (cdiv(mlen, SCAN_TILE) + 1));
+    pairs.ensure(ne * 8);
+    entries.ensure(ne * 4);
+    offsets.ensure((nb + 1) * 4);
+    uint32_t* mat = matrix.as<uint32_t>();
+    uint32_t* msc = mscan.as<uint32_t>();
     if (n) {
-        k_msm_hist<<<cdiv(n, 256), 256, 0, st>>>((const uint32_t*)d_scalars, n, g.c, g.W, g.K, counts.as<uint32_t>(),
-                                                ranks.as<uint32_t>());
+        k_sort_tile_hist<<<nt, 1024, G * 4, st>>>((const uint32_t*)d_scalars, n, tile, g.c, g.W, g.K, sh, G, nt, mat);
+        ZK_HIP(hipGetLastError());
+        fill_u32(mat + mlen - 1, 0u, 1, st);
+    } else {
+        fill_u32(mat, 0u, mlen, st);
+    }
+    exclusive_scan(mat, msc, msc + mlen, mlen, st);
+    if (n) {
+        k_sort_tile_scatter<<<nt, 1024, 2 * G * 4, st>>>((const uint32_t*)d_scalars, n, tile, g.c, g.W, g.K, sh, G, nt,
+                                                          msc, pairs.as<uint2>());
         ZK_HIP(hipGetLastError());
     }
-    uint32_t* off = offsets.as<uint32_t>();
-    exclusive_scan(counts.as<uint32_t>(), off, off + nb + 1, nb + 1, st);
-    if (n) {
-        k_msm_scatter<<<cdiv(n, 256), 256, 0, st>>>((const uint32_t*)d_scalars, n, g.c, g.W, g.K, off,
-                                                   ranks.as<uint32_t>(), entries.as<uint32_t>());
-        ZK_HIP(hipGetLastError());
-    }
+    k_sort_group<<<G, 256, (4u << sh), st>>>(pairs.as<uint2>(), msc, nt, sh, (uint32_t)nb, offsets.as<uint32_t>(),
+                                             entries.as<uint32_t>());
+    ZK_HIP(hipGetLastError());
+    const uint32_t* off = offsets.as<uint32_t>();
     // accumulation order. Oversized threshold: 8x the mean bucket load
     // (uniform scalars put about n/K entries in a bucket).
     lmax = std::max<uint32_t>(L_MIN_LARGE, 8u * (uint32_t)((n + g.K - 1) / g.K));

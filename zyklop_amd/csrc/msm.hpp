@@ -3,9 +3,9 @@
 // Reference: msm_g1 (include/msm_g1.cuh:714-913), msm_g2_device
 // (include/msm_g2.cuh:669-854). Same mathematics (signed c-bit digits,
 // buckets, running-sum reduction), different machine design:
-//   * digit sort: one atomic per (scalar, window) that both counts the
-//     bucket and returns the entry's rank in it, own LDS/wave scan, then a
-//     store-only scatter to offset + rank; no CUB, no host synchronisation;
+//   * digit sort: a two-pass LDS radix sort (tile histograms of bucket
+//     groups + device scan + LDS-ranked scatter, then per-group LDS counting
+//     sort); no global atomics, no CUB, no host synchronisation;
 //   * one sort can serve several MSMs that share a scalar vector;
 //   * buckets in XYZZ coordinates, one thread per bucket; oversized buckets
 //     are split into fixed chunks reduced by whole workgroups (device-side
@@ -44,7 +44,7 @@ struct MsmSort {
     size_t n = 0;
     uint32_t lmax = 0, nkey = 0;
     int qs = 0;
-    DevBuf counts, offsets, entries, ranks, order, lkeys;
+    DevBuf matrix, mscan, pairs, offsets, entries, order, lkeys;
     // sorts canonical scalars (n x 32 B, device) into bucket order
     void run(const void* d_scalars, size_t n, int c, hipStream_t st);
 };

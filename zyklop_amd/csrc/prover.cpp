@@ -255,7 +255,8 @@ ProveSlot::~ProveSlot() {
 size_t ProveSlot::device_bytes() const {
     size_t b = scal.bytes + aw.bytes + bw.bytes + cw.bytes + t0.bytes + t1.bytes + t2.bytes + h.bytes;
     for (const MsmSort* q : {&sortW, &sortH})
-        b += q->counts.bytes + q->offsets.bytes + q->entries.bytes + q->ranks.bytes + q->order.bytes + q->lkeys.bytes;
+        b += q->matrix.bytes + q->mscan.bytes + q->pairs.bytes + q->offsets.bytes + q->entries.bytes + q->order.bytes +
+             q->lkeys.bytes;
     for (const MsmScratch* s : {&scrA, &scrB1, &scrC, &scrB2, &scrH})
         b += s->buckets.bytes + s->r1.bytes + s->t1.bytes + s->r2.bytes + s->t2.bytes + s->chunks.bytes + s->large.bytes +
              s->partials.bytes;
@@ -357,8 +358,9 @@ size_t ProveSlot::bytes_needed(const Circuit& c) {
     const size_t nS = (size_t)c.n_vars + 4, n = c.n;
     MsmGeom gw = msm_geom(nS, 0, false), gh = msm_geom(n, 0, false);
     size_t b = nS * 32 + 7 * n * 32;                                    // scalars + Fr arrays
-    b += 2 * ((size_t)gw.W * nS * 4 + (size_t)gh.W * n * 4);            // sorted entries + ranks
-    b += 4 * 4 * ((size_t)gw.W * gw.K + (size_t)gh.W * gh.K);           // counts / offsets / order
+    b += 3 * ((size_t)gw.W * nS * 4 + (size_t)gh.W * n * 4);            // sorted entries + (entry, bucket) pairs
+    b += 4 * 4 * ((size_t)gw.W * gw.K + (size_t)gh.W * gh.K);           // offsets / order
+    b += 2 * 4 * (size_t)4096 * (std::min<size_t>(2048, nS / 1024 + 1) + std::min<size_t>(2048, n / 1024 + 1));  // group matrices
     size_t bw = (size_t)gw.W * gw.K, bh = (size_t)gh.W * gh.K;
     b += 3 * bw * 128 + bw * 256 + bh * 128;                            // buckets A, B1, C, B2, H
     b += (3 * bw + bw * 2 + bh) * 128 / 4;                              // reduction partials (upper bound)
