@@ -93,9 +93,10 @@ ZK_DEV Xyzz<T> xyzz_mdbl(const Aff<T>& a) {
     return {X3, Y3, V, W};
 }
 
-// dbl-2008-s-1, a = 0
+// dbl-2008-s-1, a = 0 (inline form; the rare equal-points branch of the
+// additions uses it so that their operands never need an address)
 template <class T>
-ZK_NOINL Xyzz<T> xyzz_dbl(const Xyzz<T>& p) {
+ZK_DEV Xyzz<T> xyzz_dbl_i(const Xyzz<T>& p) {
     T U = add(p.Y, p.Y);
     T V = sqr(U);
     T W = mul(U, V);
@@ -107,6 +108,9 @@ ZK_NOINL Xyzz<T> xyzz_dbl(const Xyzz<T>& p) {
     // doubling infinity keeps ZZ = 0
     return {X3, Y3, mul(V, p.ZZ), mul(W, p.ZZZ)};
 }
+// out-of-line form for code outside the hot loops (one copy per kernel)
+template <class T>
+ZK_NOINL Xyzz<T> xyzz_dbl(const Xyzz<T>& p) { return xyzz_dbl_i(p); }
 
 // madd-2008-s: acc += a (a affine, not infinity)
 template <class T>
@@ -137,7 +141,7 @@ ZK_DEV void xyzz_madd(Xyzz<T>& acc, const Aff<T>& a) {
 
 // add-2008-s: p + q, both XYZZ
 template <class T>
-ZK_NOINL Xyzz<T> xyzz_add(const Xyzz<T>& p, const Xyzz<T>& q) {
+ZK_DEV Xyzz<T> xyzz_add_i(const Xyzz<T>& p, const Xyzz<T>& q) {
     if (xyzz_is_inf(p)) return q;
     if (xyzz_is_inf(q)) return p;
     T U1 = mul(p.X, q.ZZ);
@@ -147,7 +151,7 @@ ZK_NOINL Xyzz<T> xyzz_add(const Xyzz<T>& p, const Xyzz<T>& q) {
     T P = sub(U2, U1);
     T R = sub(S2, S1);
     if (is_zero(P)) {
-        if (is_zero(R)) return xyzz_dbl(p);
+        if (is_zero(R)) return xyzz_dbl_i(p);
         return xyzz_inf<T>();
     }
     T PP = sqr(P);
@@ -157,6 +161,10 @@ ZK_NOINL Xyzz<T> xyzz_add(const Xyzz<T>& p, const Xyzz<T>& q) {
     T Y3 = sub(mul(R, sub(Q, X3)), mul(S1, PPP));
     return {X3, Y3, mul(mul(p.ZZ, q.ZZ), PP), mul(mul(p.ZZZ, q.ZZZ), PPP)};
 }
+
+// out-of-line form for code outside the hot loops (one copy per kernel)
+template <class T>
+ZK_NOINL Xyzz<T> xyzz_add(const Xyzz<T>& p, const Xyzz<T>& q) { return xyzz_add_i(p, q); }
 
 // ---- global memory access ----
 // G1 affine point: 64 B = x (32 B) then y (32 B). G2: 128 B = x.c0, x.c1, y.c0, y.c1.

@@ -353,11 +353,16 @@ __global__ void __launch_bounds__(256) k_msm_merge(const uint32_t* __restrict__ 
 }
 
 // ---------------------------------------------------------------- reduction
+// The point additions here are inlined, each loop body holding one or two
+// addition sites (level 2 selects its operands per step; the step index is
+// uniform across the wave): out-of-line calls would pass the 128/256-byte
+// operands through scratch memory on every step of these latency-bound chains.
+//
 // Level 1: thread j owns buckets [j*L1, (j+1)*L1) of one window; with bucket
 // b holding digit b+1: R_j = sum B_b, T_j = sum (b - j*L1 + 1) B_b, so the
 // window value is sum_j (T_j + j*L1*R_j).
 template <class T>
-__global__ void __launch_bounds__(256) k_msm_reduce1(const void* __restrict__ buckets, uint32_t K, int W,
+__global__ void __launch_bounds__(128) k_msm_reduce1(const void* __restrict__ buckets, uint32_t K, int W,
                                                      void* __restrict__ R, void* __restrict__ Tt) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t nseg = K / L1;
@@ -365,9 +370,10 @@ __global__ void __launch_bounds__(256) k_msm_reduce1(const void* __restrict__ bu
     uint32_t w = j / nseg, jj = j - w * nseg;
     size_t base = (size_t)w * K + (size_t)jj * L1;
     Xyzz<T> run = xyzz_inf<T>(), acc = xyzz_inf<T>();
+#pragma unroll 1
     for (int b = L1 - 1; b >= 0; b--) {
-        run = xyzz_add(run, PointIO<T>::load_xyzz(buckets, base + b));
-        acc = xyzz_add(acc, run);
+        run = xyzz_add_i(run, PointIO<T>::load_xyzz(buckets, base + b));
+        acc = xyzz_add_i(acc, run);
     }
     PointIO<T>::store_xyzz(R, j, run);
     PointIO<T>::store_xyzz(Tt, j, acc);
@@ -375,61 +381,69 @@ __global__ void __launch_bounds__(256) k_msm_reduce1(const void* __restrict__ bu
 
 // Level 2: input per window is M pairs (R_j, T_j) worth sum_j (T_j + j*U*R_j),
 // U = 2^logU. Block blk covers j in [blk*BLK*L2, (blk+1)*BLK*L2); thread t owns
-// L2 consecutive j. Output per block: R'' = sum R_j and
-// T'' = sum (T_j + (j - j_blk) * U * R_j), so the window value is
+// L2 consecutive j. Per thread: run = sum R_j, acc = sum (j - j0) R_j,
+// tsum = sum T_j. With S_t = sum_{t' >= t} run_t' (suffix scan over the block),
+// sum_t t*run_t = sum_{t >= 1} S_t, so the block's value relative to j_blk is
+// sum_t V_t with V_t = tsum_t + U*(acc_t + L2*[t >= 1]*S_t): one tree sum.
+// Output per block: R'' = S_0 and T'' = sum_t V_t, so the window value is
 // sum_blk (T''_blk + blk * (U*L2*BLK) * R''_blk).
 template <class T, int BLK>
 __global__ void __launch_bounds__(BLK) k_msm_reduce2(const void* __restrict__ Rin, const void* __restrict__ Tin, uint32_t M,
                                                      int logU, uint32_t nblk, void* __restrict__ Rout,
                                                      void* __restrict__ Tout) {
-    __shared__ Xyzz<T> shA[BLK];
-    __shared__ Xyzz<T> shB[BLK];
+    __shared__ Xyzz<T> sh[BLK];
     const int t = threadIdx.x;
     const uint32_t w = blockIdx.x / nblk, blk = blockIdx.x - w * nblk;
     const uint32_t j0 = (blk * BLK + t) * L2;
     Xyzz<T> run = xyzz_inf<T>(), acc = xyzz_inf<T>(), tsum = xyzz_inf<T>();
-    for (int q = L2 - 1; q >= 0; q--) {
-        uint32_t j = j0 + q;
+    // per item, highest j first: acc += run; run += R_j; tsum += T_j
+#pragma unroll 1
+    for (int step = 0; step < 3 * L2; step++) {
+        const int q = L2 - 1 - step / 3, k = step % 3;
+        const uint32_t j = j0 + q;
         if (j >= M) continue;
-        acc = xyzz_add(acc, run);
-        run = xyzz_add(run, PointIO<T>::load_xyzz(Rin, (size_t)w * M + j));
-        tsum = xyzz_add(tsum, PointIO<T>::load_xyzz(Tin, (size_t)w * M + j));
+        Xyzz<T> x = k == 0 ? acc : (k == 1 ? run : tsum);
+        Xyzz<T> y = k == 0 ? run : PointIO<T>::load_xyzz(k == 1 ? Rin : Tin, (size_t)w * M + j);
+        Xyzz<T> r = xyzz_add_i(x, y);
+        if (k == 0) acc = r;
+        else if (k == 1) run = r;
+        else tsum = r;
     }
-    // this thread's share: tsum + U * acc  (acc = sum (j - j0) R_j)
-    for (int d = 0; d < logU; d++) acc = xyzz_dbl(acc);
-    Xyzz<T> tp = xyzz_add(tsum, acc);
     // suffix scan of run across the block: S_t = sum_{t' >= t} run_t'
-    shA[t] = run;
+    sh[t] = run;
+#pragma unroll 1
     for (int d = 1; d < BLK; d <<= 1) {
         __syncthreads();
-        Xyzz<T> mine = shA[t];
-        Xyzz<T> other = (t + d < BLK) ? shA[t + d] : xyzz_inf<T>();
+        Xyzz<T> mine = sh[t];
+        Xyzz<T> other = (t + d < BLK) ? sh[t + d] : xyzz_inf<T>();
         __syncthreads();
-        shA[t] = xyzz_add(mine, other);
+        sh[t] = xyzz_add_i(mine, other);
     }
     __syncthreads();
-    Xyzz<T> total_r = shA[0];
-    // sum_{t>=1} S_t = sum_t t * run_t ; reduce it together with tp
-    Xyzz<T> a = (t >= 1) ? shA[t] : xyzz_inf<T>();
+    const Xyzz<T> S = sh[t], total_r = sh[0];
     __syncthreads();
-    shA[t] = a;
-    shB[t] = tp;
+    // V_t = tsum + U * (acc + L2 * [t >= 1] * S_t): doublings and additions
+    // through one site each
+    Xyzz<T> v = t >= 1 ? S : xyzz_inf<T>();
+#pragma unroll 1
+    for (int step = 0; step < LOG_L2 + 1 + logU + 1; step++) {
+        if (step < LOG_L2 || (step > LOG_L2 && step <= LOG_L2 + logU)) {
+            v = xyzz_dbl_i(v);
+        } else {
+            v = xyzz_add_i(v, step == LOG_L2 ? acc : tsum);
+        }
+    }
+    // tree sum of V over the block
+    sh[t] = v;
+#pragma unroll 1
     for (int h = BLK >> 1; h >= 1; h >>= 1) {
         __syncthreads();
-        if (t < h) {
-            shA[t] = xyzz_add(shA[t], shA[t + h]);
-            shB[t] = xyzz_add(shB[t], shB[t + h]);
-        }
+        if (t < h) sh[t] = xyzz_add_i(sh[t], sh[t + h]);
     }
     __syncthreads();
     if (t == 0) {
-        // T'' = sum tp + (U * L2) * sum_t t * run_t
-        Xyzz<T> wsum = shA[0];
-        int sh = logU + LOG_L2;
-        for (int d = 0; d < sh; d++) wsum = xyzz_dbl(wsum);
-        Xyzz<T> tt = xyzz_add(shB[0], wsum);
         PointIO<T>::store_xyzz(Rout, blockIdx.x, total_r);
-        PointIO<T>::store_xyzz(Tout, blockIdx.x, tt);
+        PointIO<T>::store_xyzz(Tout, blockIdx.x, sh[0]);
     }
 }
 
