@@ -1,0 +1,277 @@
+// BN254 Fp in 9 limbs of 29 bits, Montgomery with R' = 2^261, for the G1
+// bucket-accumulation loop.
+//
+// Why: with 32-bit limbs a column of the Montgomery product sums up to 16
+// products of 64 bits, so every v_mad_u64_u32 needs a second instruction to
+// catch its carry (about 2 VALU ops per 32x32 product). With 29-bit limbs a
+// column of 9 products a_i*b_j and 9 products m_i*p_j (each < 2^60 for inputs
+// with limbs < 2^31 and 2^29) plus the incoming carry stays below 2^64: every
+// product is one v_mad_u64_u32 and a column ends with one shift. The product
+// costs about 210 VALU ops instead of about 320.
+//
+// Forms (every bound below is checked for BN254's p ~ 2^253.6):
+//   N: limbs < 2^29, value < 2p       (products, buckets, converted points)
+//   L: limbs < 3*2^29, value < 6p     (sub4 results; normalize() -> limbs < 2^29)
+// mul(a, b) needs value(a)*value(b) < R'*p ~ 168 p^2 and per-column sums
+// < 2^64; it accepts (L or N) x N and normalized(<6p) x normalized(<6p), and
+// returns N (t = (ab + mp) / R' < ab/R' + p < 2p).
+//
+// Points enter this form once per circuit (from the standard 8 x 32-bit,
+// R = 2^256 Montgomery form) and buckets leave it once per MSM, so every
+// other kernel keeps the standard form.
+#pragma once
+#include <stdint.h>
+
+#include "curve.hpp"
+
+namespace zk {
+
+struct F29 {
+    uint32_t l[9];
+};
+
+namespace f29c {
+constexpr uint32_t MASK = (1u << 29) - 1;
+// p, and -p^-1 mod 2^29
+constexpr uint32_t P[9] = {0x187cfd47u, 0x010460b6u, 0x1c72a34fu, 0x02d522d0u, 0x1585d978u,
+                           0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu};
+constexpr uint32_t NP = 0x04866389u;
+// 2p and 4p with every limb but the top one in [2^29, 2^30) (borrowed from
+// the next limb), so that kp - b never borrows for normalized b < kp; 3 x 4p
+constexpr uint32_t P2[9] = {0x30f9fa8eu, 0x2208c16cu, 0x38e5469du, 0x25aa45a0u, 0x2b0bb2efu,
+                            0x25b68180u, 0x214dc281u, 0x3cb84c67u, 0x0060c89bu};
+constexpr uint32_t P4[9] = {0x21f3f51cu, 0x241182dau, 0x31ca8d3bu, 0x2b548b42u, 0x361765dfu,
+                            0x2b6d0301u, 0x229b8503u, 0x397098cfu, 0x00c19138u};
+constexpr uint32_t P12[9] = {0x65dbdf54u, 0x6c34888eu, 0x955fa7b1u, 0x81fda1c6u, 0xa246319du,
+                             0x82470903u, 0x67d28f09u, 0xac51ca6du, 0x0244b3a8u};
+// 3p, 4p, 5p normalized (a sub4 result is 0 mod p iff it equals one of them)
+constexpr uint32_t P3N[9] = {0x0976f7d5u, 0x030d2224u, 0x1557e9edu, 0x087f6872u, 0x00918c68u,
+                             0x0891c242u, 0x01f4a3c3u, 0x0b14729cu, 0x00912cebu};
+constexpr uint32_t P4N[9] = {0x01f3f51cu, 0x041182dbu, 0x11ca8d3cu, 0x0b548b43u, 0x161765e0u,
+                             0x0b6d0302u, 0x029b8504u, 0x197098d0u, 0x00c19139u};
+constexpr uint32_t P5N[9] = {0x1a70f263u, 0x0515e391u, 0x0e3d308bu, 0x0e29ae14u, 0x0b9d3f58u,
+                             0x0e4843c3u, 0x03426645u, 0x07ccbf04u, 0x00f1f588u};
+// R' mod p (one); 2^256 mod p and 2^266 mod p (form conversions by one product)
+constexpr uint32_t ONE[9] = {0x157ccc21u, 0x141c2758u, 0x185230d3u, 0x014c0419u, 0x0aa36fb9u,
+                             0x1d4240ceu, 0x11d54c07u, 0x052ac7a8u, 0x000dc836u};
+constexpr uint32_t TO_STD[9] = {0x058f0d9du, 0x1aea1c6eu, 0x11c2cf74u, 0x11d651ebu, 0x1462c0a7u,
+                                0x11b7bc3cu, 0x1cbd99bau, 0x183340fbu, 0x000e0a77u};
+constexpr uint32_t FROM_STD[9] = {0x13349ca1u, 0x1a5d84a8u, 0x0a3e5cacu, 0x100249e0u, 0x12b951e8u,
+                                  0x0e92d304u, 0x14cb95b3u, 0x041b9d3du, 0x00058003u};
+constexpr uint32_t PTOP1 = 3171407u;  // (p >> 232) + 1
+}  // namespace f29c
+
+ZK_DEV F29 f29_const(const uint32_t (&c)[9]) {
+    F29 r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.l[i] = c[i];
+    return r;
+}
+
+// Montgomery product a*b/R' mod p (product scanning, one 64-bit accumulator)
+ZK_DEV F29 f29_mul(const F29& a, const F29& b) {
+    using namespace f29c;
+    uint32_t m[9];
+    F29 t;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < 17; k++) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            const int j = k - i;
+            if (j >= 0 && j < 9) acc += (uint64_t)a.l[i] * b.l[j];
+        }
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            const int j = k - i;
+            if (i < k && j >= 0 && j < 9) acc += (uint64_t)m[i] * P[j];
+        }
+        if (k < 9) {
+            m[k] = ((uint32_t)acc * NP) & MASK;
+            acc += (uint64_t)m[k] * P[0];
+        } else {
+            t.l[k - 9] = (uint32_t)acc & MASK;
+        }
+        acc >>= 29;
+    }
+    t.l[8] = (uint32_t)acc;
+    return t;
+}
+
+// carry propagation: limbs < 2^32 -> limbs < 2^29 (same value)
+ZK_DEV F29 f29_norm(const F29& a) {
+    F29 r;
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const uint32_t x = a.l[i] + c;
+        r.l[i] = i < 8 ? (x & f29c::MASK) : x;
+        c = x >> 29;
+    }
+    return r;
+}
+
+// a + 4p - b for normalized a, b (value < 2p): form L, value in (2p, 6p)
+ZK_DEV F29 f29_sub4(const F29& a, const F29& b) {
+    F29 r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.l[i] = a.l[i] + f29c::P4[i] - b.l[i];
+    return r;
+}
+
+// normalized value < 16p -> value < 2p (subtract q*p, q from the top limb)
+ZK_DEV F29 f29_reduce(const F29& a) {
+    const uint32_t q = a.l[8] / f29c::PTOP1;
+    F29 r;
+    int64_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const int64_t x = (int64_t)a.l[i] - (int64_t)((uint64_t)q * f29c::P[i]) + c;
+        r.l[i] = (uint32_t)x & f29c::MASK;
+        c = x >> 29;
+    }
+    return r;
+}
+
+ZK_DEV bool f29_eq(const F29& a, const uint32_t (&c)[9]) {
+    uint32_t d = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) d |= a.l[i] ^ c[i];
+    return d == 0;
+}
+
+// for normalized x = u + 4p - v (u, v < 2p): x = 0 mod p iff x is 3p, 4p or 5p
+ZK_DEV bool f29_sub4_is_zero(const F29& x) {
+    using namespace f29c;
+    const uint32_t l0 = x.l[0];
+    if (l0 != P3N[0] && l0 != P4N[0] && l0 != P5N[0]) return false;
+    return f29_eq(x, P3N) || f29_eq(x, P4N) || f29_eq(x, P5N);
+}
+
+ZK_DEV bool f29_is_zero(const F29& a) {
+    uint32_t d = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) d |= a.l[i];
+    return d == 0;
+}
+
+// -y for y < p: 2p - y normalized (value in (p, 2p])
+ZK_DEV F29 f29_neg(const F29& y) {
+    F29 r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.l[i] = f29c::P2[i] - y.l[i];
+    return f29_norm(r);
+}
+
+// 8 x 32-bit words (value < 2^256) <-> 9 x 29-bit limbs
+ZK_DEV F29 f29_unpack(const uint32_t w[8]) {
+    F29 r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const int bit = 29 * i, k = bit >> 5, sh = bit & 31;
+        const uint32_t lo = w[k], hi = k + 1 < 8 ? w[k + 1] : 0u;
+        r.l[i] = __builtin_amdgcn_alignbit(hi, lo, sh) & f29c::MASK;
+    }
+    return r;
+}
+ZK_DEV void f29_pack(const F29& a, uint32_t w[8]) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int bit = 32 * k, i = bit / 29, sh = bit % 29;
+        uint64_t v = (uint64_t)a.l[i] >> sh;
+        if (i + 1 < 9) v |= (uint64_t)a.l[i + 1] << (29 - sh);
+        if (i + 2 < 9 && 58 - sh < 64) v |= (uint64_t)a.l[i + 2] << (58 - sh);
+        w[k] = (uint32_t)v;
+    }
+}
+
+// standard Montgomery (x*2^256, < p) -> this form (x*2^261, < p)
+ZK_DEV F29 f29_from_std(const Fp& x) {
+    F29 t = f29_mul(f29_unpack(x.v), f29_const(f29c::FROM_STD));
+    Fp s;
+    f29_pack(t, s.v);
+    reduce_once<FpCfg>(s.v);  // t < 2p -> < p
+    return f29_unpack(s.v);
+}
+
+// this form (x*2^261, value < 2p) -> standard Montgomery (x*2^256, < p)
+ZK_DEV Fp f29_to_std(const F29& v) {
+    F29 t = f29_mul(v, f29_const(f29c::TO_STD));
+    Fp s;
+    f29_pack(t, s.v);
+    reduce_once<FpCfg>(s.v);
+    return s;
+}
+
+// XYZZ bucket in this form; inf = no point added yet (or the sum is infinity)
+struct Xyzz29 {
+    F29 X, Y, ZZ, ZZZ;
+    bool inf;
+};
+
+// madd-2008-s, acc += (x2, y2) with x2 < p, y2 < 2p (normalized). The equal-x
+// cases (doubling, or the sum is infinity) are rare: the doubling goes through
+// the standard-form formulas.
+ZK_DEV void f29_madd(Xyzz29& A, const F29& x2, const F29& y2) {
+    using namespace f29c;
+    if (A.inf) {
+        A.X = x2;
+        A.Y = y2;
+        A.ZZ = f29_const(ONE);
+        A.ZZZ = f29_const(ONE);
+        A.inf = false;
+        return;
+    }
+    const F29 U2 = f29_mul(x2, A.ZZ);
+    const F29 S2 = f29_mul(y2, A.ZZZ);
+    const F29 P_ = f29_norm(f29_sub4(U2, A.X));
+    const F29 R_ = f29_norm(f29_sub4(S2, A.Y));
+    if (f29_sub4_is_zero(P_)) {
+        if (f29_sub4_is_zero(R_)) {
+            Aff<Fp> a;
+            a.x = f29_to_std(x2);
+            a.y = f29_to_std(y2);
+            const Xyzz<Fp> d = xyzz_mdbl(a);
+            A.X = f29_from_std(d.X);
+            A.Y = f29_from_std(d.Y);
+            A.ZZ = f29_from_std(d.ZZ);
+            A.ZZZ = f29_from_std(d.ZZZ);
+        } else {
+            A.inf = true;
+        }
+        return;
+    }
+    const F29 PP = f29_mul(P_, P_);
+    const F29 PPP = f29_mul(P_, PP);
+    const F29 Q = f29_mul(A.X, PP);
+    const F29 RR = f29_mul(R_, R_);
+    // X3 = RR - PPP - 2Q + 12p: limbs stay in [0, 2^32)
+    F29 x3;
+#pragma unroll
+    for (int i = 0; i < 9; i++) x3.l[i] = (P12[i] - PPP.l[i] - 2 * Q.l[i]) + RR.l[i];
+    const F29 X3 = f29_reduce(f29_norm(x3));
+    const F29 T = f29_norm(f29_sub4(Q, X3));
+    const F29 m1 = f29_mul(R_, T);
+    const F29 m2 = f29_mul(A.Y, PPP);
+    A.Y = f29_reduce(f29_norm(f29_sub4(m1, m2)));
+    A.X = X3;
+    A.ZZ = f29_mul(A.ZZ, PP);
+    A.ZZZ = f29_mul(A.ZZZ, PPP);
+}
+
+// G1 affine point in this form: x[9], y[9], 2 words of padding (80 bytes)
+constexpr int PT29_WORDS = 20;
+
+ZK_DEV void f29_load_point(const void* base, size_t idx, F29& x, F29& y) {
+    const uint4* p = reinterpret_cast<const uint4*>(base) + 5 * idx;
+    const uint4 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
+    const uint32_t w[20] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y,
+                            c.z, c.w, d.x, d.y, d.z, d.w, e.x, e.y, e.z, e.w};
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        x.l[i] = w[i];
+        y.l[i] = w[9 + i];
+    }
+}
+
+}  // namespace zk

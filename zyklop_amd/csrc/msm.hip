@@ -3,6 +3,7 @@
 #include <type_traits>
 
 #include "curve.hpp"
+#include "field29.hpp"
 #include "msm.hpp"
 #include "qap.hpp"
 #include "scan.hpp"
@@ -11,7 +12,7 @@ namespace zk {
 
 constexpr uint32_t L_MIN_LARGE = 64;  // floor of the oversized-bucket threshold
 constexpr uint32_t CHUNK = 256 * 16;  // entries per chunk of an oversized bucket
-constexpr int L1 = 8;                 // buckets per level-1 reduction thread
+constexpr uint32_t L1_MAX = 8;        // buckets per level-1 reduction thread (fewer if K is smaller)
 constexpr int L2 = 4;                 // level-1 segments per level-2 thread
 constexpr int LOG_L2 = 2;
 template <class T> struct RedBlk;
@@ -36,7 +37,8 @@ MsmGeom msm_geom(size_t n, int c, bool g2) {
     g.c = c;
     g.W = (255 + c - 1) / c;
     g.K = 1u << (c - 1);
-    g.nseg = g.K / L1;
+    g.l1 = std::min<uint32_t>(L1_MAX, g.K);
+    g.nseg = g.K / g.l1;
     uint32_t per_blk = (uint32_t)(g2 ? RedBlk<Fq2>::V : RedBlk<Fp>::V) * L2;
     g.nblk = (g.nseg + per_blk - 1) / per_blk;
     return g;
@@ -214,8 +216,9 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
     int sh = 7;
     while (((nb + (1ull << sh) - 1) >> sh) > 4096) sh++;
     const uint32_t G = (uint32_t)((nb + (1ull << sh) - 1) >> sh);
-    // tiles of pass A: multiples of 1024 scalars, at most 2048 tiles
-    const size_t tile = (size_t)1024 * std::max<size_t>(1, (n + 1024 * 2048 - 1) / (1024 * 2048));
+    // tiles of pass A: multiples of 4096 scalars (long runs per group for the
+    // pair writes), at most 2048 tiles
+    const size_t tile = (size_t)4096 * std::max<size_t>(1, (n + 4096 * 2048 - 1) / (4096 * 2048));
     const uint32_t nt = (uint32_t)std::max<size_t>(1, (n + tile - 1) / tile);
     const size_t mlen = (size_t)G * nt + 1;
     const size_t ne = std::max<size_t>(1, (size_t)g.W * n);
@@ -299,6 +302,71 @@ __global__ void __launch_bounds__(256) k_msm_accum(const uint32_t* __restrict__ 
     PointIO<T>::store_xyzz(buckets, b, acc);
 }
 
+// G1 accumulation in the 29-bit-limb form (field29.hpp): same work split and
+// oversized-bucket hand-off as k_msm_accum; points come from the converted
+// copy of the bases, and each bucket is converted back to the standard form
+// when it is stored.
+__global__ void __launch_bounds__(256) k_msm_accum29(const uint32_t* __restrict__ entries,
+                                                     const uint32_t* __restrict__ offsets,
+                                                     const uint32_t* __restrict__ order, uint32_t nb, uint32_t lmax,
+                                                     const void* __restrict__ points29, void* __restrict__ buckets,
+                                                     uint32_t* __restrict__ counters, ChunkRec* __restrict__ chunks,
+                                                     LargeRec* __restrict__ large) {
+    uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= nb) return;
+    uint32_t b = order[tid];
+    uint32_t st = offsets[b], en = offsets[b + 1];
+    uint32_t cnt = en - st;
+    if (cnt > lmax) {
+        uint32_t nch = (cnt + CHUNK - 1) / CHUNK;
+        uint32_t first = atomicAdd(&counters[0], nch);
+        for (uint32_t j = 0; j < nch; j++) {
+            uint32_t s0 = st + j * CHUNK;
+            chunks[first + j] = {b, s0, min(CHUNK, en - s0), 0u};
+        }
+        uint32_t li = atomicAdd(&counters[1], 1u);
+        large[li] = {b, first, nch, 0u};
+        return;
+    }
+    Xyzz29 acc;
+    acc.inf = true;
+    for (uint32_t e = st; e < en; e++) {
+        uint32_t ref = entries[e];
+        F29 x, y;
+        f29_load_point(points29, ref & 0x7fffffffu, x, y);
+        if (f29_is_zero(x) && f29_is_zero(y)) continue;   // infinity
+        if (ref >> 31) y = f29_neg(y);
+        f29_madd(acc, x, y);
+    }
+    Xyzz<Fp> o = xyzz_inf<Fp>();
+    if (!acc.inf) o = {f29_to_std(acc.X), f29_to_std(acc.Y), f29_to_std(acc.ZZ), f29_to_std(acc.ZZZ)};
+    PointIO<Fp>::store_xyzz(buckets, b, o);
+}
+
+__global__ void k_points_to29(const void* __restrict__ pts, size_t n, void* __restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Aff<Fp> a = PointIO<Fp>::load_aff(pts, i);
+    uint32_t w[PT29_WORDS] = {};
+    if (!aff_is_inf(a)) {
+        const F29 x = f29_from_std(a.x), y = f29_from_std(a.y);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            w[k] = x.l[k];
+            w[9 + k] = y.l[k];
+        }
+    }
+    uint4* o = reinterpret_cast<uint4*>(out) + 5 * i;
+#pragma unroll
+    for (int k = 0; k < 5; k++) o[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+}
+
+void msm_points_to29(const void* d_points, size_t n, void* d_out, hipStream_t st) {
+    if (!n) return;
+    k_points_to29<<<cdiv(n, 256), 256, 0, st>>>(d_points, n, d_out);
+    ZK_HIP(hipGetLastError());
+}
+
 // tree-sum of one XYZZ per thread across the block (blockDim.x must be a power of 2)
 template <class T>
 __device__ Xyzz<T> block_sum(Xyzz<T> v, Xyzz<T>* sh) {
@@ -362,7 +430,7 @@ __global__ void __launch_bounds__(256) k_msm_merge(const uint32_t* __restrict__ 
 // b holding digit b+1: R_j = sum B_b, T_j = sum (b - j*L1 + 1) B_b, so the
 // window value is sum_j (T_j + j*L1*R_j).
 template <class T>
-__global__ void __launch_bounds__(128) k_msm_reduce1(const void* __restrict__ buckets, uint32_t K, int W,
+__global__ void __launch_bounds__(128) k_msm_reduce1(const void* __restrict__ buckets, uint32_t K, int W, uint32_t L1,
                                                      void* __restrict__ R, void* __restrict__ Tt) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t nseg = K / L1;
@@ -371,7 +439,7 @@ __global__ void __launch_bounds__(128) k_msm_reduce1(const void* __restrict__ bu
     size_t base = (size_t)w * K + (size_t)jj * L1;
     Xyzz<T> run = xyzz_inf<T>(), acc = xyzz_inf<T>();
 #pragma unroll 1
-    for (int b = L1 - 1; b >= 0; b--) {
+    for (int b = (int)L1 - 1; b >= 0; b--) {
         run = xyzz_add_i(run, PointIO<T>::load_xyzz(buckets, base + b));
         acc = xyzz_add_i(acc, run);
     }
@@ -477,17 +545,24 @@ void MsmScratch::ensure(const MsmGeom& g, size_t n_entries, bool g2) {
 }
 
 template <class T>
-static void launch_t(const MsmSort& sort, const void* d_points, MsmScratch& scr, MsmPartials& out, hipStream_t st) {
+static void launch_t(const MsmSort& sort, const void* d_points, const void* d_points29, MsmScratch& scr,
+                     MsmPartials& out, hipStream_t st) {
     const bool g2 = std::is_same<T, Fq2>::value;
     const MsmGeom g = msm_geom(sort.n, sort.g.c, g2);
     scr.ensure(g, (size_t)g.W * sort.n, g2);
     const uint32_t nb = (uint32_t)((size_t)g.W * g.K);
     fill_u32(scr.counters.p, 0u, 16, st);
     if (scr.ev_acc0) ZK_HIP(hipEventRecord(scr.ev_acc0, st));
-    k_msm_accum<T><<<cdiv(nb, 128), 128, 0, st>>>(sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(),
-                                                  sort.order.as<uint32_t>(), nb, sort.lmax, d_points,
-                                                  scr.buckets.p, scr.counters.as<uint32_t>(), scr.chunks.as<ChunkRec>(),
-                                                  scr.large.as<LargeRec>());
+    if (!g2 && d_points29)
+        k_msm_accum29<<<cdiv(nb, 128), 128, 0, st>>>(sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(),
+                                                     sort.order.as<uint32_t>(), nb, sort.lmax, d_points29, scr.buckets.p,
+                                                     scr.counters.as<uint32_t>(), scr.chunks.as<ChunkRec>(),
+                                                     scr.large.as<LargeRec>());
+    else
+        k_msm_accum<T><<<cdiv(nb, 128), 128, 0, st>>>(sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(),
+                                                      sort.order.as<uint32_t>(), nb, sort.lmax, d_points,
+                                                      scr.buckets.p, scr.counters.as<uint32_t>(),
+                                                      scr.chunks.as<ChunkRec>(), scr.large.as<LargeRec>());
     ZK_HIP(hipGetLastError());
     if (scr.ev_acc1) ZK_HIP(hipEventRecord(scr.ev_acc1, st));
     k_msm_chunks<T><<<512, 256, 0, st>>>(sort.entries.as<uint32_t>(), d_points, scr.counters.as<uint32_t>(),
@@ -497,10 +572,10 @@ static void launch_t(const MsmSort& sort, const void* d_points, MsmScratch& scr,
                                          scr.buckets.p);
     ZK_HIP(hipGetLastError());
     uint32_t nseg_all = (uint32_t)g.W * g.nseg;
-    k_msm_reduce1<T><<<cdiv(nseg_all, 128), 128, 0, st>>>(scr.buckets.p, g.K, g.W, scr.r1.p, scr.t1.p);
+    k_msm_reduce1<T><<<cdiv(nseg_all, 128), 128, 0, st>>>(scr.buckets.p, g.K, g.W, g.l1, scr.r1.p, scr.t1.p);
     ZK_HIP(hipGetLastError());
     constexpr int BLK = RedBlk<T>::V;
-    k_msm_reduce2<T, BLK><<<(unsigned)(g.W * g.nblk), BLK, 0, st>>>(scr.r1.p, scr.t1.p, g.nseg, ilog2(L1), g.nblk,
+    k_msm_reduce2<T, BLK><<<(unsigned)(g.W * g.nblk), BLK, 0, st>>>(scr.r1.p, scr.t1.p, g.nseg, ilog2(g.l1), g.nblk,
                                                                      scr.r2.p, scr.t2.p);
     ZK_HIP(hipGetLastError());
     size_t xb = g2 ? 256 : 128;
@@ -513,9 +588,10 @@ static void launch_t(const MsmSort& sort, const void* d_points, MsmScratch& scr,
     if (scr.ev_end) ZK_HIP(hipEventRecord(scr.ev_end, st));
 }
 
-void msm_launch(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, MsmPartials& out, hipStream_t st) {
-    if (g2) launch_t<Fq2>(sort, d_points, scr, out, st);
-    else launch_t<Fp>(sort, d_points, scr, out, st);
+void msm_launch(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, MsmPartials& out, hipStream_t st,
+                const void* d_points29) {
+    if (g2) launch_t<Fq2>(sort, d_points, nullptr, scr, out, st);
+    else launch_t<Fp>(sort, d_points, d_points29, scr, out, st);
 }
 
 // ---------------------------------------------------------------- host finish
@@ -532,7 +608,7 @@ static zkh::Xyzz<HT> finish_t(const MsmPartials& p) {
     const int lw = sizeof(HT) / 8;  // u64 words per coordinate
     const uint64_t* R = p.host.as<uint64_t>();
     const uint64_t* Tt = R + (size_t)g.W * g.nblk * 4 * lw;
-    int logU = ilog2(L1) + ilog2(L2) + ilog2(p.g2 ? RedBlk<Fq2>::V : RedBlk<Fp>::V);
+    int logU = ilog2(g.l1) + ilog2(L2) + ilog2(p.g2 ? RedBlk<Fq2>::V : RedBlk<Fp>::V);
     auto ld = [&](const uint64_t* base, size_t i) {
         const uint64_t* q = base + i * 4 * lw;
         return zkh::Xyzz<HT>{load_h<HT>(q), load_h<HT>(q + lw), load_h<HT>(q + 2 * lw), load_h<HT>(q + 3 * lw)};

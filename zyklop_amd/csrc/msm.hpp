@@ -27,7 +27,8 @@ struct MsmGeom {
     int c;          // window bits
     int W;          // windows = ceil(255 / c)
     uint32_t K;     // buckets per window = 2^(c-1)
-    uint32_t nseg;  // level-1 segments per window = K / L1
+    uint32_t l1;    // buckets per level-1 reduction thread
+    uint32_t nseg;  // level-1 segments per window = K / l1
     uint32_t nblk;  // level-2 blocks per window
 };
 
@@ -70,8 +71,14 @@ struct MsmScratch {
 // Launch bucket accumulation + GPU reduction for points (affine, Montgomery,
 // AoS: G1 64 B / G2 128 B per point) against a finished sort; queue the D2H of
 // the partials into out.host on st. The caller synchronises st before finish.
+// d_points29 (G1 only, optional): the same points in the 29-bit-limb form of
+// msm_points_to29; when given, buckets are accumulated in that form.
 void msm_launch(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, MsmPartials& out,
-                hipStream_t st);
+                hipStream_t st, const void* d_points29 = nullptr);
+
+// G1 affine points (standard form, 64 B each) -> 29-bit-limb form (80 B each)
+constexpr size_t PT29_BYTES = 80;
+void msm_points_to29(const void* d_points, size_t n, void* d_out, hipStream_t st);
 
 // Host finish: combine the partials into the MSM value.
 zkh::Xyzz<zkh::Fp> msm_finish_g1(const MsmPartials& p);

@@ -93,6 +93,68 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass(NttPtrs ptrs, int logM
     }
 }
 
+// Two consecutive passes j, j+1 (radix R1 = 2^LR1, then R2 = 2^LR2) in one
+// kernel, the intermediate result staged in LDS. With M = R1*S (pass j),
+// S = R2*S' (pass j+1), the R1*R2 points x[beta*M + b*S + b2*S' + c2]
+// (b < R1, b2 < R2) of one unit u = (beta, c2) touch no other unit in either
+// pass. A workgroup takes U consecutive units:
+//   step 1: thread (b2, u) runs pass j's DFT over b for column c = b2*S' + c2,
+//           multiplies output k by w_M^(c k) and writes LDS[k][b2][u];
+//   step 2: thread (k, u) runs pass j+1's DFT over b2 for sub-block (beta, k),
+//           multiplies output k2 by w_S^(c2 k2) and stores x[beta*M + k*S +
+//           k2*S' + c2].
+// Consecutive lanes take consecutive units, so global accesses stay coalesced
+// (runs of min(S', U) elements).
+template <int LR1, int LR2, int U>
+__global__ void __launch_bounds__((1 << LR2) * U) k_ntt_pass2(NttPtrs ptrs, int logM, size_t n, RootsR rt1, RootsR rt2,
+                                                              const void* __restrict__ tw1, const void* __restrict__ tw2,
+                                                              const void* __restrict__ pre) {
+    constexpr int R1 = 1 << LR1, R2 = 1 << LR2;
+    static_assert(R1 == R2, "one task per thread in both steps");
+    __shared__ Fr lds[R1 * R2 * U];
+    void* data = ptrs.in[blockIdx.y];
+    const int logS = logM - LR1, logS2 = logS - LR2;
+    const size_t S2 = (size_t)1 << logS2;
+    const int ul = threadIdx.x % U, q = threadIdx.x / U;   // unit lane, digit (b2 in step 1, k in step 2)
+    const size_t u = (size_t)blockIdx.x * U + ul;
+    const size_t beta = u >> logS2, c2 = u & (S2 - 1);
+    const size_t base = beta << logM;
+    const bool live = u < (n >> (LR1 + LR2));
+    if (live) {
+        // step 1: pass j over b for column c = q*S' + c2
+        const size_t c = ((size_t)q << logS2) + c2;
+        Fr x[R1];
+#pragma unroll
+        for (int b = 0; b < R1; b++) x[bitrev_c(b, LR1)] = load<FrCfg>(data, base + ((size_t)b << logS) + c);
+        if (pre) {
+#pragma unroll
+            for (int b = 0; b < R1; b++)
+                x[bitrev_c(b, LR1)] = mul<FrCfg>(x[bitrev_c(b, LR1)], load<FrCfg>(pre, base + ((size_t)b << logS) + c));
+        }
+        dft_reg<LR1>(x, rt1);
+#pragma unroll
+        for (int k = 0; k < R1; k++) {
+            Fr y = x[k];
+            if (k && c) y = mul<FrCfg>(y, load<FrCfg>(tw1, ((size_t)k << logS) + c));
+            lds[(k * R2 + q) * U + ul] = y;
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    // step 2: pass j+1 over b2 for sub-block (beta, k = q), column c2
+    Fr x[R2];
+#pragma unroll
+    for (int b2 = 0; b2 < R2; b2++) x[bitrev_c(b2, LR2)] = lds[(q * R2 + b2) * U + ul];
+    dft_reg<LR2>(x, rt2);
+    const size_t sub = base + ((size_t)q << logS);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; k2++) {
+        Fr y = x[k2];
+        if (k2 && c2) y = mul<FrCfg>(y, load<FrCfg>(tw2, ((size_t)k2 << logS2) + c2));
+        store<FrCfg>(data, sub + ((size_t)k2 << logS2) + c2, y);
+    }
+}
+
 // Last pass: thread = output column rv (digit-reversed row index); the row of
 // R contiguous points at unrev(rv)*R gives out[rv + O*k], k < R.
 template <int LR>
@@ -250,6 +312,14 @@ static void launch_pass(const NttPtrs& ptrs, int batch, int logM, size_t n, cons
     k_ntt_pass<LR><<<dim3(grid, batch), NTT_THREADS, 0, st>>>(ptrs, logM, n, rt, tw, pre);
 }
 
+constexpr int NTT2_UNITS = 32;
+
+static void launch_pass2(const NttPtrs& ptrs, int batch, int logM, size_t n, const RootsR& rt1, const RootsR& rt2,
+                         const void* tw1, const void* tw2, const void* pre, hipStream_t st) {
+    unsigned grid = cdiv(n >> 6, NTT2_UNITS);
+    k_ntt_pass2<3, 3, NTT2_UNITS><<<dim3(grid, batch), 8 * NTT2_UNITS, 0, st>>>(ptrs, logM, n, rt1, rt2, tw1, tw2, pre);
+}
+
 template <int LR>
 static void launch_final(const NttPtrs& ptrs, int batch, const NttDigits& dg, size_t n, const RootsR& rt, const void* pre,
                          const void* post, hipStream_t st) {
@@ -279,6 +349,16 @@ void ntt_run(const NttPlan& p, void* const* in, void* const* out, int batch, boo
             const int logM = p.log_n - consumed;
             const void* pj = j == 0 ? pre : nullptr;
             const void* twj = tw + p.tw_off[j] * 32;
+            // two radix-8 passes through LDS when both are regular passes
+            if (lrj == 3 && j + 1 < p.np - 1 && p.lr[j + 1] == 3) {
+                RootsR rt2;
+                memcpy(&rt2, inverse ? p.roots_inv[j + 1] : p.roots_fwd[j + 1], sizeof rt2);
+                launch_pass2(ptrs, batch, logM, p.n, rt, rt2, twj, tw + p.tw_off[j + 1] * 32, pj, st);
+                ZK_HIP(hipGetLastError());
+                consumed += 6;
+                j++;
+                continue;
+            }
             switch (lrj) {
                 case 1: launch_pass<1>(ptrs, batch, logM, p.n, rt, twj, pj, st); break;
                 case 2: launch_pass<2>(ptrs, batch, logM, p.n, rt, twj, pj, st); break;

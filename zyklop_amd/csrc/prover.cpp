@@ -91,7 +91,7 @@ void Context::ensure_comb() {
 // ------------------------------------------------------------------ circuits
 size_t Circuit::device_bytes() const {
     return rowA.bytes + colA.bytes + valA.bytes + rowB.bytes + colB.bytes + valB.bytes + A.bytes + B1.bytes + B2.bytes +
-           C.bytes + H.bytes;
+           C.bytes + H.bytes + A29.bytes + B1_29.bytes + C29.bytes + H29.bytes;
 }
 
 static void upload(DevBuf& d, const void* src, size_t bytes) {
@@ -137,6 +137,15 @@ static void finish_circuit_bases(Circuit& c, const uint8_t* A, const uint8_t* B1
     build_ext(c.B2, nullptr, 0, B2, nv, exB2, 128);
     build_ext(c.C, nullptr, l + 1, C, nv - l - 1, exC, 64);
     upload(c.H, H, (size_t)c.n * 64);
+    const size_t ne = nv + 4;
+    hipStream_t st = c.ctx->stream;
+    for (auto pr : {std::make_pair(&c.A, &c.A29), std::make_pair(&c.B1, &c.B1_29), std::make_pair(&c.C, &c.C29)}) {
+        pr.second->ensure(ne * PT29_BYTES);
+        msm_points_to29(pr.first->p, ne, pr.second->p, st);
+    }
+    c.H29.ensure((size_t)c.n * PT29_BYTES);
+    msm_points_to29(c.H.p, c.n, c.H29.p, st);
+    ZK_HIP(hipStreamSynchronize(st));
 }
 
 std::unique_ptr<Circuit> circuit_from_zkey(Context& ctx, const std::string& path) {
@@ -324,7 +333,7 @@ void ProveSlot::body(Context& ctx, const Circuit& c, ProofParts& out) {
     if (timed) ZK_HIP(hipEventRecord(ev[3], s0));
     h_final(aw.p, bw.p, cw.p, h.p, n, s0);
     sortH.run(h.p, n, window_h, s0);
-    msm_launch(sortH, c.H.p, false, scrH, out.h, s0);
+    msm_launch(sortH, c.H.p, false, scrH, out.h, s0, c.H29.p);
     // witness MSMs on s1 (one shared sort)
     ZK_HIP(hipStreamWaitEvent(s1, ev_scal, 0));
     if (timed) ZK_HIP(hipEventRecord(ev[4], s1));
@@ -335,9 +344,9 @@ void ProveSlot::body(Context& ctx, const Circuit& c, ProofParts& out) {
     // one MSM's latency-bound reduction overlaps another's accumulation
     for (hipStream_t p : {s2, s3, s4}) ZK_HIP(hipStreamWaitEvent(p, ev_done1, 0));
     msm_launch(sortW, c.B2.p, true, scrB2, out.b2, s4);
-    msm_launch(sortW, c.A.p, false, scrA, out.a, s1);
-    msm_launch(sortW, c.B1.p, false, scrB1, out.b1, s2);
-    msm_launch(sortW, c.C.p, false, scrC, out.c, s3);
+    msm_launch(sortW, c.A.p, false, scrA, out.a, s1, c.A29.p);
+    msm_launch(sortW, c.B1.p, false, scrB1, out.b1, s2, c.B1_29.p);
+    msm_launch(sortW, c.C.p, false, scrC, out.c, s3, c.C29.p);
     hipStream_t br[4] = {s1, s2, s3, s4};
     for (int k = 0; k < 4; k++) {
         ZK_HIP(hipEventRecord(ev_join[k], br[k]));
