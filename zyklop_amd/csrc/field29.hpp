@@ -68,6 +68,16 @@ ZK_DEV F29 f29_const(const uint32_t (&c)[9]) {
     return r;
 }
 
+// (lo * (-p^-1)) mod 2^29 with one full-rate v_mad_u64_u32 (a 32-bit
+// v_mul_lo_u32 would issue at quarter rate); its carry-out goes to an SGPR
+// pair that nothing reads
+ZK_DEV uint32_t f29_qword(uint32_t lo) {
+    uint64_t t, cc;
+    asm("v_mad_u64_u32 %0, %1, %2, %3, 0" : "=v"(t), "=s"(cc) : "v"(lo), "s"(f29c::NP));
+    (void)cc;
+    return (uint32_t)t & f29c::MASK;
+}
+
 // Montgomery product a*b/R' mod p (product scanning, one 64-bit accumulator)
 ZK_DEV F29 f29_mul(const F29& a, const F29& b) {
     using namespace f29c;
@@ -87,7 +97,7 @@ ZK_DEV F29 f29_mul(const F29& a, const F29& b) {
             if (i < k && j >= 0 && j < 9) acc += (uint64_t)m[i] * P[j];
         }
         if (k < 9) {
-            m[k] = ((uint32_t)acc * NP) & MASK;
+            m[k] = f29_qword((uint32_t)acc);
             acc += (uint64_t)m[k] * P[0];
         } else {
             t.l[k - 9] = (uint32_t)acc & MASK;
