@@ -269,11 +269,14 @@ ZK_DEV void f29_madd(Xyzz29& A, const F29& x2, const F29& y2) {
     A.ZZZ = f29_mul(A.ZZZ, PPP);
 }
 
-// G1 affine point in this form: x[9], y[9], 2 words of padding (80 bytes)
+// G1 affine point in this form: x[9], y[9] in the first 80 bytes of a
+// 128-byte slot, so that the random gathers of the bucket loop touch one cache
+// line per point
 constexpr int PT29_WORDS = 20;
+constexpr int PT29_SLOT_UINT4 = 8;
 
 ZK_DEV void f29_load_point(const void* base, size_t idx, F29& x, F29& y) {
-    const uint4* p = reinterpret_cast<const uint4*>(base) + 5 * idx;
+    const uint4* p = reinterpret_cast<const uint4*>(base) + PT29_SLOT_UINT4 * idx;
     const uint4 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
     const uint32_t w[20] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y,
                             c.z, c.w, d.x, d.y, d.z, d.w, e.x, e.y, e.z, e.w};

@@ -356,9 +356,10 @@ __global__ void k_points_to29(const void* __restrict__ pts, size_t n, void* __re
             w[9 + k] = y.l[k];
         }
     }
-    uint4* o = reinterpret_cast<uint4*>(out) + 5 * i;
+    uint4* o = reinterpret_cast<uint4*>(out) + PT29_SLOT_UINT4 * i;
 #pragma unroll
-    for (int k = 0; k < 5; k++) o[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+    for (int k = 0; k < PT29_SLOT_UINT4; k++)
+        o[k] = k < 5 ? make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]) : make_uint4(0u, 0u, 0u, 0u);
 }
 
 void msm_points_to29(const void* d_points, size_t n, void* d_out, hipStream_t st) {

@@ -76,8 +76,9 @@ struct MsmScratch {
 void msm_launch(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, MsmPartials& out,
                 hipStream_t st, const void* d_points29 = nullptr);
 
-// G1 affine points (standard form, 64 B each) -> 29-bit-limb form (80 B each)
-constexpr size_t PT29_BYTES = 80;
+// G1 affine points (standard form, 64 B each) -> 29-bit-limb form (80 B of
+// limbs in a 128 B slot each)
+constexpr size_t PT29_BYTES = 128;
 void msm_points_to29(const void* d_points, size_t n, void* d_out, hipStream_t st);
 
 // Host finish: combine the partials into the MSM value.
