@@ -97,6 +97,32 @@ def test_msm_g1_random_vs_oracle(ctx, n, window):
     assert inf == einf and np.array_equal(out, exp)
 
 
+@pytest.mark.parametrize("window", [0, 13])
+def test_msm_g1_equal_points_in_bucket(ctx, window):
+    """Buckets that receive a point equal to their running sum (doubling) or its
+    negative (the sum becomes infinity, then restarts): two-entry buckets make
+    the order inside the bucket irrelevant. Checked against the closed form."""
+    a, b = 123456789, 987654321
+    cases = [
+        ([a, a], [1, 1]),                      # P + P: doubling branch
+        ([a, pyref.R - a], [1, 1]),            # P + (-P): infinity
+        ([a, pyref.R - a, b], [5, 5, 5]),      # cancellation, then another point
+        ([a, a, a], [7, 7, 7]),                # doubling, then a plain add
+        ([b, b], [pyref.R - 1, pyref.R - 1]),  # negative digits on equal points
+    ]
+    for ks, ss in cases:
+        pts = mont_pts_g1(ks)
+        sc = O.arr(ss)
+        out, inf = ctx.msm_g1(pts, sc, window)
+        tot = sum(s * k for s, k in zip(ss, ks)) % pyref.R
+        if tot == 0:
+            assert inf, (ks, ss)
+            continue
+        e, _ = O.g1_mul(O.g1_gen(), tot)
+        exp = O.field_op(O.FP, O.TO_MONT, e.reshape(2, 4)).reshape(8)
+        assert not inf and np.array_equal(out, exp), (ks, ss, window)
+
+
 def test_msm_g1_skewed_large_buckets(ctx):
     """Most scalars equal 1 or 2: a few buckets get thousands of entries and take
     the chunked (oversized-bucket) path; the rest stay small."""
@@ -142,6 +168,18 @@ def test_msm_g2_random_vs_oracle(ctx, n):
     out, inf = ctx.msm_g2(pts, sc)
     exp, einf = oracle_msm_g2(pts, sc)
     assert inf == einf and np.array_equal(out, exp)
+
+
+def test_msm_g2_equal_points_in_bucket(ctx):
+    """G2 buckets receiving their own running sum (doubling) or its negative."""
+    a, b = 1122334455, 5544332211
+    for ks, ss in [([a, a], [1, 1]), ([a, pyref.R - a], [1, 1]), ([a, pyref.R - a, b], [3, 3, 3]),
+                   ([b, b], [pyref.R - 1, pyref.R - 1])]:
+        pts = mont_pts_g2(ks)
+        sc = O.arr(ss)
+        out, inf = ctx.msm_g2(pts, sc)
+        exp, einf = oracle_msm_g2(pts, sc)
+        assert inf == einf and np.array_equal(out, exp), (ks, ss)
 
 
 # ------------------------------------------------------------------ NTT
