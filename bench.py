@@ -43,10 +43,10 @@ MACS_PER_FPMUL = 136                    # 8x8 products + 8x8 reduction + 8 quoti
 MADD_FPMUL = 10                         # XYZZ mixed add (madd-2008-s): 8M + 2S
 MADD_FPMUL_G2 = 30                      # the same in Fp2: 10 Fp2 products x 3 Fp products (Karatsuba)
 CALIB_BYTES = 384 << 20                 # device copy before the probes (profiler counter calibration)
-# HBM bytes per entry of k_msm_accum<Fp> from rocprofv3 FETCH_SIZE + WRITE_SIZE
-# runs of the probe (c = 17, 15,790,005 entries: 3.428 GB per launch), scaled by
+# HBM bytes per entry of the G1 accumulation kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE
+# runs of the probe (c = 17, 15,790,005 entries: 3.039 GB per launch), scaled by
 # the calibration copy; profiles/msm_accum_m2p20_hbm_counters.md
-HBM_BYTES_PER_ENTRY_G1 = 3.428e9 / 15790005
+HBM_BYTES_PER_ENTRY_G1 = 3.039e9 / 15790005
 
 
 def log(msg):
@@ -260,14 +260,16 @@ def section_probes(args, zk, ctx, circ):
         f"accumulation {t1[2]:.3f} ms over {int(units)} entries -> {achieved:.2f} TMAC/s")
     log(f"[bench] isolated G2 witness MSM: total {t2[0]:.3f} ms, accumulation {t2[2]:.3f} ms -> {g2_achieved:.2f} TMAC/s")
     roofline = {
-        "bound": "valu", "kernel": "k_msm_accum<Fp> (G1 bucket accumulation)",
+        "bound": "valu", "kernel": "k_msm_accum29 (G1 bucket accumulation, 9 x 29-bit limbs)",
         "achieved": round(achieved, 3), "peak": round(PEAK_MAC_TS, 2), "unit": "TMAC/s",
         "frac": round(achieved / PEAK_MAC_TS, 4), "traffic": int(units * HBM_BYTES_PER_ENTRY_G1),
         "traffic_note": ("HBM bytes per launch: per-entry figure from the profiler's FETCH_SIZE + WRITE_SIZE runs "
-                         "of this probe (calibrated by a 384 MiB copy) x entries; algorithmic bytes 68 per entry"),
-        "algorithmic_bytes": int(units * 68),
+                         "of this probe (calibrated by a 384 MiB copy) x entries; algorithmic bytes 84 per entry "
+                         "(80 B point + 4 B entry)"),
+        "algorithmic_bytes": int(units * 84),
         "avg_launch_ms": round(t_launch * 1e3, 4), "units_per_launch": int(units),
-        "unit_def": "one signed-digit (point, bucket) entry = one XYZZ mixed add = 10 Fp-mul x 136 32x32 MACs",
+        "unit_def": ("one signed-digit (point, bucket) entry = one XYZZ mixed add = 10 Fp products x 136 32x32 "
+                     "MACs (the algorithmic count of a 256-bit Montgomery product, whatever the limb size)"),
         "probe": (f"G1 MSM over the resident A' bases, n={n_ext}, c={c_w}, alone on the device, "
                   f"{args.probe_reps} launches, HIP events on the MSM stream"),
         "msm_total_ms": r3(t1[0]), "sort_ms": r3(t1[1]),
