@@ -126,16 +126,16 @@ void msm_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t
     zk::MsmScratch scr;
     zk::MsmPartials part;
     scr.enable_timing();
-    // G1 buckets are accumulated in the 29-bit-limb form: convert the points first
+    // buckets are accumulated in the 29-bit-limb form: convert the points first
     zk::DevBuf p29;
-    if (!g2 && n) {
-        p29.ensure(n * zk::PT29_BYTES);
-        zk::msm_points_to29(d_points, n, p29.p, st);
+    if (n) {
+        p29.ensure(n * (g2 ? zk::PT29Q_BYTES : zk::PT29_BYTES));
+        zk::msm_points_to29(d_points, n, p29.p, g2, st);
     }
     ZK_HIP(hipEventRecord(ctx->ev0, st));
     sort.run(d_scalars, n, window, st);
     ZK_HIP(hipEventRecord(ctx->ev1, st));
-    zk::msm_launch(sort, d_points, g2, scr, part, st, g2 ? nullptr : p29.p);
+    zk::msm_launch(sort, d_points, g2, scr, part, st, p29.p);
     ZK_HIP(hipEventRecord(ctx->ev2, st));
     ZK_HIP(hipStreamSynchronize(st));
     ZK_HIP(hipEventElapsedTime(&ctx->last_ms[0], ctx->ev0, ctx->ev2));

@@ -44,6 +44,10 @@ constexpr uint32_t P4[9] = {0x21f3f51cu, 0x241182dau, 0x31ca8d3bu, 0x2b548b42u, 
                             0x2b6d0301u, 0x229b8503u, 0x397098cfu, 0x00c19138u};
 constexpr uint32_t P12[9] = {0x65dbdf54u, 0x6c34888eu, 0x955fa7b1u, 0x81fda1c6u, 0xa246319du,
                              0x82470903u, 0x67d28f09u, 0xac51ca6du, 0x0244b3a8u};
+// 8p with every limb but the top one in [2^30, 2^31): 8p - a - b never
+// borrows for normalized a, b
+constexpr uint32_t P8[9] = {0x43e7ea38u, 0x482305b4u, 0x43951a76u, 0x56a91685u, 0x4c2ecbbeu,
+                            0x56da0603u, 0x45370a06u, 0x52e1319eu, 0x01832271u};
 // 3p, 4p, 5p normalized (a sub4 result is 0 mod p iff it equals one of them)
 constexpr uint32_t P3N[9] = {0x0976f7d5u, 0x030d2224u, 0x1557e9edu, 0x087f6872u, 0x00918c68u,
                              0x0891c242u, 0x01f4a3c3u, 0x0b14729cu, 0x00912cebu};
@@ -103,6 +107,10 @@ ZK_DEV F29 f29_mul(const F29& a, const F29& b) {
             t.l[k - 9] = (uint32_t)acc & MASK;
         }
         acc >>= 29;
+        // keeps the columns of consecutive products in program order: without it
+        // the scheduler interleaves independent products (such as the three
+        // Karatsuba products of Fp2) and runs out of registers
+        asm volatile("" : "+v"(acc));
     }
     t.l[8] = (uint32_t)acc;
     return t;
@@ -285,6 +293,160 @@ ZK_DEV void f29_load_point(const void* base, size_t idx, F29& x, F29& y) {
         x.l[i] = w[i];
         y.l[i] = w[9 + i];
     }
+}
+
+// ---------------------------------------------------------------- Fq2 (G2)
+// Fp2 = Fp[u]/(u^2 + 1) over this form, component-wise bounds as above.
+struct F29q {
+    F29 c0, c1;
+};
+
+ZK_DEV F29 f29_add(const F29& a, const F29& b) {
+    F29 r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.l[i] = a.l[i] + b.l[i];
+    return r;
+}
+
+// a: components normalized with value < 6p; b: components N. Karatsuba: three
+// products, c0 = a0b0 - a1b1, c1 = (a0+a1)(b0+b1) - a0b0 - a1b1; result N.
+ZK_DEV F29q f29q_mul(const F29q& a, const F29q& b) {
+    const F29 t0 = f29_mul(a.c0, b.c0);
+    const F29 t1 = f29_mul(a.c1, b.c1);
+    const F29 t2 = f29_mul(f29_add(a.c0, a.c1), f29_add(b.c0, b.c1));   // 12p x 4p < R'p
+    F29 c1;
+#pragma unroll
+    for (int i = 0; i < 9; i++) c1.l[i] = t2.l[i] + f29c::P8[i] - t0.l[i] - t1.l[i];
+    return {f29_reduce(f29_norm(f29_sub4(t0, t1))), f29_reduce(f29_norm(c1))};
+}
+
+// a: components N. (a0+a1)(a0-a1) + (2 a0 a1) u; result N.
+ZK_DEV F29q f29q_sqr(const F29q& a) {
+    const F29 d = f29_norm(f29_sub4(a.c0, a.c1));
+    return {f29_mul(f29_add(a.c0, a.c1), d), f29_mul(f29_add(a.c0, a.c0), a.c1)};
+}
+
+// a - b component-wise for N inputs: reduced to N, or only normalized (< 6p)
+ZK_DEV F29q f29q_sub(const F29q& a, const F29q& b) {
+    return {f29_reduce(f29_norm(f29_sub4(a.c0, b.c0))), f29_reduce(f29_norm(f29_sub4(a.c1, b.c1)))};
+}
+ZK_DEV F29q f29q_sub_nr(const F29q& a, const F29q& b) {
+    return {f29_norm(f29_sub4(a.c0, b.c0)), f29_norm(f29_sub4(a.c1, b.c1))};
+}
+
+ZK_DEV F29q f29q_from_std(const Fq2& x) { return {f29_from_std(x.c0), f29_from_std(x.c1)}; }
+ZK_DEV Fq2 f29q_to_std(const F29q& v) { return {f29_to_std(v.c0), f29_to_std(v.c1)}; }
+
+// G2 affine point in this form: x.c0, x.c1, y.c0, y.c1 (9 limbs each) in the
+// first 144 bytes of a 256-byte slot (two cache lines per gather)
+constexpr int PT29Q_WORDS = 36;
+constexpr int PT29Q_SLOT_UINT4 = 16;
+
+ZK_DEV void f29q_load_point(const void* base, size_t idx, F29q& x, F29q& y) {
+    const uint4* p = reinterpret_cast<const uint4*>(base) + PT29Q_SLOT_UINT4 * idx;
+    uint32_t w[36];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const uint4 v = p[k];
+        w[4 * k] = v.x;
+        w[4 * k + 1] = v.y;
+        w[4 * k + 2] = v.z;
+        w[4 * k + 3] = v.w;
+    }
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        x.c0.l[i] = w[i];
+        x.c1.l[i] = w[9 + i];
+        y.c0.l[i] = w[18 + i];
+        y.c1.l[i] = w[27 + i];
+    }
+}
+
+// A G2 XYZZ accumulator kept in LDS: 4 coordinates x 18 limbs, laid out
+// [coordinate][limb][lane] over a block of NT threads (consecutive lanes hit
+// consecutive banks). Coordinates are loaded where they are used, so the
+// accumulator does not occupy 72 registers for the whole loop.
+template <int NT>
+struct AccLds {
+    uint32_t* base;   // &lds[0][0][lane]
+    enum { X = 0, Y = 1, ZZ = 2, ZZZ = 3 };
+    ZK_DEV F29q ld(int c) const {
+        asm volatile("" ::: "memory");
+        F29q v;
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            v.c0.l[i] = base[(c * 18 + i) * NT];
+            v.c1.l[i] = base[(c * 18 + 9 + i) * NT];
+        }
+        return v;
+    }
+    ZK_DEV void st(int c, const F29q& v) const {
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            base[(c * 18 + i) * NT] = v.c0.l[i];
+            base[(c * 18 + 9 + i) * NT] = v.c1.l[i];
+        }
+        asm volatile("" ::: "memory");
+    }
+};
+
+// madd-2008-s over Fp2 on an LDS accumulator, acc += (x2, y2) with components
+// of x2 < p and y2 < 2p (normalized); inf tracks the empty/infinite sum. The
+// equal-x cases go through the standard-form formulas.
+template <int NT>
+ZK_DEV void f29q_madd(const AccLds<NT>& A, bool& inf, const F29q& x2, const F29q& y2) {
+    using namespace f29c;
+    using L = AccLds<NT>;
+    if (inf) {
+        const F29q one = {f29_const(ONE), F29{}};
+        A.st(L::X, x2);
+        A.st(L::Y, y2);
+        A.st(L::ZZ, one);
+        A.st(L::ZZZ, one);
+        inf = false;
+        return;
+    }
+    const F29q U2 = f29q_mul(x2, A.ld(L::ZZ));
+    const F29q S2 = f29q_mul(y2, A.ld(L::ZZZ));
+    const F29q Pn = f29q_sub_nr(U2, A.ld(L::X));   // normalized, in (2p, 6p): zero test
+    const F29q Rn = f29q_sub_nr(S2, A.ld(L::Y));
+    if (f29_sub4_is_zero(Pn.c0) && f29_sub4_is_zero(Pn.c1)) {
+        if (f29_sub4_is_zero(Rn.c0) && f29_sub4_is_zero(Rn.c1)) {
+            Aff<Fq2> a;
+            a.x = f29q_to_std(x2);
+            a.y = f29q_to_std(y2);
+            const Xyzz<Fq2> d = xyzz_mdbl(a);
+            A.st(L::X, f29q_from_std(d.X));
+            A.st(L::Y, f29q_from_std(d.Y));
+            A.st(L::ZZ, f29q_from_std(d.ZZ));
+            A.st(L::ZZZ, f29q_from_std(d.ZZZ));
+        } else {
+            inf = true;
+        }
+        return;
+    }
+    const F29q P_ = {f29_reduce(Pn.c0), f29_reduce(Pn.c1)};
+    const F29q PP = f29q_sqr(P_);
+    const F29q PPP = f29q_mul(P_, PP);
+    A.st(L::ZZ, f29q_mul(A.ld(L::ZZ), PP));
+    A.st(L::ZZZ, f29q_mul(A.ld(L::ZZZ), PPP));
+    const F29q Q = f29q_mul(A.ld(L::X), PP);
+    const F29q R_ = {f29_reduce(Rn.c0), f29_reduce(Rn.c1)};
+    F29q X3;
+    {
+        const F29q RR = f29q_sqr(R_);
+        F29 x0, x1;
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            x0.l[i] = (P12[i] - PPP.c0.l[i] - 2 * Q.c0.l[i]) + RR.c0.l[i];
+            x1.l[i] = (P12[i] - PPP.c1.l[i] - 2 * Q.c1.l[i]) + RR.c1.l[i];
+        }
+        X3 = {f29_reduce(f29_norm(x0)), f29_reduce(f29_norm(x1))};
+    }
+    A.st(L::X, X3);
+    const F29q m1 = f29q_mul(f29q_sub_nr(Q, X3), R_);
+    const F29q m2 = f29q_mul(A.ld(L::Y), PPP);
+    A.st(L::Y, f29q_sub(m1, m2));
 }
 
 }  // namespace zk

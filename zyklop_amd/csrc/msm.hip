@@ -343,6 +343,72 @@ __global__ void __launch_bounds__(256) k_msm_accum29(const uint32_t* __restrict_
     PointIO<Fp>::store_xyzz(buckets, b, o);
 }
 
+// G2 accumulation in the 29-bit-limb form: the Fp2 state does not fit the
+// register file at two waves per SIMD, so each thread's bucket accumulator
+// lives in LDS (field29.hpp AccLds) and its coordinates are loaded where used.
+constexpr int ACC29Q_THREADS = 128;
+__global__ void __launch_bounds__(ACC29Q_THREADS) k_msm_accum29q(const uint32_t* __restrict__ entries,
+                                                                 const uint32_t* __restrict__ offsets,
+                                                                 const uint32_t* __restrict__ order, uint32_t nb,
+                                                                 uint32_t lmax, const void* __restrict__ points29,
+                                                                 void* __restrict__ buckets,
+                                                                 uint32_t* __restrict__ counters,
+                                                                 ChunkRec* __restrict__ chunks,
+                                                                 LargeRec* __restrict__ large) {
+    __shared__ uint32_t lds[4 * 18 * ACC29Q_THREADS];
+    uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= nb) return;
+    uint32_t b = order[tid];
+    uint32_t st = offsets[b], en = offsets[b + 1];
+    uint32_t cnt = en - st;
+    if (cnt > lmax) {
+        uint32_t nch = (cnt + CHUNK - 1) / CHUNK;
+        uint32_t first = atomicAdd(&counters[0], nch);
+        for (uint32_t j = 0; j < nch; j++) {
+            uint32_t s0 = st + j * CHUNK;
+            chunks[first + j] = {b, s0, min(CHUNK, en - s0), 0u};
+        }
+        uint32_t li = atomicAdd(&counters[1], 1u);
+        large[li] = {b, first, nch, 0u};
+        return;
+    }
+    const AccLds<ACC29Q_THREADS> A{lds + threadIdx.x};
+    bool inf = true;
+    for (uint32_t e = st; e < en; e++) {
+        uint32_t ref = entries[e];
+        F29q x, y;
+        f29q_load_point(points29, ref & 0x7fffffffu, x, y);
+        if (f29_is_zero(x.c0) && f29_is_zero(x.c1) && f29_is_zero(y.c0) && f29_is_zero(y.c1)) continue;
+        if (ref >> 31) y = {f29_neg(y.c0), f29_neg(y.c1)};
+        f29q_madd(A, inf, x, y);
+    }
+    using L = AccLds<ACC29Q_THREADS>;
+    Xyzz<Fq2> o = xyzz_inf<Fq2>();
+    if (!inf) o = {f29q_to_std(A.ld(L::X)), f29q_to_std(A.ld(L::Y)), f29q_to_std(A.ld(L::ZZ)), f29q_to_std(A.ld(L::ZZZ))};
+    PointIO<Fq2>::store_xyzz(buckets, b, o);
+}
+
+__global__ void k_points_to29q(const void* __restrict__ pts, size_t n, void* __restrict__ out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Aff<Fq2> a = PointIO<Fq2>::load_aff(pts, i);
+    uint32_t w[PT29Q_WORDS] = {};
+    if (!aff_is_inf(a)) {
+        const F29q x = f29q_from_std(a.x), y = f29q_from_std(a.y);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            w[k] = x.c0.l[k];
+            w[9 + k] = x.c1.l[k];
+            w[18 + k] = y.c0.l[k];
+            w[27 + k] = y.c1.l[k];
+        }
+    }
+    uint4* o = reinterpret_cast<uint4*>(out) + PT29Q_SLOT_UINT4 * i;
+#pragma unroll
+    for (int k = 0; k < PT29Q_SLOT_UINT4; k++)
+        o[k] = k < 9 ? make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]) : make_uint4(0u, 0u, 0u, 0u);
+}
+
 __global__ void k_points_to29(const void* __restrict__ pts, size_t n, void* __restrict__ out) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -362,9 +428,10 @@ __global__ void k_points_to29(const void* __restrict__ pts, size_t n, void* __re
         o[k] = k < 5 ? make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]) : make_uint4(0u, 0u, 0u, 0u);
 }
 
-void msm_points_to29(const void* d_points, size_t n, void* d_out, hipStream_t st) {
+void msm_points_to29(const void* d_points, size_t n, void* d_out, bool g2, hipStream_t st) {
     if (!n) return;
-    k_points_to29<<<cdiv(n, 256), 256, 0, st>>>(d_points, n, d_out);
+    if (g2) k_points_to29q<<<cdiv(n, 256), 256, 0, st>>>(d_points, n, d_out);
+    else k_points_to29<<<cdiv(n, 256), 256, 0, st>>>(d_points, n, d_out);
     ZK_HIP(hipGetLastError());
 }
 
@@ -554,7 +621,11 @@ static void launch_t(const MsmSort& sort, const void* d_points, const void* d_po
     const uint32_t nb = (uint32_t)((size_t)g.W * g.K);
     fill_u32(scr.counters.p, 0u, 16, st);
     if (scr.ev_acc0) ZK_HIP(hipEventRecord(scr.ev_acc0, st));
-    if (!g2 && d_points29)
+    if (g2 && d_points29)
+        k_msm_accum29q<<<cdiv(nb, ACC29Q_THREADS), ACC29Q_THREADS, 0, st>>>(
+            sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(), sort.order.as<uint32_t>(), nb, sort.lmax,
+            d_points29, scr.buckets.p, scr.counters.as<uint32_t>(), scr.chunks.as<ChunkRec>(), scr.large.as<LargeRec>());
+    else if (!g2 && d_points29)
         k_msm_accum29<<<cdiv(nb, 128), 128, 0, st>>>(sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(),
                                                      sort.order.as<uint32_t>(), nb, sort.lmax, d_points29, scr.buckets.p,
                                                      scr.counters.as<uint32_t>(), scr.chunks.as<ChunkRec>(),
@@ -591,7 +662,7 @@ static void launch_t(const MsmSort& sort, const void* d_points, const void* d_po
 
 void msm_launch(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, MsmPartials& out, hipStream_t st,
                 const void* d_points29) {
-    if (g2) launch_t<Fq2>(sort, d_points, nullptr, scr, out, st);
+    if (g2) launch_t<Fq2>(sort, d_points, d_points29, scr, out, st);
     else launch_t<Fp>(sort, d_points, d_points29, scr, out, st);
 }
 

@@ -71,15 +71,15 @@ struct MsmScratch {
 // Launch bucket accumulation + GPU reduction for points (affine, Montgomery,
 // AoS: G1 64 B / G2 128 B per point) against a finished sort; queue the D2H of
 // the partials into out.host on st. The caller synchronises st before finish.
-// d_points29 (G1 only, optional): the same points in the 29-bit-limb form of
+// d_points29 (optional): the same points in the 29-bit-limb form of
 // msm_points_to29; when given, buckets are accumulated in that form.
 void msm_launch(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, MsmPartials& out,
                 hipStream_t st, const void* d_points29 = nullptr);
 
-// G1 affine points (standard form, 64 B each) -> 29-bit-limb form (80 B of
-// limbs in a 128 B slot each)
-constexpr size_t PT29_BYTES = 128;
-void msm_points_to29(const void* d_points, size_t n, void* d_out, hipStream_t st);
+// affine points (standard form, G1 64 B / G2 128 B each) -> 29-bit-limb form
+// (G1: 80 B of limbs in a 128 B slot; G2: 144 B in a 256 B slot)
+constexpr size_t PT29_BYTES = 128, PT29Q_BYTES = 256;
+void msm_points_to29(const void* d_points, size_t n, void* d_out, bool g2, hipStream_t st);
 
 // Host finish: combine the partials into the MSM value.
 zkh::Xyzz<zkh::Fp> msm_finish_g1(const MsmPartials& p);

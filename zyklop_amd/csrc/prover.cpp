@@ -91,7 +91,7 @@ void Context::ensure_comb() {
 // ------------------------------------------------------------------ circuits
 size_t Circuit::device_bytes() const {
     return rowA.bytes + colA.bytes + valA.bytes + rowB.bytes + colB.bytes + valB.bytes + A.bytes + B1.bytes + B2.bytes +
-           C.bytes + H.bytes + A29.bytes + B1_29.bytes + C29.bytes + H29.bytes;
+           C.bytes + H.bytes + A29.bytes + B1_29.bytes + B2_29.bytes + C29.bytes + H29.bytes;
 }
 
 static void upload(DevBuf& d, const void* src, size_t bytes) {
@@ -141,10 +141,12 @@ static void finish_circuit_bases(Circuit& c, const uint8_t* A, const uint8_t* B1
     hipStream_t st = c.ctx->stream;
     for (auto pr : {std::make_pair(&c.A, &c.A29), std::make_pair(&c.B1, &c.B1_29), std::make_pair(&c.C, &c.C29)}) {
         pr.second->ensure(ne * PT29_BYTES);
-        msm_points_to29(pr.first->p, ne, pr.second->p, st);
+        msm_points_to29(pr.first->p, ne, pr.second->p, false, st);
     }
+    c.B2_29.ensure(ne * PT29Q_BYTES);
+    msm_points_to29(c.B2.p, ne, c.B2_29.p, true, st);
     c.H29.ensure((size_t)c.n * PT29_BYTES);
-    msm_points_to29(c.H.p, c.n, c.H29.p, st);
+    msm_points_to29(c.H.p, c.n, c.H29.p, false, st);
     ZK_HIP(hipStreamSynchronize(st));
 }
 
@@ -343,7 +345,7 @@ void ProveSlot::body(Context& ctx, const Circuit& c, ProofParts& out) {
     // the four witness MSMs read the same sorted entries on their own streams:
     // one MSM's latency-bound reduction overlaps another's accumulation
     for (hipStream_t p : {s2, s3, s4}) ZK_HIP(hipStreamWaitEvent(p, ev_done1, 0));
-    msm_launch(sortW, c.B2.p, true, scrB2, out.b2, s4);
+    msm_launch(sortW, c.B2.p, true, scrB2, out.b2, s4, c.B2_29.p);
     msm_launch(sortW, c.A.p, false, scrA, out.a, s1, c.A29.p);
     msm_launch(sortW, c.B1.p, false, scrB1, out.b1, s2, c.B1_29.p);
     msm_launch(sortW, c.C.p, false, scrC, out.c, s3, c.C29.p);

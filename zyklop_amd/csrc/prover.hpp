@@ -40,8 +40,8 @@ struct Circuit {
     DevBuf rowA, colA, valA, rowB, colB, valB;
     // extended bases, n_vars + 4 points each (see header comment); H: n points
     DevBuf A, B1, B2, C, H;
-    // the G1 bases again in the 29-bit-limb form of the bucket accumulation
-    DevBuf A29, B1_29, C29, H29;
+    // the bases again in the 29-bit-limb form of the bucket accumulation
+    DevBuf A29, B1_29, B2_29, C29, H29;
     ZkeyHeader hdr;               // VK in file layout
     std::vector<uint8_t> coefs;   // §4 bytes, kept to write the circuit back out
     std::vector<uint64_t> ic;     // (l+1) x 8

@@ -71,7 +71,7 @@ float time_window(Context& ctx, const Circuit& circ, int kind, int c, const void
             msm_launch(tb.sort, circ.A.p, false, tb.g1, tb.part, st, circ.A29.p);
             msm_launch(tb.sort, circ.B1.p, false, tb.g1, tb.part, st, circ.B1_29.p);
             msm_launch(tb.sort, circ.C.p, false, tb.g1, tb.part, st, circ.C29.p);
-            msm_launch(tb.sort, circ.B2.p, true, tb.g2, tb.part, st);
+            msm_launch(tb.sort, circ.B2.p, true, tb.g2, tb.part, st, circ.B2_29.p);
         } else {
             msm_launch(tb.sort, circ.H.p, false, tb.g1, tb.part, st, circ.H29.p);
         }
