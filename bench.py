@@ -44,9 +44,9 @@ MADD_FPMUL = 10                         # XYZZ mixed add (madd-2008-s): 8M + 2S
 MADD_FPMUL_G2 = 30                      # the same in Fp2: 10 Fp2 products x 3 Fp products (Karatsuba)
 CALIB_BYTES = 384 << 20                 # device copy before the probes (profiler counter calibration)
 # HBM bytes per entry of the G1 accumulation kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE
-# runs of the probe (c = 17, 15,790,005 entries: 3.039 GB per launch), scaled by
+# runs of the probe (c = 17, 15,790,005 entries: 3.010 GB per launch), scaled by
 # the calibration copy; profiles/msm_accum_m2p20_hbm_counters.md
-HBM_BYTES_PER_ENTRY_G1 = 3.039e9 / 15790005
+HBM_BYTES_PER_ENTRY_G1 = 3.010e9 / 15790005
 
 
 def log(msg):
@@ -273,7 +273,7 @@ def section_probes(args, zk, ctx, circ):
         "probe": (f"G1 MSM over the resident A' bases, n={n_ext}, c={c_w}, alone on the device, "
                   f"{args.probe_reps} launches, HIP events on the MSM stream"),
         "msm_total_ms": r3(t1[0]), "sort_ms": r3(t1[1]),
-        "g2": {"kernel": "k_msm_accum<Fq2>", "achieved": round(g2_achieved, 3),
+        "g2": {"kernel": "k_msm_accum29q (G2, 29-bit limbs, accumulator in LDS)", "achieved": round(g2_achieved, 3),
                "frac": round(g2_achieved / PEAK_MAC_TS, 4), "avg_launch_ms": r3(t2[2]),
                "msm_total_ms": r3(t2[0]), "unit_def": "entry = Fp2 XYZZ mixed add = 30 Fp-mul x 136 MACs"},
     }

@@ -266,46 +266,12 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------- buckets
-// One thread per bucket, in the sort's accumulation order (longest first, so
-// a wave's lanes carry near-equal loads and the launch has no long tail).
-// Buckets above lmax entries go to the chunked path.
-template <class T>
-__global__ void __launch_bounds__(256) k_msm_accum(const uint32_t* __restrict__ entries, const uint32_t* __restrict__ offsets,
-                                                   const uint32_t* __restrict__ order,
-                                                   uint32_t nb, uint32_t lmax, const void* __restrict__ points,
-                                                   void* __restrict__ buckets, uint32_t* __restrict__ counters,
-                                                   ChunkRec* __restrict__ chunks, LargeRec* __restrict__ large) {
-    uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= nb) return;
-    uint32_t b = order[tid];
-    uint32_t st = offsets[b], en = offsets[b + 1];
-    uint32_t cnt = en - st;
-    if (cnt > lmax) {
-        uint32_t nch = (cnt + CHUNK - 1) / CHUNK;
-        uint32_t first = atomicAdd(&counters[0], nch);
-        for (uint32_t j = 0; j < nch; j++) {
-            uint32_t s0 = st + j * CHUNK;
-            chunks[first + j] = {b, s0, min(CHUNK, en - s0), 0u};
-        }
-        uint32_t li = atomicAdd(&counters[1], 1u);
-        large[li] = {b, first, nch, 0u};
-        return;
-    }
-    Xyzz<T> acc = xyzz_inf<T>();
-    for (uint32_t e = st; e < en; e++) {
-        uint32_t ref = entries[e];
-        Aff<T> P = PointIO<T>::load_aff(points, ref & 0x7fffffffu);
-        if (aff_is_inf(P)) continue;
-        if (ref >> 31) P.y = PointIO<T>::neg_y(P.y);
-        xyzz_madd(acc, P);
-    }
-    PointIO<T>::store_xyzz(buckets, b, acc);
-}
-
-// G1 accumulation in the 29-bit-limb form (field29.hpp): same work split and
-// oversized-bucket hand-off as k_msm_accum; points come from the converted
-// copy of the bases, and each bucket is converted back to the standard form
-// when it is stored.
+// Bucket accumulation, G1. One thread per bucket, in the sort's accumulation
+// order (longest first, so a wave's lanes carry near-equal loads and the
+// launch has no long tail); buckets above lmax entries go to the chunked path.
+// Arithmetic in the 29-bit-limb form (field29.hpp): points come from the
+// converted copy of the bases, and each bucket is converted back to the
+// standard form when it is stored.
 __global__ void __launch_bounds__(256) k_msm_accum29(const uint32_t* __restrict__ entries,
                                                      const uint32_t* __restrict__ offsets,
                                                      const uint32_t* __restrict__ order, uint32_t nb, uint32_t lmax,
@@ -621,20 +587,15 @@ static void launch_t(const MsmSort& sort, const void* d_points, const void* d_po
     const uint32_t nb = (uint32_t)((size_t)g.W * g.K);
     fill_u32(scr.counters.p, 0u, 16, st);
     if (scr.ev_acc0) ZK_HIP(hipEventRecord(scr.ev_acc0, st));
-    if (g2 && d_points29)
+    if (g2)
         k_msm_accum29q<<<cdiv(nb, ACC29Q_THREADS), ACC29Q_THREADS, 0, st>>>(
             sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(), sort.order.as<uint32_t>(), nb, sort.lmax,
             d_points29, scr.buckets.p, scr.counters.as<uint32_t>(), scr.chunks.as<ChunkRec>(), scr.large.as<LargeRec>());
-    else if (!g2 && d_points29)
+    else
         k_msm_accum29<<<cdiv(nb, 128), 128, 0, st>>>(sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(),
                                                      sort.order.as<uint32_t>(), nb, sort.lmax, d_points29, scr.buckets.p,
                                                      scr.counters.as<uint32_t>(), scr.chunks.as<ChunkRec>(),
                                                      scr.large.as<LargeRec>());
-    else
-        k_msm_accum<T><<<cdiv(nb, 128), 128, 0, st>>>(sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(),
-                                                      sort.order.as<uint32_t>(), nb, sort.lmax, d_points,
-                                                      scr.buckets.p, scr.counters.as<uint32_t>(),
-                                                      scr.chunks.as<ChunkRec>(), scr.large.as<LargeRec>());
     ZK_HIP(hipGetLastError());
     if (scr.ev_acc1) ZK_HIP(hipEventRecord(scr.ev_acc1, st));
     k_msm_chunks<T><<<512, 256, 0, st>>>(sort.entries.as<uint32_t>(), d_points, scr.counters.as<uint32_t>(),
@@ -662,6 +623,7 @@ static void launch_t(const MsmSort& sort, const void* d_points, const void* d_po
 
 void msm_launch(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, MsmPartials& out, hipStream_t st,
                 const void* d_points29) {
+    ZK_REQUIRE(sort.n == 0 || d_points29, ZK_E_INTERNAL, "MSM launch without the 29-bit copy of its points");
     if (g2) launch_t<Fq2>(sort, d_points, d_points29, scr, out, st);
     else launch_t<Fp>(sort, d_points, d_points29, scr, out, st);
 }

@@ -7,9 +7,10 @@
 //     groups + device scan + LDS-ranked scatter, then per-group LDS counting
 //     sort); no global atomics, no CUB, no host synchronisation;
 //   * one sort can serve several MSMs that share a scalar vector;
-//   * buckets in XYZZ coordinates, one thread per bucket; oversized buckets
-//     are split into fixed chunks reduced by whole workgroups (device-side
-//     work lists, no host round trip);
+//   * buckets in XYZZ coordinates, one thread per bucket, accumulated in a
+//     9 x 29-bit-limb Montgomery form (field29.hpp) over a converted copy of
+//     the bases; oversized buckets are split into fixed chunks reduced by
+//     whole workgroups (device-side work lists, no host round trip);
 //   * bucket reduction: thread-level running sums over segments, then an LDS
 //     suffix-scan combine per workgroup; the host finishes a few points per
 //     window and runs Horner across windows.
@@ -71,10 +72,11 @@ struct MsmScratch {
 // Launch bucket accumulation + GPU reduction for points (affine, Montgomery,
 // AoS: G1 64 B / G2 128 B per point) against a finished sort; queue the D2H of
 // the partials into out.host on st. The caller synchronises st before finish.
-// d_points29 (optional): the same points in the 29-bit-limb form of
-// msm_points_to29; when given, buckets are accumulated in that form.
+// d_points29: the same points in the 29-bit-limb form of msm_points_to29, in
+// which the buckets are accumulated (d_points serves the oversized-bucket
+// path); may be null only for n = 0.
 void msm_launch(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, MsmPartials& out,
-                hipStream_t st, const void* d_points29 = nullptr);
+                hipStream_t st, const void* d_points29);
 
 // affine points (standard form, G1 64 B / G2 128 B each) -> 29-bit-limb form
 // (G1: 80 B of limbs in a 128 B slot; G2: 144 B in a 256 B slot)
