@@ -143,7 +143,8 @@ int zk_set_batch_slots(zk_ctx* ctx, int n);
  *  [15] digit entries of the witness sort  [16] of the h sort
  *  [17] window c (witness) [18] c (h) [19] windows W (witness) [20] W (h)
  *  [21] stream-0 span ms  [22] witness-MSM span ms (shared sort start to
- *  the last of the four MSMs)  [23] host tail ms */
+ *  the last of the four MSMs)  [23] host tail ms (host work left once the
+ *  last MSM's partials reached the host) */
 int zk_set_profiling(zk_ctx* ctx, int on);
 int zk_prove_stats(zk_ctx* ctx, double* out, int n);
 
@@ -244,6 +245,14 @@ int zk_synth_blinding(uint64_t seed, uint64_t r[4], uint64_t s[4]);
 int zk_synth_uniform(uint64_t seed, size_t count, uint64_t* out);
 /* n_vars / domain of the synthetic circuit (m, n_pub) */
 int zk_synth_sizes(uint32_t m, uint32_t n_pub, uint32_t* n_vars, uint32_t* domain);
+
+/* ---- host-side check of the MSM's scalar split (no GPU) ---- */
+/* No counterpart in the reference, whose MSMs run over full 254-bit scalars
+ * (include/msm_g1.cuh:271-329). The GLV split the MSM kernels apply to every
+ * scalar, run on the host from the same source: canonical k (4 words each) ->
+ * out[4i..4i+1] = k1, out[4i+2..4i+3] = k2, each a magnitude < 2^127 with the
+ * sign in bit 127, k = k1 + k2*lambda (mod r). For tests and tools. */
+int zk_glv_split(const uint64_t* k, size_t n, uint64_t* out);
 
 #ifdef __cplusplus
 }

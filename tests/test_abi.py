@@ -195,3 +195,28 @@ def test_groth16_header_compiles(tmp_path):
     r = subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(out),
                         "-L", os.path.join(ROOT, "zyklop_amd"), "-lzkp"], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_glv_split_identity_and_bounds():
+    """The MSM's scalar split, host build of the device source: for edge and
+    random scalars, k1 + k2*lambda = k (mod r) with |k1|, |k2| < 2^127, where
+    lambda is a cube root of unity mod r (a root of x^2 + x + 1)."""
+    R = pyref.R
+    lam = 0xB3C4D79D41A917585BFC41088D8DAAA78B17EA66B99C90DD
+    assert (lam * lam + lam + 1) % R == 0
+    rng = np.random.default_rng(9)
+    ks = [0, 1, 2, R - 1, R - 2, lam, lam * lam % R, R - lam, (R - 1) // 2, (R + 1) // 2,
+          2**127, 2**127 - 1, 2**128, R - 2**127, 2**253]
+    ks += [int.from_bytes(rng.bytes(32), "little") % R for _ in range(3000)]
+    out = zk.glv_split(O.arr(ks))
+    for k, o in zip(ks, out):
+        v1 = int(o[0]) | int(o[1]) << 64
+        v2 = int(o[2]) | int(o[3]) << 64
+        k1 = -(v1 & (2**127 - 1)) if v1 >> 127 else v1
+        k2 = -(v2 & (2**127 - 1)) if v2 >> 127 else v2
+        assert (k1 + k2 * lam - k) % R == 0, k
+        assert abs(k1) < 2**127 and abs(k2) < 2**127, k
+        assert not (v1 >> 127 and k1 == 0) and not (v2 >> 127 and k2 == 0), k   # no negative zero
+    with pytest.raises(zk.ZkError) as e:
+        zk.glv_split(O.arr([R]))
+    assert e.value.code == 1

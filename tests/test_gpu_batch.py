@@ -123,3 +123,32 @@ def test_autotune_table(tmp_path):
     c.free()
     ctx2.close()
     ctx.close()
+
+
+def test_bucket_splits_leave_proofs_unchanged(tmp_path):
+    """Every accumulation split (threads per bucket) through a tuning table:
+    the proof is the same bit for bit, and one of them is checked against the
+    oracle and verified."""
+    ctx = zk.Context(0)
+    c = ctx.synth_circuit(85, 1500, 8)
+    w = c.witness(11)
+    r, s = zk.synth_blinding(12)
+    n_w, n_h = c.n_vars + 4, c.n
+    base = ctx.prove(c, w, r, s).words()
+    ctx.autotune(c)
+    path = tmp_path / "t.txt"
+    ctx.tune_save(path)
+    lines = [ln.split() for ln in path.read_text().splitlines() if not ln.startswith("#")]
+    assert len(lines) == 2 and all(len(ln) == 6 for ln in lines)
+    for sg1, sg2 in [(1, 1), (2, 3), (3, 2), (4, 4)]:
+        txt = "".join(f"{a} {k} {cls} {cw} {sg1} {sg2}\n" for a, k, cls, cw, _, _ in lines)
+        path.write_text(txt)
+        ctx2 = zk.Context(0)
+        assert ctx2.tune_load(path) == 2
+        assert np.array_equal(ctx2.prove(c, w, r, s).words(), base), (sg1, sg2)
+        ctx2.close()
+    ref = O.Circuit(85, 1500, 8, with_crs=False)
+    assert np.array_equal(base, ref.prove_trapdoor(r, s, w=w))
+    assert ref.verify(base, r, s, w=w)
+    c.free()
+    ctx.close()

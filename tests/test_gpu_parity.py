@@ -182,6 +182,41 @@ def test_msm_g2_equal_points_in_bucket(ctx):
         assert inf == einf and np.array_equal(out, exp), (ks, ss)
 
 
+LAMBDA = 0xB3C4D79D41A917585BFC41088D8DAAA78B17EA66B99C90DD   # cube root of unity mod r (GLV)
+
+
+def glv_edge_case():
+    """Bases P_i = k_i*G where lambda*P_0 = P_1 and lambda*P_1 = P_2 (the
+    endomorphism image of one base is another base, so the split puts equal
+    points in one bucket), and scalars at the edges of the split: 0, 1,
+    lambda and its powers, r - 1, (r +- 1)/2, values around 2^127 and r - 2^127."""
+    R = pyref.R
+    a, b = 987654321123, 55555
+    ks = [a, LAMBDA * a % R, LAMBDA * LAMBDA * a % R, b, R - b, 7, 8, 9, 10, 11, 12, 13, 14, 15]
+    ss = [5, 5, 5, 0, 1, LAMBDA, LAMBDA * LAMBDA % R, R - LAMBDA, R - 1, (R - 1) // 2, (R + 1) // 2,
+          2**127, 2**127 - 1, R - 2**127]
+    return ks, ss
+
+
+def test_msm_g1_glv_edges(ctx):
+    ks, ss = glv_edge_case()
+    pts = mont_pts_g1(ks)
+    sc = O.arr(ss)
+    for window in (0, 4, 13):
+        out, inf = ctx.msm_g1(pts, sc, window)
+        exp, einf = oracle_msm_g1(pts, sc)
+        assert inf == einf and np.array_equal(out, exp), window
+
+
+def test_msm_g2_glv_edges(ctx):
+    ks, ss = glv_edge_case()
+    pts = mont_pts_g2(ks)
+    sc = O.arr(ss)
+    out, inf = ctx.msm_g2(pts, sc)
+    exp, einf = oracle_msm_g2(pts, sc)
+    assert inf == einf and np.array_equal(out, exp)
+
+
 # ------------------------------------------------------------------ NTT
 def test_ntt_golden(ctx, ref_vectors):
     for v in ref_vectors("test_vectors_ntt.json"):

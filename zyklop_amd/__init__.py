@@ -118,6 +118,7 @@ ABI = {
     "zk_circuit_witness": (ctypes.c_int, [_vp, ctypes.c_uint64, _u64p]),
     "zk_synth_blinding": (ctypes.c_int, [ctypes.c_uint64, _u64p, _u64p]),
     "zk_synth_uniform": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_size_t, _u64p]),
+    "zk_glv_split": (ctypes.c_int, [_u64p, ctypes.c_size_t, _u64p]),
     "zk_synth_sizes": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p]),
 }
 
@@ -579,4 +580,14 @@ def synth_blinding(seed):
 def synth_uniform(seed, count):
     out = np.zeros((count, 4), np.uint64)
     _check(lib().zk_synth_uniform(seed, count, u64p(out)))
+    return out
+
+
+def glv_split(scalars):
+    """Host run of the MSM's GLV split: (n,4) canonical scalars -> (n,4) words,
+    [k1 lo, k1 hi, k2 lo, k2 hi], each half a magnitude < 2^127 with its sign
+    in bit 127."""
+    k = np.ascontiguousarray(scalars, dtype=np.uint64).reshape(-1, 4)
+    out = np.zeros_like(k)
+    _check(lib().zk_glv_split(u64p(k), k.shape[0], u64p(out)))
     return out

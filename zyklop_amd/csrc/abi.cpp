@@ -5,6 +5,7 @@
 #include <string>
 
 #include "../../include/zk_abi.h"
+#include "glv.hpp"
 #include "msm.hpp"
 #include "ntt.hpp"
 #include "prover.hpp"
@@ -135,7 +136,7 @@ void msm_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t
     ZK_HIP(hipEventRecord(ctx->ev0, st));
     sort.run(d_scalars, n, window, st);
     ZK_HIP(hipEventRecord(ctx->ev1, st));
-    zk::msm_launch(sort, d_points, g2, scr, part, st, p29.p);
+    zk::msm_launch(sort, d_points, g2, scr, part, st, p29.p, zk::msm_default_split(n, sort.g.c, g2));
     ZK_HIP(hipEventRecord(ctx->ev2, st));
     ZK_HIP(hipStreamSynchronize(st));
     ZK_HIP(hipEventElapsedTime(&ctx->last_ms[0], ctx->ev0, ctx->ev2));
@@ -794,6 +795,16 @@ int zk_synth_uniform(uint64_t seed, size_t count, uint64_t* out) {
     return guard([&] {
         need(out != nullptr, "NULL argument");
         zk::synth_uniform(seed, count, out);
+    });
+}
+
+int zk_glv_split(const uint64_t* k, size_t n, uint64_t* out) {
+    return guard([&] {
+        need(n == 0 || (k && out), "NULL argument");
+        for (size_t i = 0; i < n; i++) {
+            need(zkh::Fr::is_canonical(k + 4 * i), "scalar not canonical (>= r)");
+            zk::glv::split(k + 4 * i, out + 4 * i, out + 4 * i + 2);
+        }
     });
 }
 

@@ -277,22 +277,26 @@ ZK_DEV void f29_madd(Xyzz29& A, const F29& x2, const F29& y2) {
     A.ZZZ = f29_mul(A.ZZZ, PPP);
 }
 
-// G1 affine point in this form: x[9], y[9] in the first 80 bytes of a
-// 128-byte slot, so that the random gathers of the bucket loop touch one cache
-// line per point
-constexpr int PT29_WORDS = 20;
+// One 128-byte slot per G1 base P = (x, y), so that the random gathers of the
+// bucket loop touch one cache line per point: the coordinates of P in this
+// form, each packed into 8 words (canonical, < p), and the x of the
+// endomorphism image lambda*P = (beta*x, y) of the GLV split:
+//   words 0..7 y, 8..15 x, 16..23 beta*x, 24..31 unused.
+// Base index i < n reads (x, y) of slot i; n <= i < 2n reads (beta*x, y) of
+// slot i - n.
 constexpr int PT29_SLOT_UINT4 = 8;
 
-ZK_DEV void f29_load_point(const void* base, size_t idx, F29& x, F29& y) {
-    const uint4* p = reinterpret_cast<const uint4*>(base) + PT29_SLOT_UINT4 * idx;
-    const uint4 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
-    const uint32_t w[20] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y,
-                            c.z, c.w, d.x, d.y, d.z, d.w, e.x, e.y, e.z, e.w};
-#pragma unroll
-    for (int i = 0; i < 9; i++) {
-        x.l[i] = w[i];
-        y.l[i] = w[9 + i];
-    }
+ZK_DEV F29 f29_load_packed(const uint4* q) {
+    const uint4 a = q[0], b = q[1];
+    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    return f29_unpack(w);
+}
+
+ZK_DEV void f29_load_point(const void* base, uint32_t idx, uint32_t n, F29& x, F29& y) {
+    const bool phi = idx >= n;
+    const uint4* p = reinterpret_cast<const uint4*>(base) + (size_t)PT29_SLOT_UINT4 * (phi ? idx - n : idx);
+    y = f29_load_packed(p);
+    x = f29_load_packed(p + (phi ? 4 : 2));
 }
 
 // ---------------------------------------------------------------- Fq2 (G2)
@@ -337,29 +341,19 @@ ZK_DEV F29q f29q_sub_nr(const F29q& a, const F29q& b) {
 ZK_DEV F29q f29q_from_std(const Fq2& x) { return {f29_from_std(x.c0), f29_from_std(x.c1)}; }
 ZK_DEV Fq2 f29q_to_std(const F29q& v) { return {f29_to_std(v.c0), f29_to_std(v.c1)}; }
 
-// G2 affine point in this form: x.c0, x.c1, y.c0, y.c1 (9 limbs each) in the
-// first 144 bytes of a 256-byte slot (two cache lines per gather)
-constexpr int PT29Q_WORDS = 36;
+// One 256-byte slot per G2 base (two cache lines per gather), coordinates
+// packed as for G1: words 0..15 y (c0, c1), 16..31 x, 32..47 beta*x,
+// 48..63 unused; base index i >= n reads (beta*x, y) of slot i - n.
 constexpr int PT29Q_SLOT_UINT4 = 16;
 
-ZK_DEV void f29q_load_point(const void* base, size_t idx, F29q& x, F29q& y) {
-    const uint4* p = reinterpret_cast<const uint4*>(base) + PT29Q_SLOT_UINT4 * idx;
-    uint32_t w[36];
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-        const uint4 v = p[k];
-        w[4 * k] = v.x;
-        w[4 * k + 1] = v.y;
-        w[4 * k + 2] = v.z;
-        w[4 * k + 3] = v.w;
-    }
-#pragma unroll
-    for (int i = 0; i < 9; i++) {
-        x.c0.l[i] = w[i];
-        x.c1.l[i] = w[9 + i];
-        y.c0.l[i] = w[18 + i];
-        y.c1.l[i] = w[27 + i];
-    }
+ZK_DEV void f29q_load_point(const void* base, uint32_t idx, uint32_t n, F29q& x, F29q& y) {
+    const bool phi = idx >= n;
+    const uint4* p = reinterpret_cast<const uint4*>(base) + (size_t)PT29Q_SLOT_UINT4 * (phi ? idx - n : idx);
+    y.c0 = f29_load_packed(p);
+    y.c1 = f29_load_packed(p + 2);
+    const uint4* px = p + (phi ? 8 : 4);
+    x.c0 = f29_load_packed(px);
+    x.c1 = f29_load_packed(px + 2);
 }
 
 // A G2 XYZZ accumulator kept in LDS: 4 coordinates x 18 limbs, laid out

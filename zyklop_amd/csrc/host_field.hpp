@@ -30,6 +30,7 @@ struct FrP {
 
 template <class P>
 struct F {
+    static_assert(P::M[3] < (1ULL << 62), "the no-carry product needs a modulus below 2^254");
     uint64_t v[4];
 
     static F zero() { F r; memset(r.v, 0, 32); return r; }
@@ -72,21 +73,30 @@ struct F {
         return r;
     }
     F operator-() const { return zero() - *this; }
-    // Montgomery CIOS product a*b*2^-256
+    // Montgomery CIOS product a*b*2^-256. Both BN254 moduli have a top word
+    // below 2^62, so the running value stays below 2m after every outer step
+    // and fits four words: no fifth carry word is kept ("no-carry" CIOS).
     friend F operator*(const F& a, const F& b) {
-        uint64_t t[6] = {0, 0, 0, 0, 0, 0};
+        uint64_t t0 = 0, t1 = 0, t2 = 0, t3 = 0;
         for (int i = 0; i < 4; i++) {
-            u128 c = 0;
-            for (int j = 0; j < 4; j++) { c += (u128)a.v[j] * b.v[i] + t[j]; t[j] = (uint64_t)c; c >>= 64; }
-            c += t[4]; t[4] = (uint64_t)c; t[5] = (uint64_t)(c >> 64);
-            uint64_t m = t[0] * P::NP;
-            c = (u128)m * P::M[0] + t[0]; c >>= 64;
-            for (int j = 1; j < 4; j++) { c += (u128)m * P::M[j] + t[j]; t[j - 1] = (uint64_t)c; c >>= 64; }
-            c += t[4]; t[3] = (uint64_t)c; t[4] = t[5] + (uint64_t)(c >> 64);
+            const uint64_t bi = b.v[i];
+            u128 x = (u128)a.v[0] * bi + t0;
+            uint64_t A = (uint64_t)(x >> 64);
+            t0 = (uint64_t)x;
+            const uint64_t m = t0 * P::NP;
+            u128 y = (u128)m * P::M[0] + t0;
+            uint64_t C = (uint64_t)(y >> 64);
+            x = (u128)a.v[1] * bi + t1 + A; A = (uint64_t)(x >> 64); t1 = (uint64_t)x;
+            y = (u128)m * P::M[1] + t1 + C; C = (uint64_t)(y >> 64); t0 = (uint64_t)y;
+            x = (u128)a.v[2] * bi + t2 + A; A = (uint64_t)(x >> 64); t2 = (uint64_t)x;
+            y = (u128)m * P::M[2] + t2 + C; C = (uint64_t)(y >> 64); t1 = (uint64_t)y;
+            x = (u128)a.v[3] * bi + t3 + A; A = (uint64_t)(x >> 64); t3 = (uint64_t)x;
+            y = (u128)m * P::M[3] + t3 + C; C = (uint64_t)(y >> 64); t2 = (uint64_t)y;
+            t3 = C + A;
         }
         F r;
-        memcpy(r.v, t, 32);
-        if (t[4] || geq_mod(r.v)) {
+        r.v[0] = t0; r.v[1] = t1; r.v[2] = t2; r.v[3] = t3;
+        if (geq_mod(r.v)) {
             u128 br = 0;
             for (int i = 0; i < 4; i++) {
                 u128 d = (u128)r.v[i] - P::M[i] - br;

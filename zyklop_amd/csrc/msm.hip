@@ -1,9 +1,11 @@
 // Pippenger MSM kernels for G1/G2 on gfx950 (see msm.hpp for the design).
 #include <algorithm>
+#include <future>
 #include <type_traits>
 
 #include "curve.hpp"
 #include "field29.hpp"
+#include "glv.hpp"
 #include "msm.hpp"
 #include "qap.hpp"
 #include "scan.hpp"
@@ -20,7 +22,7 @@ template <> struct RedBlk<Fp> { static constexpr int V = 256; };
 template <> struct RedBlk<Fq2> { static constexpr int V = 128; };
 
 struct ChunkRec { uint32_t bucket, start, len, pad; };
-struct LargeRec { uint32_t bucket, first, count, pad; };
+struct LargeRec { uint32_t slot, first, count, pad; };
 
 int msm_default_c(size_t n) {
     int lg = ilog2(n ? n : 1);
@@ -30,12 +32,20 @@ int msm_default_c(size_t n) {
     return c;
 }
 
+int msm_default_split(size_t n, int c, bool g2) {
+    const MsmGeom g = msm_geom(n, c, g2);
+    // resident waves per SIMD: G1 accumulation 6 (80 VGPRs), G2 2 (LDS)
+    const size_t target = (size_t)1024 * 64 * 2 * (g2 ? 2 : 6);
+    const size_t nb = (size_t)g.W * g.K;
+    return (int)std::max<size_t>(1, std::min<size_t>(3, target / nb));
+}
+
 MsmGeom msm_geom(size_t n, int c, bool g2) {
     if (c <= 0) c = msm_default_c(n);
     ZK_REQUIRE(c >= 4 && c <= 22, ZK_E_ARG, "MSM window must be in [4, 22]");
     MsmGeom g;
     g.c = c;
-    g.W = (255 + c - 1) / c;
+    g.W = (GLV_BITS + c - 1) / c;
     g.K = 1u << (c - 1);
     g.l1 = std::min<uint32_t>(L1_MAX, g.K);
     g.nseg = g.K / g.l1;
@@ -44,17 +54,37 @@ MsmGeom msm_geom(size_t n, int c, bool g2) {
     return g;
 }
 
+// ---------------------------------------------------------------- GLV split
+// Every scalar k_i becomes two half scalars (glv.hpp): half[i] = k1 for P_i and
+// half[n + i] = k2 for lambda*P_i, each a 127-bit magnitude with its sign in
+// bit 127 (4 x u32 LE).
+__global__ void k_glv_split(const uint32_t* __restrict__ sc, size_t n, uint4* __restrict__ half) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint4* p = reinterpret_cast<const uint4*>(sc) + 2 * i;
+    const uint4 a = p[0], b = p[1];
+    const uint64_t k[4] = {a.x | (uint64_t)a.y << 32, a.z | (uint64_t)a.w << 32, b.x | (uint64_t)b.y << 32,
+                           b.z | (uint64_t)b.w << 32};
+    uint64_t k1[2], k2[2];
+    glv::split(k, k1, k2);
+    half[i] = make_uint4((uint32_t)k1[0], (uint32_t)(k1[0] >> 32), (uint32_t)k1[1], (uint32_t)(k1[1] >> 32));
+    half[n + i] = make_uint4((uint32_t)k2[0], (uint32_t)(k2[0] >> 32), (uint32_t)k2[1], (uint32_t)(k2[1] >> 32));
+}
+
 // ---------------------------------------------------------------- digits
-// Signed c-bit digits of a canonical scalar, least significant window first:
+// Signed c-bit digits of a half scalar, least significant window first:
 // v = window bits + carry; v > 2^(c-1) becomes v - 2^c with a carry into the
-// next window (include/msm_g1.cuh:271-329 uses the same recoding).
+// next window (include/msm_g1.cuh:271-329 uses the same recoding). The
+// magnitude is below 2^127, so W*c >= 128 leaves no carry out of the top
+// window. A negative half scalar flips the sign of every digit.
 struct DigitIter {
-    uint32_t s[8];
+    uint32_t s[4];
     uint32_t carry, mask, half, c;
-    ZK_DEV void init(const uint32_t* sc, size_t i, int cc) {
-        const uint4* p = reinterpret_cast<const uint4*>(sc) + 2 * i;
-        uint4 a = p[0], b = p[1];
-        s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w; s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
+    bool flip;
+    ZK_DEV void init(const uint32_t* hs, size_t i, int cc) {
+        const uint4 a = reinterpret_cast<const uint4*>(hs)[i];
+        s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w & 0x7fffffffu;
+        flip = (a.w >> 31) != 0;
         carry = 0;
         c = (uint32_t)cc;
         mask = (1u << cc) - 1;
@@ -64,15 +94,15 @@ struct DigitIter {
     ZK_DEV uint32_t next(bool& neg) {
         uint32_t v = (s[0] & mask) + carry;
 #pragma unroll
-        for (int k = 0; k < 7; k++) s[k] = __builtin_amdgcn_alignbit(s[k + 1], s[k], c);
-        s[7] >>= c;
+        for (int k = 0; k < 3; k++) s[k] = __builtin_amdgcn_alignbit(s[k + 1], s[k], c);
+        s[3] >>= c;
         if (v > half) {
             carry = 1;
-            neg = true;
+            neg = !flip;
             return (mask + 1) - v;
         }
         carry = 0;
-        neg = false;
+        neg = flip;
         return v;
     }
 };
@@ -208,9 +238,17 @@ __global__ void __launch_bounds__(1024) k_load_scatter(const uint32_t* __restric
 }
 
 void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
-    ZK_REQUIRE(n_ < 0x80000000ull, ZK_E_ARG, "MSM size must be < 2^31");
+    ZK_REQUIRE(n_ < 0x40000000ull, ZK_E_ARG, "MSM size must be < 2^30");
     n = n_;
     g = msm_geom(n, c, false);
+    // the sort runs over the 2n half scalars of the GLV split
+    const size_t n2 = 2 * n;
+    half.ensure(std::max<size_t>(1, n2) * 16);
+    if (n) {
+        k_glv_split<<<cdiv(n, 256), 256, 0, st>>>((const uint32_t*)d_scalars, n, half.as<uint4>());
+        ZK_HIP(hipGetLastError());
+    }
+    const uint32_t* hs = half.as<uint32_t>();
     size_t nb = (size_t)g.W * g.K;
     // group size 2^s: at most 4096 groups (LDS histograms of pass A)
     int sh = 7;
@@ -218,10 +256,10 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
     const uint32_t G = (uint32_t)((nb + (1ull << sh) - 1) >> sh);
     // tiles of pass A: multiples of 4096 scalars (long runs per group for the
     // pair writes), at most 2048 tiles
-    const size_t tile = (size_t)4096 * std::max<size_t>(1, (n + 4096 * 2048 - 1) / (4096 * 2048));
-    const uint32_t nt = (uint32_t)std::max<size_t>(1, (n + tile - 1) / tile);
+    const size_t tile = (size_t)4096 * std::max<size_t>(1, (n2 + 4096 * 2048 - 1) / (4096 * 2048));
+    const uint32_t nt = (uint32_t)std::max<size_t>(1, (n2 + tile - 1) / tile);
     const size_t mlen = (size_t)G * nt + 1;
-    const size_t ne = std::max<size_t>(1, (size_t)g.W * n);
+    const size_t ne = std::max<size_t>(1, (size_t)g.W * n2);
     matrix.ensure(mlen * 4);
     mscan.ensure(mlen * 4 + (size_t)4 * (cdiv(mlen, SCAN_TILE) + 1));
     pairs.ensure(ne * 8);
@@ -230,7 +268,7 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
     uint32_t* mat = matrix.as<uint32_t>();
     uint32_t* msc = mscan.as<uint32_t>();
     if (n) {
-        k_sort_tile_hist<<<nt, 1024, G * 4, st>>>((const uint32_t*)d_scalars, n, tile, g.c, g.W, g.K, sh, G, nt, mat);
+        k_sort_tile_hist<<<nt, 1024, G * 4, st>>>(hs, n2, tile, g.c, g.W, g.K, sh, G, nt, mat);
         ZK_HIP(hipGetLastError());
         fill_u32(mat + mlen - 1, 0u, 1, st);
     } else {
@@ -238,8 +276,8 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
     }
     exclusive_scan(mat, msc, msc + mlen, mlen, st);
     if (n) {
-        k_sort_tile_scatter<<<nt, 1024, 2 * G * 4, st>>>((const uint32_t*)d_scalars, n, tile, g.c, g.W, g.K, sh, G, nt,
-                                                          msc, pairs.as<uint2>());
+        k_sort_tile_scatter<<<nt, 1024, 2 * G * 4, st>>>(hs, n2, tile, g.c, g.W, g.K, sh, G, nt, msc,
+                                                          pairs.as<uint2>());
         ZK_HIP(hipGetLastError());
     }
     k_sort_group<<<G, 256, (4u << sh), st>>>(pairs.as<uint2>(), msc, nt, sh, (uint32_t)nb, offsets.as<uint32_t>(),
@@ -247,8 +285,8 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
     ZK_HIP(hipGetLastError());
     const uint32_t* off = offsets.as<uint32_t>();
     // accumulation order. Oversized threshold: 8x the mean bucket load
-    // (uniform scalars put about n/K entries in a bucket).
-    lmax = std::max<uint32_t>(L_MIN_LARGE, 8u * (uint32_t)((n + g.K - 1) / g.K));
+    // (uniform scalars put about 2n/K entries in a bucket).
+    lmax = std::max<uint32_t>(L_MIN_LARGE, 8u * (uint32_t)((n2 + g.K - 1) / g.K));
     qs = 0;
     while ((lmax >> qs) + 2 > 4096) qs++;
     nkey = (lmax >> qs) + 2;
@@ -272,41 +310,69 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
 // Arithmetic in the 29-bit-limb form (field29.hpp): points come from the
 // converted copy of the bases, and each bucket is converted back to the
 // standard form when it is stored.
+// Thread tid of an accumulation launch over nb buckets, each split into
+// `split` parts of consecutive entries (bucket b's parts go to slots
+// b*split + h, summed by the level-1 reduction). Returns false when the thread
+// has nothing to accumulate: past the end, or a bucket above lmax entries,
+// which goes to the chunked path (part 0 lists its chunks, whose merged sum
+// lands in slot b*split; the other parts store infinity). Otherwise [st, en)
+// are its entries and slot its bucket slot.
+template <class T>
+ZK_DEV bool acc_range(uint32_t tid, uint32_t nb, uint32_t split, const uint32_t* __restrict__ order,
+                      const uint32_t* __restrict__ offsets, uint32_t lmax, void* __restrict__ buckets,
+                      uint32_t* __restrict__ counters, ChunkRec* __restrict__ chunks, LargeRec* __restrict__ large,
+                      uint32_t& st, uint32_t& en, uint32_t& slot) {
+    if (tid >= nb * split) return false;
+    const uint32_t ob = tid / split, h = tid - ob * split;
+    const uint32_t b = order[ob];
+    const uint32_t s0 = offsets[b], e0 = offsets[b + 1];
+    const uint32_t cnt = e0 - s0;
+    slot = b * split + h;
+    if (cnt > lmax) {
+        if (h == 0) {
+            const uint32_t nch = (cnt + CHUNK - 1) / CHUNK;
+            const uint32_t first = atomicAdd(&counters[0], nch);
+            for (uint32_t j = 0; j < nch; j++) {
+                const uint32_t c0 = s0 + j * CHUNK;
+                chunks[first + j] = {b, c0, min(CHUNK, e0 - c0), 0u};
+            }
+            const uint32_t li = atomicAdd(&counters[1], 1u);
+            large[li] = {slot, first, nch, 0u};
+        } else {
+            PointIO<T>::store_xyzz(buckets, slot, xyzz_inf<T>());
+        }
+        return false;
+    }
+    st = s0 + (uint32_t)(((uint64_t)cnt * h) / split);
+    en = s0 + (uint32_t)(((uint64_t)cnt * (h + 1)) / split);
+    return true;
+}
+
 __global__ void __launch_bounds__(256) k_msm_accum29(const uint32_t* __restrict__ entries,
                                                      const uint32_t* __restrict__ offsets,
-                                                     const uint32_t* __restrict__ order, uint32_t nb, uint32_t lmax,
-                                                     const void* __restrict__ points29, void* __restrict__ buckets,
+                                                     const uint32_t* __restrict__ order, uint32_t nb, uint32_t split,
+                                                     uint32_t lmax,
+                                                     const void* __restrict__ points29, uint32_t nbase,
+                                                     void* __restrict__ buckets,
                                                      uint32_t* __restrict__ counters, ChunkRec* __restrict__ chunks,
                                                      LargeRec* __restrict__ large) {
-    uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= nb) return;
-    uint32_t b = order[tid];
-    uint32_t st = offsets[b], en = offsets[b + 1];
-    uint32_t cnt = en - st;
-    if (cnt > lmax) {
-        uint32_t nch = (cnt + CHUNK - 1) / CHUNK;
-        uint32_t first = atomicAdd(&counters[0], nch);
-        for (uint32_t j = 0; j < nch; j++) {
-            uint32_t s0 = st + j * CHUNK;
-            chunks[first + j] = {b, s0, min(CHUNK, en - s0), 0u};
-        }
-        uint32_t li = atomicAdd(&counters[1], 1u);
-        large[li] = {b, first, nch, 0u};
+    uint32_t st, en, slot;
+    if (!acc_range<Fp>(blockIdx.x * blockDim.x + threadIdx.x, nb, split, order, offsets, lmax, buckets, counters, chunks,
+                       large, st, en, slot))
         return;
-    }
     Xyzz29 acc;
     acc.inf = true;
     for (uint32_t e = st; e < en; e++) {
         uint32_t ref = entries[e];
         F29 x, y;
-        f29_load_point(points29, ref & 0x7fffffffu, x, y);
+        f29_load_point(points29, ref & 0x7fffffffu, nbase, x, y);
         if (f29_is_zero(x) && f29_is_zero(y)) continue;   // infinity
         if (ref >> 31) y = f29_neg(y);
         f29_madd(acc, x, y);
     }
     Xyzz<Fp> o = xyzz_inf<Fp>();
     if (!acc.inf) o = {f29_to_std(acc.X), f29_to_std(acc.Y), f29_to_std(acc.ZZ), f29_to_std(acc.ZZZ)};
-    PointIO<Fp>::store_xyzz(buckets, b, o);
+    PointIO<Fp>::store_xyzz(buckets, slot, o);
 }
 
 // G2 accumulation in the 29-bit-limb form: the Fp2 state does not fit the
@@ -316,34 +382,23 @@ constexpr int ACC29Q_THREADS = 128;
 __global__ void __launch_bounds__(ACC29Q_THREADS) k_msm_accum29q(const uint32_t* __restrict__ entries,
                                                                  const uint32_t* __restrict__ offsets,
                                                                  const uint32_t* __restrict__ order, uint32_t nb,
-                                                                 uint32_t lmax, const void* __restrict__ points29,
-                                                                 void* __restrict__ buckets,
+                                                                 uint32_t split, uint32_t lmax,
+                                                                 const void* __restrict__ points29,
+                                                                 uint32_t nbase, void* __restrict__ buckets,
                                                                  uint32_t* __restrict__ counters,
                                                                  ChunkRec* __restrict__ chunks,
                                                                  LargeRec* __restrict__ large) {
     __shared__ uint32_t lds[4 * 18 * ACC29Q_THREADS];
-    uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= nb) return;
-    uint32_t b = order[tid];
-    uint32_t st = offsets[b], en = offsets[b + 1];
-    uint32_t cnt = en - st;
-    if (cnt > lmax) {
-        uint32_t nch = (cnt + CHUNK - 1) / CHUNK;
-        uint32_t first = atomicAdd(&counters[0], nch);
-        for (uint32_t j = 0; j < nch; j++) {
-            uint32_t s0 = st + j * CHUNK;
-            chunks[first + j] = {b, s0, min(CHUNK, en - s0), 0u};
-        }
-        uint32_t li = atomicAdd(&counters[1], 1u);
-        large[li] = {b, first, nch, 0u};
+    uint32_t st, en, slot;
+    if (!acc_range<Fq2>(blockIdx.x * blockDim.x + threadIdx.x, nb, split, order, offsets, lmax, buckets, counters, chunks,
+                       large, st, en, slot))
         return;
-    }
     const AccLds<ACC29Q_THREADS> A{lds + threadIdx.x};
     bool inf = true;
     for (uint32_t e = st; e < en; e++) {
         uint32_t ref = entries[e];
         F29q x, y;
-        f29q_load_point(points29, ref & 0x7fffffffu, x, y);
+        f29q_load_point(points29, ref & 0x7fffffffu, nbase, x, y);
         if (f29_is_zero(x.c0) && f29_is_zero(x.c1) && f29_is_zero(y.c0) && f29_is_zero(y.c1)) continue;
         if (ref >> 31) y = {f29_neg(y.c0), f29_neg(y.c1)};
         f29q_madd(A, inf, x, y);
@@ -351,53 +406,85 @@ __global__ void __launch_bounds__(ACC29Q_THREADS) k_msm_accum29q(const uint32_t*
     using L = AccLds<ACC29Q_THREADS>;
     Xyzz<Fq2> o = xyzz_inf<Fq2>();
     if (!inf) o = {f29q_to_std(A.ld(L::X)), f29q_to_std(A.ld(L::Y)), f29q_to_std(A.ld(L::ZZ)), f29q_to_std(A.ld(L::ZZZ))};
-    PointIO<Fq2>::store_xyzz(buckets, b, o);
+    PointIO<Fq2>::store_xyzz(buckets, slot, o);
 }
 
-__global__ void k_points_to29q(const void* __restrict__ pts, size_t n, void* __restrict__ out) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    Aff<Fq2> a = PointIO<Fq2>::load_aff(pts, i);
-    uint32_t w[PT29Q_WORDS] = {};
-    if (!aff_is_inf(a)) {
-        const F29q x = f29q_from_std(a.x), y = f29q_from_std(a.y);
+// endomorphism constant (Montgomery, 8 x u32) as a field element
+ZK_DEV Fp beta_mont(const uint32_t* c) {
+    Fp b;
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-            w[k] = x.c0.l[k];
-            w[9 + k] = x.c1.l[k];
-            w[18 + k] = y.c0.l[k];
-            w[27 + k] = y.c1.l[k];
-        }
+    for (int k = 0; k < 8; k++) b.v[k] = c[k];
+    return b;
+}
+
+// base i of an MSM over the 2n points of the GLV split: P_i for i < n, else
+// lambda*P_(i-n) = (beta*x, y) (glv.hpp)
+template <class T>
+ZK_DEV Aff<T> load_base(const void* pts, uint32_t i, uint32_t n);
+template <>
+ZK_DEV Aff<Fp> load_base<Fp>(const void* pts, uint32_t i, uint32_t n) {
+    if (i < n) return PointIO<Fp>::load_aff(pts, i);
+    Aff<Fp> a = PointIO<Fp>::load_aff(pts, i - n);
+    if (aff_is_inf(a)) return a;
+    a.x = mul<FpCfg>(a.x, beta_mont(glv::BETA_G1_MONT));
+    return a;
+}
+template <>
+ZK_DEV Aff<Fq2> load_base<Fq2>(const void* pts, uint32_t i, uint32_t n) {
+    if (i < n) return PointIO<Fq2>::load_aff(pts, i);
+    Aff<Fq2> a = PointIO<Fq2>::load_aff(pts, i - n);
+    if (aff_is_inf(a)) return a;
+    const Fp b = beta_mont(glv::BETA_G2_MONT);
+    a.x.c0 = mul<FpCfg>(a.x.c0, b);
+    a.x.c1 = mul<FpCfg>(a.x.c1, b);
+    return a;
+}
+
+// packed words of the 29-bit form of x (std Montgomery), at w[0..7]
+ZK_DEV void pack29(const Fp& x, uint32_t* w) { f29_pack(f29_from_std(x), w); }
+
+// base slots of msm_points_to29 (layout in field29.hpp); infinity stays all zero
+__global__ void k_points_to29q(const void* __restrict__ pts, uint32_t n, void* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Aff<Fq2> a = PointIO<Fq2>::load_aff(pts, i);
+    uint32_t w[48] = {};
+    if (!aff_is_inf(a)) {
+        const Fp b = beta_mont(glv::BETA_G2_MONT);
+        pack29(a.y.c0, w);
+        pack29(a.y.c1, w + 8);
+        pack29(a.x.c0, w + 16);
+        pack29(a.x.c1, w + 24);
+        pack29(mul<FpCfg>(a.x.c0, b), w + 32);
+        pack29(mul<FpCfg>(a.x.c1, b), w + 40);
     }
     uint4* o = reinterpret_cast<uint4*>(out) + PT29Q_SLOT_UINT4 * i;
 #pragma unroll
     for (int k = 0; k < PT29Q_SLOT_UINT4; k++)
-        o[k] = k < 9 ? make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]) : make_uint4(0u, 0u, 0u, 0u);
+        o[k] = k < 12 ? make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]) : make_uint4(0u, 0u, 0u, 0u);
 }
 
-__global__ void k_points_to29(const void* __restrict__ pts, size_t n, void* __restrict__ out) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_points_to29(const void* __restrict__ pts, uint32_t n, void* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    Aff<Fp> a = PointIO<Fp>::load_aff(pts, i);
-    uint32_t w[PT29_WORDS] = {};
+    const Aff<Fp> a = PointIO<Fp>::load_aff(pts, i);
+    uint32_t w[24] = {};
     if (!aff_is_inf(a)) {
-        const F29 x = f29_from_std(a.x), y = f29_from_std(a.y);
-#pragma unroll
-        for (int k = 0; k < 9; k++) {
-            w[k] = x.l[k];
-            w[9 + k] = y.l[k];
-        }
+        pack29(a.y, w);
+        pack29(a.x, w + 8);
+        pack29(mul<FpCfg>(a.x, beta_mont(glv::BETA_G1_MONT)), w + 16);
     }
     uint4* o = reinterpret_cast<uint4*>(out) + PT29_SLOT_UINT4 * i;
 #pragma unroll
     for (int k = 0; k < PT29_SLOT_UINT4; k++)
-        o[k] = k < 5 ? make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]) : make_uint4(0u, 0u, 0u, 0u);
+        o[k] = k < 6 ? make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]) : make_uint4(0u, 0u, 0u, 0u);
 }
 
 void msm_points_to29(const void* d_points, size_t n, void* d_out, bool g2, hipStream_t st) {
     if (!n) return;
-    if (g2) k_points_to29q<<<cdiv(n, 256), 256, 0, st>>>(d_points, n, d_out);
-    else k_points_to29<<<cdiv(n, 256), 256, 0, st>>>(d_points, n, d_out);
+    ZK_REQUIRE(n < 0x40000000ull, ZK_E_ARG, "MSM size must be < 2^30");
+    if (g2) k_points_to29q<<<cdiv(n, 256), 256, 0, st>>>(d_points, (uint32_t)n, d_out);
+    else k_points_to29<<<cdiv(n, 256), 256, 0, st>>>(d_points, (uint32_t)n, d_out);
     ZK_HIP(hipGetLastError());
 }
 
@@ -419,7 +506,7 @@ __device__ Xyzz<T> block_sum(Xyzz<T> v, Xyzz<T>* sh) {
 // one workgroup per chunk of an oversized bucket (grid-stride over the chunk list)
 template <class T>
 __global__ void __launch_bounds__(256) k_msm_chunks(const uint32_t* __restrict__ entries, const void* __restrict__ points,
-                                                    const uint32_t* __restrict__ counters,
+                                                    uint32_t nbase, const uint32_t* __restrict__ counters,
                                                     const ChunkRec* __restrict__ chunks, void* __restrict__ partials) {
     __shared__ Xyzz<T> sh[256];
     const uint32_t nch = counters[0];
@@ -428,7 +515,7 @@ __global__ void __launch_bounds__(256) k_msm_chunks(const uint32_t* __restrict__
         Xyzz<T> acc = xyzz_inf<T>();
         for (uint32_t e = threadIdx.x; e < cr.len; e += blockDim.x) {
             uint32_t ref = entries[cr.start + e];
-            Aff<T> P = PointIO<T>::load_aff(points, ref & 0x7fffffffu);
+            Aff<T> P = load_base<T>(points, ref & 0x7fffffffu, nbase);
             if (aff_is_inf(P)) continue;
             if (ref >> 31) P.y = PointIO<T>::neg_y(P.y);
             xyzz_madd(acc, P);
@@ -450,7 +537,7 @@ __global__ void __launch_bounds__(256) k_msm_merge(const uint32_t* __restrict__ 
         for (uint32_t j = threadIdx.x; j < lr.count; j += blockDim.x)
             acc = xyzz_add(acc, PointIO<T>::load_xyzz(partials, lr.first + j));
         Xyzz<T> s = block_sum<T>(acc, sh);
-        if (threadIdx.x == 0) PointIO<T>::store_xyzz(buckets, lr.bucket, s);
+        if (threadIdx.x == 0) PointIO<T>::store_xyzz(buckets, lr.slot, s);
     }
 }
 
@@ -462,10 +549,11 @@ __global__ void __launch_bounds__(256) k_msm_merge(const uint32_t* __restrict__ 
 //
 // Level 1: thread j owns buckets [j*L1, (j+1)*L1) of one window; with bucket
 // b holding digit b+1: R_j = sum B_b, T_j = sum (b - j*L1 + 1) B_b, so the
-// window value is sum_j (T_j + j*L1*R_j).
+// window value is sum_j (T_j + j*L1*R_j). B_b is the sum of the bucket's
+// `split` slots from the accumulation.
 template <class T>
 __global__ void __launch_bounds__(128) k_msm_reduce1(const void* __restrict__ buckets, uint32_t K, int W, uint32_t L1,
-                                                     void* __restrict__ R, void* __restrict__ Tt) {
+                                                     uint32_t split, void* __restrict__ R, void* __restrict__ Tt) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t nseg = K / L1;
     if (j >= (uint32_t)W * nseg) return;
@@ -474,7 +562,8 @@ __global__ void __launch_bounds__(128) k_msm_reduce1(const void* __restrict__ bu
     Xyzz<T> run = xyzz_inf<T>(), acc = xyzz_inf<T>();
 #pragma unroll 1
     for (int b = (int)L1 - 1; b >= 0; b--) {
-        run = xyzz_add_i(run, PointIO<T>::load_xyzz(buckets, base + b));
+#pragma unroll 1
+        for (uint32_t h = 0; h < split; h++) run = xyzz_add_i(run, PointIO<T>::load_xyzz(buckets, (base + b) * split + h));
         acc = xyzz_add_i(acc, run);
     }
     PointIO<T>::store_xyzz(R, j, run);
@@ -562,10 +651,10 @@ MsmScratch::~MsmScratch() {
     if (ev_end) (void)hipEventDestroy(ev_end);
 }
 
-void MsmScratch::ensure(const MsmGeom& g, size_t n_entries, bool g2) {
+void MsmScratch::ensure(const MsmGeom& g, size_t n_entries, bool g2, int split) {
     size_t xb = g2 ? 256 : 128;
     size_t nb = (size_t)g.W * g.K;
-    buckets.ensure(nb * xb);
+    buckets.ensure(nb * split * xb);
     r1.ensure((size_t)g.W * g.nseg * xb);
     t1.ensure((size_t)g.W * g.nseg * xb);
     r2.ensure((size_t)g.W * g.nblk * xb);
@@ -580,32 +669,37 @@ void MsmScratch::ensure(const MsmGeom& g, size_t n_entries, bool g2) {
 
 template <class T>
 static void launch_t(const MsmSort& sort, const void* d_points, const void* d_points29, MsmScratch& scr,
-                     MsmPartials& out, hipStream_t st) {
+                     MsmPartials& out, hipStream_t st, int split) {
     const bool g2 = std::is_same<T, Fq2>::value;
     const MsmGeom g = msm_geom(sort.n, sort.g.c, g2);
-    scr.ensure(g, (size_t)g.W * sort.n, g2);
-    const uint32_t nb = (uint32_t)((size_t)g.W * g.K);
+    scr.ensure(g, (size_t)g.W * 2 * sort.n, g2, split);
+    const uint32_t nb = (uint32_t)((size_t)g.W * g.K), nt = nb * (uint32_t)split;
     fill_u32(scr.counters.p, 0u, 16, st);
     if (scr.ev_acc0) ZK_HIP(hipEventRecord(scr.ev_acc0, st));
     if (g2)
-        k_msm_accum29q<<<cdiv(nb, ACC29Q_THREADS), ACC29Q_THREADS, 0, st>>>(
-            sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(), sort.order.as<uint32_t>(), nb, sort.lmax,
-            d_points29, scr.buckets.p, scr.counters.as<uint32_t>(), scr.chunks.as<ChunkRec>(), scr.large.as<LargeRec>());
+        k_msm_accum29q<<<cdiv(nt, ACC29Q_THREADS), ACC29Q_THREADS, 0, st>>>(
+            sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(), sort.order.as<uint32_t>(), nb, (uint32_t)split,
+            sort.lmax,
+            d_points29, (uint32_t)sort.n, scr.buckets.p, scr.counters.as<uint32_t>(), scr.chunks.as<ChunkRec>(),
+            scr.large.as<LargeRec>());
     else
-        k_msm_accum29<<<cdiv(nb, 128), 128, 0, st>>>(sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(),
-                                                     sort.order.as<uint32_t>(), nb, sort.lmax, d_points29, scr.buckets.p,
+        k_msm_accum29<<<cdiv(nt, 128), 128, 0, st>>>(sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(),
+                                                     sort.order.as<uint32_t>(), nb, (uint32_t)split, sort.lmax,
+                                                     d_points29,
+                                                     (uint32_t)sort.n, scr.buckets.p,
                                                      scr.counters.as<uint32_t>(), scr.chunks.as<ChunkRec>(),
                                                      scr.large.as<LargeRec>());
     ZK_HIP(hipGetLastError());
     if (scr.ev_acc1) ZK_HIP(hipEventRecord(scr.ev_acc1, st));
-    k_msm_chunks<T><<<512, 256, 0, st>>>(sort.entries.as<uint32_t>(), d_points, scr.counters.as<uint32_t>(),
-                                          scr.chunks.as<ChunkRec>(), scr.partials.p);
+    k_msm_chunks<T><<<512, 256, 0, st>>>(sort.entries.as<uint32_t>(), d_points, (uint32_t)sort.n,
+                                          scr.counters.as<uint32_t>(), scr.chunks.as<ChunkRec>(), scr.partials.p);
     ZK_HIP(hipGetLastError());
     k_msm_merge<T><<<256, 256, 0, st>>>(scr.counters.as<uint32_t>(), scr.large.as<LargeRec>(), scr.partials.p,
                                          scr.buckets.p);
     ZK_HIP(hipGetLastError());
     uint32_t nseg_all = (uint32_t)g.W * g.nseg;
-    k_msm_reduce1<T><<<cdiv(nseg_all, 128), 128, 0, st>>>(scr.buckets.p, g.K, g.W, g.l1, scr.r1.p, scr.t1.p);
+    k_msm_reduce1<T><<<cdiv(nseg_all, 128), 128, 0, st>>>(scr.buckets.p, g.K, g.W, g.l1, (uint32_t)split, scr.r1.p,
+                                                           scr.t1.p);
     ZK_HIP(hipGetLastError());
     constexpr int BLK = RedBlk<T>::V;
     k_msm_reduce2<T, BLK><<<(unsigned)(g.W * g.nblk), BLK, 0, st>>>(scr.r1.p, scr.t1.p, g.nseg, ilog2(g.l1), g.nblk,
@@ -622,10 +716,11 @@ static void launch_t(const MsmSort& sort, const void* d_points, const void* d_po
 }
 
 void msm_launch(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, MsmPartials& out, hipStream_t st,
-                const void* d_points29) {
+                const void* d_points29, int split) {
     ZK_REQUIRE(sort.n == 0 || d_points29, ZK_E_INTERNAL, "MSM launch without the 29-bit copy of its points");
-    if (g2) launch_t<Fq2>(sort, d_points, d_points29, scr, out, st);
-    else launch_t<Fp>(sort, d_points, d_points29, scr, out, st);
+    ZK_REQUIRE(split >= 1 && split <= MSM_SPLIT_MAX, ZK_E_ARG, "bucket split must be in [1, 4]");
+    if (g2) launch_t<Fq2>(sort, d_points, d_points29, scr, out, st, split);
+    else launch_t<Fp>(sort, d_points, d_points29, scr, out, st, split);
 }
 
 // ---------------------------------------------------------------- host finish
@@ -636,33 +731,54 @@ zkh::Fp load_h<zkh::Fp>(const uint64_t* p) { return zkh::Fp::from_raw(p); }
 template <>
 zkh::Fp2 load_h<zkh::Fp2>(const uint64_t* p) { return {zkh::Fp::from_raw(p), zkh::Fp::from_raw(p + 4)}; }
 
+// value of window w from its level-2 partials: sum_blk (T_blk + blk*U*R_blk)
+// with U = 2^logU, by running sums over the blocks
 template <class HT>
-static zkh::Xyzz<HT> finish_t(const MsmPartials& p) {
+static zkh::Xyzz<HT> window_value(const MsmPartials& p, int w) {
     const MsmGeom& g = p.g;
     const int lw = sizeof(HT) / 8;  // u64 words per coordinate
     const uint64_t* R = p.host.as<uint64_t>();
     const uint64_t* Tt = R + (size_t)g.W * g.nblk * 4 * lw;
-    int logU = ilog2(g.l1) + ilog2(L2) + ilog2(p.g2 ? RedBlk<Fq2>::V : RedBlk<Fp>::V);
+    const int logU = ilog2(g.l1) + ilog2(L2) + ilog2(p.g2 ? RedBlk<Fq2>::V : RedBlk<Fp>::V);
     auto ld = [&](const uint64_t* base, size_t i) {
         const uint64_t* q = base + i * 4 * lw;
         return zkh::Xyzz<HT>{load_h<HT>(q), load_h<HT>(q + lw), load_h<HT>(q + 2 * lw), load_h<HT>(q + 3 * lw)};
     };
+    zkh::Xyzz<HT> run = zkh::Xyzz<HT>::infinity(), acc = run, tsum = run;
+    for (int b = (int)g.nblk - 1; b >= 0; b--) {
+        acc = zkh::xyzz_add(acc, run);
+        run = zkh::xyzz_add(run, ld(R, (size_t)w * g.nblk + b));
+        tsum = zkh::xyzz_add(tsum, ld(Tt, (size_t)w * g.nblk + b));
+    }
+    for (int d = 0; d < logU; d++) acc = zkh::xyzz_dbl(acc);
+    return zkh::xyzz_add(tsum, acc);
+}
+
+// window values (on `tasks` threads when > 1), then Horner across windows
+template <class HT>
+static zkh::Xyzz<HT> finish_t(const MsmPartials& p, int tasks) {
+    const MsmGeom& g = p.g;
+    std::vector<zkh::Xyzz<HT>> S(g.W);
+    if (tasks <= 1 || g.W < 2) {
+        for (int w = 0; w < g.W; w++) S[w] = window_value<HT>(p, w);
+    } else {
+        std::vector<std::future<void>> fs;
+        for (int t = 0; t < tasks; t++)
+            fs.push_back(std::async(std::launch::async, [&, t] {
+                for (int w = t; w < g.W; w += tasks) S[w] = window_value<HT>(p, w);
+            }));
+        for (auto& f : fs) f.get();
+    }
     zkh::Xyzz<HT> total = zkh::Xyzz<HT>::infinity();
     for (int w = g.W - 1; w >= 0; w--) {
         for (int d = 0; d < g.c; d++) total = zkh::xyzz_dbl(total);
-        zkh::Xyzz<HT> run = zkh::Xyzz<HT>::infinity(), acc = run, tsum = run;
-        for (int b = (int)g.nblk - 1; b >= 0; b--) {
-            acc = zkh::xyzz_add(acc, run);
-            run = zkh::xyzz_add(run, ld(R, (size_t)w * g.nblk + b));
-            tsum = zkh::xyzz_add(tsum, ld(Tt, (size_t)w * g.nblk + b));
-        }
-        for (int d = 0; d < logU; d++) acc = zkh::xyzz_dbl(acc);
-        total = zkh::xyzz_add(total, zkh::xyzz_add(tsum, acc));
+        total = zkh::xyzz_add(total, S[w]);
     }
     return total;
 }
 
-zkh::Xyzz<zkh::Fp> msm_finish_g1(const MsmPartials& p) { return finish_t<zkh::Fp>(p); }
-zkh::Xyzz<zkh::Fp2> msm_finish_g2(const MsmPartials& p) { return finish_t<zkh::Fp2>(p); }
+zkh::Xyzz<zkh::Fp> msm_finish_g1(const MsmPartials& p) { return finish_t<zkh::Fp>(p, 1); }
+// G2 window values cost about 4x those of G1: spread them over 4 threads
+zkh::Xyzz<zkh::Fp2> msm_finish_g2(const MsmPartials& p) { return finish_t<zkh::Fp2>(p, 4); }
 
 }  // namespace zk

@@ -3,6 +3,11 @@
 // Reference: msm_g1 (include/msm_g1.cuh:714-913), msm_g2_device
 // (include/msm_g2.cuh:669-854). Same mathematics (signed c-bit digits,
 // buckets, running-sum reduction), different machine design:
+//   * GLV split (glv.hpp): each 254-bit scalar becomes two signed 127-bit
+//     halves over P_i and lambda*P_i = (beta*x, y), so an MSM of n points is
+//     one of 2n points with ceil(128/c) windows: the same bucket entries as
+//     ceil(255/c) windows over n points, half the buckets to reduce and half
+//     the windows to combine;
 //   * digit sort: a two-pass LDS radix sort (tile histograms of bucket
 //     groups + device scan + LDS-ranked scatter, then per-group LDS counting
 //     sort); no global atomics, no CUB, no host synchronisation;
@@ -24,9 +29,11 @@
 
 namespace zk {
 
+constexpr int GLV_BITS = 128;  // signed half scalars: magnitude < 2^127
+
 struct MsmGeom {
     int c;          // window bits
-    int W;          // windows = ceil(255 / c)
+    int W;          // windows = ceil(GLV_BITS / c)
     uint32_t K;     // buckets per window = 2^(c-1)
     uint32_t l1;    // buckets per level-1 reduction thread
     uint32_t nseg;  // level-1 segments per window = K / l1
@@ -43,11 +50,13 @@ MsmGeom msm_geom(size_t n, int c, bool g2);
 // entries take the chunked path and come first.
 struct MsmSort {
     MsmGeom g{};
-    size_t n = 0;
+    size_t n = 0;                 // scalars; the sort itself runs over 2n half scalars
     uint32_t lmax = 0, nkey = 0;
     int qs = 0;
-    DevBuf matrix, mscan, pairs, offsets, entries, order, lkeys;
-    // sorts canonical scalars (n x 32 B, device) into bucket order
+    DevBuf half, matrix, mscan, pairs, offsets, entries, order, lkeys;
+    // sorts canonical scalars (n x 32 B, device, n < 2^30) into bucket order;
+    // an entry is the index i < 2n of a base (msm_points_to29 order) with the
+    // digit's sign in bit 31
     void run(const void* d_scalars, size_t n, int c, hipStream_t st);
 };
 
@@ -66,20 +75,27 @@ struct MsmScratch {
     hipEvent_t ev_acc0 = nullptr, ev_acc1 = nullptr, ev_end = nullptr;
     void enable_timing();
     ~MsmScratch();
-    void ensure(const MsmGeom& g, size_t n_entries, bool g2);
+    void ensure(const MsmGeom& g, size_t n_entries, bool g2, int split);
 };
 
 // Launch bucket accumulation + GPU reduction for points (affine, Montgomery,
 // AoS: G1 64 B / G2 128 B per point) against a finished sort; queue the D2H of
 // the partials into out.host on st. The caller synchronises st before finish.
-// d_points29: the same points in the 29-bit-limb form of msm_points_to29, in
-// which the buckets are accumulated (d_points serves the oversized-bucket
-// path); may be null only for n = 0.
+// d_points29: the base slots of msm_points_to29, in which the buckets are
+// accumulated (d_points serves the oversized-bucket path, which applies the
+// endomorphism itself); may be null only for n = 0.
+// split: threads per bucket in the accumulation (1..MSM_SPLIT_MAX), each over
+// a consecutive part of the bucket's entries; the level-1 reduction adds the
+// parts. More threads than buckets fill the GPU when the buckets are few.
+constexpr int MSM_SPLIT_MAX = 4;
+// split that gives about two rounds of resident waves on a 256-CU GPU
+int msm_default_split(size_t n, int c, bool g2);
 void msm_launch(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, MsmPartials& out,
-                hipStream_t st, const void* d_points29);
+                hipStream_t st, const void* d_points29, int split = 1);
 
-// affine points (standard form, G1 64 B / G2 128 B each) -> 29-bit-limb form
-// (G1: 80 B of limbs in a 128 B slot; G2: 144 B in a 256 B slot)
+// n affine points (standard form, G1 64 B / G2 128 B each) -> n slots of the
+// 29-bit-limb form (G1 128 B, G2 256 B; layout in field29.hpp), each holding
+// P_i and the x of lambda*P_i: the 2n bases of the GLV split.
 constexpr size_t PT29_BYTES = 128, PT29Q_BYTES = 256;
 void msm_points_to29(const void* d_points, size_t n, void* d_out, bool g2, hipStream_t st);
 

@@ -1,12 +1,13 @@
 #include "prover.hpp"
 
-#include <thread>
-
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
+#include <future>
+#include <thread>
 
 #include "qap.hpp"
 #include "synth.hpp"
@@ -243,6 +244,7 @@ ProveSlot::ProveSlot() {
     ZK_HIP(hipEventCreateWithFlags(&ev_scal, hipEventDisableTiming));
     ZK_HIP(hipEventCreateWithFlags(&ev_done1, hipEventDisableTiming));
     for (auto& e : ev_join) ZK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ZK_HIP(hipEventCreateWithFlags(&ev_h, hipEventDisableTiming));
 }
 
 void ProveSlot::enable_timing() {
@@ -259,6 +261,7 @@ ProveSlot::~ProveSlot() {
         if (p) { (void)hipStreamSynchronize(p); (void)hipStreamDestroy(p); }
     if (ev_scal) (void)hipEventDestroy(ev_scal);
     if (ev_done1) (void)hipEventDestroy(ev_done1);
+    if (ev_h) (void)hipEventDestroy(ev_h);
     for (auto& e : ev_join)
         if (e) (void)hipEventDestroy(e);
 }
@@ -322,7 +325,23 @@ void ProveSlot::body(Context& ctx, const Circuit& c, ProofParts& out) {
     NttPlan& plan = ctx.ntt_plan(c.log_n);
     window_w = ctx.window_for(0, nS);
     window_h = ctx.window_for(1, n);
+    const int sp_w1 = ctx.split_for(0, nS, false), sp_w2 = ctx.split_for(0, nS, true), sp_h = ctx.split_for(1, n, false);
     ZK_HIP(hipEventRecord(ev_scal, s0));
+    // witness MSMs first: the G2 one is the longest chain of the proof, so its
+    // sort and accumulation are queued before the H pipeline. One shared sort
+    // on s1, then the four MSMs read the same sorted entries on their own
+    // streams: one MSM's latency-bound reduction overlaps another's
+    // accumulation.
+    ZK_HIP(hipStreamWaitEvent(s1, ev_scal, 0));
+    if (timed) ZK_HIP(hipEventRecord(ev[4], s1));
+    sortW.run(scal.p, nS, window_w, s1);
+    ZK_HIP(hipEventRecord(ev_done1, s1));
+    if (timed) ZK_HIP(hipEventRecord(ev[5], s1));
+    for (hipStream_t p : {s2, s3, s4}) ZK_HIP(hipStreamWaitEvent(p, ev_done1, 0));
+    msm_launch(sortW, c.B2.p, true, scrB2, out.b2, s4, c.B2_29.p, sp_w2);
+    msm_launch(sortW, c.A.p, false, scrA, out.a, s1, c.A29.p, sp_w1);
+    msm_launch(sortW, c.B1.p, false, scrB1, out.b1, s2, c.B1_29.p, sp_w1);
+    msm_launch(sortW, c.C.p, false, scrC, out.c, s3, c.C29.p, sp_w1);
     // H pipeline on s0
     spmv_abc(c.rowA.as<uint32_t>(), c.colA.as<uint32_t>(), c.valA.p, c.n_rowsA, c.rowB.as<uint32_t>(),
              c.colB.as<uint32_t>(), c.valB.p, c.n_rowsB, scal.p, n, aw.p, bw.p, cw.p, s0);
@@ -335,20 +354,8 @@ void ProveSlot::body(Context& ctx, const Circuit& c, ProofParts& out) {
     if (timed) ZK_HIP(hipEventRecord(ev[3], s0));
     h_final(aw.p, bw.p, cw.p, h.p, n, s0);
     sortH.run(h.p, n, window_h, s0);
-    msm_launch(sortH, c.H.p, false, scrH, out.h, s0, c.H29.p);
-    // witness MSMs on s1 (one shared sort)
-    ZK_HIP(hipStreamWaitEvent(s1, ev_scal, 0));
-    if (timed) ZK_HIP(hipEventRecord(ev[4], s1));
-    sortW.run(scal.p, nS, window_w, s1);
-    ZK_HIP(hipEventRecord(ev_done1, s1));
-    if (timed) ZK_HIP(hipEventRecord(ev[5], s1));
-    // the four witness MSMs read the same sorted entries on their own streams:
-    // one MSM's latency-bound reduction overlaps another's accumulation
-    for (hipStream_t p : {s2, s3, s4}) ZK_HIP(hipStreamWaitEvent(p, ev_done1, 0));
-    msm_launch(sortW, c.B2.p, true, scrB2, out.b2, s4, c.B2_29.p);
-    msm_launch(sortW, c.A.p, false, scrA, out.a, s1, c.A29.p);
-    msm_launch(sortW, c.B1.p, false, scrB1, out.b1, s2, c.B1_29.p);
-    msm_launch(sortW, c.C.p, false, scrC, out.c, s3, c.C29.p);
+    msm_launch(sortH, c.H.p, false, scrH, out.h, s0, c.H29.p, sp_h);
+    ZK_HIP(hipEventRecord(ev_h, s0));
     hipStream_t br[4] = {s1, s2, s3, s4};
     for (int k = 0; k < 4; k++) {
         ZK_HIP(hipEventRecord(ev_join[k], br[k]));
@@ -369,13 +376,14 @@ size_t ProveSlot::bytes_needed(const Circuit& c) {
     const size_t nS = (size_t)c.n_vars + 4, n = c.n;
     MsmGeom gw = msm_geom(nS, 0, false), gh = msm_geom(n, 0, false);
     size_t b = nS * 32 + 7 * n * 32;                                    // scalars + Fr arrays
-    b += 3 * ((size_t)gw.W * nS * 4 + (size_t)gh.W * n * 4);            // sorted entries + (entry, bucket) pairs
+    b += 2 * nS * 16 + 2 * n * 16;                                      // GLV half scalars
+    b += 3 * ((size_t)gw.W * 2 * nS * 4 + (size_t)gh.W * 2 * n * 4);    // sorted entries + (entry, bucket) pairs
     b += 4 * 4 * ((size_t)gw.W * gw.K + (size_t)gh.W * gh.K);           // offsets / order
     b += 2 * 4 * (size_t)4096 * (std::min<size_t>(2048, nS / 1024 + 1) + std::min<size_t>(2048, n / 1024 + 1));  // group matrices
     size_t bw = (size_t)gw.W * gw.K, bh = (size_t)gh.W * gh.K;
-    b += 3 * bw * 128 + bw * 256 + bh * 128;                            // buckets A, B1, C, B2, H
+    b += MSM_SPLIT_MAX * (3 * bw * 128 + bw * 256 + bh * 128);          // bucket slots A, B1, C, B2, H (max split)
     b += (3 * bw + bw * 2 + bh) * 128 / 4;                              // reduction partials (upper bound)
-    b += ((size_t)gw.W * nS / 64) * (16 + 128) * 4 + ((size_t)gh.W * n / 64) * (16 + 128);  // large-bucket lists
+    b += ((size_t)gw.W * 2 * nS / 64) * (16 + 128) * 4 + ((size_t)gh.W * 2 * n / 64) * (16 + 128);  // large-bucket lists
     return b;
 }
 
@@ -388,29 +396,51 @@ static void put_aff(const zkh::Xyzz<T>& p, uint64_t* dst) {
     memcpy(dst + lw, &a.y, 8 * lw);
 }
 
-void assemble_proof(const ProofParts& p, const uint64_t r[4], const uint64_t s[4], zk_proof* out) {
-    auto A = msm_finish_g1(p.a);
-    auto B1 = msm_finish_g1(p.b1);
-    auto Cw = msm_finish_g1(p.c);
-    auto B2 = msm_finish_g2(p.b2);
-    auto Hp = msm_finish_g1(p.h);
-    auto C = zkh::xyzz_add(Cw, Hp);
-    if (!is_zero4(s)) C = zkh::xyzz_add(C, zkh::xyzz_mul(A, s));
-    if (!is_zero4(r)) C = zkh::xyzz_add(C, zkh::xyzz_mul(B1, r));
-    put_aff(A, out->pi_a);
-    put_aff(B2, out->pi_b);
+double assemble_proof(const ProofParts& p, const uint64_t r[4], const uint64_t s[4], zk_proof* out,
+                      const hipEvent_t* ready) {
+    using clk = std::chrono::steady_clock;
+    using G1 = zkh::Xyzz<zkh::Fp>;
+    std::atomic<int64_t> t_last{0};   // latest completion seen, ns since t0
+    const auto t0 = clk::now();
+    auto wait = [&](int k) {
+        if (ready && ready[k]) ZK_HIP(hipEventSynchronize(ready[k]));
+        const int64_t t = std::chrono::duration_cast<std::chrono::nanoseconds>(clk::now() - t0).count();
+        int64_t prev = t_last.load();
+        while (t > prev && !t_last.compare_exchange_weak(prev, t)) {}
+    };
+    // pi_A and s*pi_A; r*B1; C' + H: one thread each. pi_B (G2, the longest) here.
+    auto fa = std::async(std::launch::async, [&] {
+        wait(0);
+        G1 A = msm_finish_g1(p.a);
+        return std::make_pair(A, is_zero4(s) ? G1::infinity() : zkh::xyzz_mul(A, s));
+    });
+    auto fb = std::async(std::launch::async, [&] {
+        wait(1);
+        return is_zero4(r) ? G1::infinity() : zkh::xyzz_mul(msm_finish_g1(p.b1), r);
+    });
+    auto fc = std::async(std::launch::async, [&] {
+        wait(2);
+        G1 Cw = msm_finish_g1(p.c);
+        wait(4);
+        return zkh::xyzz_add(Cw, msm_finish_g1(p.h));
+    });
+    wait(3);
+    put_aff(msm_finish_g2(p.b2), out->pi_b);
+    auto A = fa.get();
+    put_aff(A.first, out->pi_a);
+    G1 C = zkh::xyzz_add(zkh::xyzz_add(fc.get(), A.second), fb.get());
     put_aff(C, out->pi_c);
+    const double total = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+    return total - t_last.load() / 1e6;
 }
 
 void ProveSlot::finish(zk_proof* out) {
-    using clk = std::chrono::steady_clock;
+    const hipEvent_t ready[5] = {ev_join[0], ev_join[1], ev_join[2], ev_join[3], ev_h};
+    const double tail_ms = assemble_proof(parts, r, s, out, ready);
     ZK_HIP(hipStreamSynchronize(s0));   // s1..s4 are joined into s0
-    auto h0 = clk::now();
-    assemble_proof(parts, r, s, out);
-    double host_ms = std::chrono::duration<double, std::milli>(clk::now() - h0).count();
     if (timed) {
         collect_stats();
-        stats[23] = host_ms;
+        stats[23] = tail_ms;
     }
 }
 

@@ -40,7 +40,8 @@ struct Circuit {
     DevBuf rowA, colA, valA, rowB, colB, valB;
     // extended bases, n_vars + 4 points each (see header comment); H: n points
     DevBuf A, B1, B2, C, H;
-    // the bases again in the 29-bit-limb form of the bucket accumulation
+    // the bases again in the 29-bit-limb form of the bucket accumulation,
+    // each slot also holding the x of its endomorphism image (msm.hpp)
     DevBuf A29, B1_29, B2_29, C29, H29;
     ZkeyHeader hdr;               // VK in file layout
     std::vector<uint8_t> coefs;   // §4 bytes, kept to write the circuit back out
@@ -55,14 +56,19 @@ struct ProofParts {
 };
 
 // Host tail of a proof: finish the MSMs and assemble pi_A, pi_B,
-// pi_C = C' + H + s*pi_A + r*B1 (Montgomery affine).
-void assemble_proof(const ProofParts& p, const uint64_t r[4], const uint64_t s[4], zk_proof* out);
+// pi_C = C' + H + s*pi_A + r*B1 (Montgomery affine). ready, when given, holds
+// one event per MSM (a, b1, c, b2, h) after which its partials are on the
+// host; each MSM is finished on its own thread as soon as its event
+// completes, so the host work overlaps the MSMs still running. Returns the
+// host milliseconds spent after the last of them completed.
+double assemble_proof(const ProofParts& p, const uint64_t r[4], const uint64_t s[4], zk_proof* out,
+                      const hipEvent_t* ready = nullptr);
 
 struct ProveSlot {
     // s0: H pipeline + H MSM; s1: witness sort + A; s2: B1; s3: C; s4: B2.
     // Each proof forks s1..s4 from s0 and joins them back into s0 at its end.
     hipStream_t s0 = nullptr, s1 = nullptr, s2 = nullptr, s3 = nullptr, s4 = nullptr;
-    hipEvent_t ev_scal = nullptr, ev_done1 = nullptr, ev_join[4] = {};
+    hipEvent_t ev_scal = nullptr, ev_done1 = nullptr, ev_join[4] = {}, ev_h = nullptr;
     DevBuf scal, aw, bw, cw, t0, t1, t2, h, wtmp;
     HostBuf staging;                 // pinned copy of a host witness (async upload)
     MsmSort sortW, sortH;
@@ -112,14 +118,20 @@ struct Context {
     std::unique_ptr<ProveSlot> slot;   // slot used by the one-shot prove entry point
     DevBuf comb_g1, comb_g2;           // fixed-base tables for synthetic setup
     // MSM windows: forced (nonzero) or tuned per (kind, ceil log2 n), else the
-    // size default. kind 0 = witness sort (A', B1', C', B2'), 1 = H.
+    // size default. kind 0 = witness sort (A', B1', C', B2'), 1 = H. With a
+    // tuned window come the tuned bucket splits of its G1 and G2 MSMs; else
+    // msm_default_split.
+    struct MsmChoice {
+        int c, split_g1, split_g2;   // split 0: msm_default_split
+    };
     int force_w = 0, force_h = 0;
-    std::map<std::pair<int, int>, int> tuned;
+    std::map<std::pair<int, int>, MsmChoice> tuned;
     explicit Context(int dev);
     ~Context();
     NttPlan& ntt_plan(int log_n);
     void ensure_comb();
     int window_for(int kind, size_t n) const;
+    int split_for(int kind, size_t n, bool g2) const;
     std::string arch() const;
     void autotune(const Circuit& c, bool force, int reps, int chosen[2]);
     int tune_load(const std::string& path);

@@ -1,11 +1,14 @@
 // MSM window autotuner.
 //
 // The reference fixes the window at compile time (MSM_G1_C_BITS, default 14,
-// include/msm_g1.cuh:11; C=14 in include/msm_g2.cuh:70-110). Here the window
-// is chosen per MSM size class by measurement on the device: for each
-// candidate c the digit sort and the MSMs that share it run on the context
-// stream over the circuit's resident bases and uniform scalars, timed with HIP
-// events, and the fastest c is recorded for (kind, ceil log2 n). Two kinds:
+// include/msm_g1.cuh:11; C=14 in include/msm_g2.cuh:70-110). Here the window c
+// and the bucket split of the accumulation (threads per bucket, msm.hpp) are
+// chosen per MSM size class by measurement on the device: for each candidate
+// c the digit sort runs once and each MSM that shares it runs at every split,
+// on the context stream over the circuit's resident bases and uniform
+// scalars, timed with HIP events. A candidate's cost is the sort plus, per
+// MSM, its fastest split; the cheapest c is recorded for (kind, ceil log2 n)
+// with its G1 and G2 splits. Two kinds:
 //   kind 0: the witness sort over n_vars + 4 scalars and its four MSMs
 //           (A', B1', C' in G1, B2' in G2), the cost one proof pays for c_w;
 //   kind 1: the sort of h over n scalars and the H MSM (c_h).
@@ -31,7 +34,17 @@ int Context::window_for(int kind, size_t n) const {
     if (kind == 0 && force_w) return force_w;
     if (kind == 1 && force_h) return force_h;
     auto it = tuned.find({kind, size_class(n)});
-    return it != tuned.end() ? it->second : msm_default_c(n);
+    return it != tuned.end() ? it->second.c : msm_default_c(n);
+}
+
+int Context::split_for(int kind, size_t n, bool g2) const {
+    const bool forced = kind == 0 ? force_w != 0 : force_h != 0;
+    auto it = tuned.find({kind, size_class(n)});
+    if (!forced && it != tuned.end()) {
+        const int sp = g2 ? it->second.split_g2 : it->second.split_g1;
+        if (sp > 0) return sp;
+    }
+    return msm_default_split(n, window_for(kind, n), g2);
 }
 
 std::string Context::arch() const {
@@ -59,29 +72,45 @@ struct TuneBufs {
     }
 };
 
-// device ms of one sort + the MSMs of `kind` at window c (mean over reps after one warm-up)
-float time_window(Context& ctx, const Circuit& circ, int kind, int c, const void* d_sc, size_t n, int reps,
-                  TuneBufs& tb) {
+float elapsed(TuneBufs& tb) {
+    ZK_HIP(hipEventSynchronize(tb.e1));
+    float ms = 0;
+    ZK_HIP(hipEventElapsedTime(&ms, tb.e0, tb.e1));
+    return ms;
+}
+
+// Device ms at window c, mean over reps after one warm-up: the sort, and each
+// MSM of `kind` at each split 1..MSM_SPLIT_MAX (msm[m][split-1]; splits that
+// would start more than 2^21 threads are skipped and left at -1).
+void time_window(Context& ctx, const Circuit& circ, int kind, int c, const void* d_sc, size_t n, int reps,
+                 TuneBufs& tb, float& sort_ms, float msm[4][MSM_SPLIT_MAX]) {
     hipStream_t st = ctx.stream;
-    float best = 0;
+    struct M { const void* pts; const void* p29; bool g2; };
+    const M w4[4] = {{circ.A.p, circ.A29.p, false}, {circ.B1.p, circ.B1_29.p, false}, {circ.C.p, circ.C29.p, false},
+                     {circ.B2.p, circ.B2_29.p, true}};
+    const M h1[1] = {{circ.H.p, circ.H29.p, false}};
+    const M* ms = kind == 0 ? w4 : h1;
+    const int nm = kind == 0 ? 4 : 1;
+    const MsmGeom g = msm_geom(n, c, false);
+    sort_ms = 0;
+    for (int m = 0; m < 4; m++)
+        for (int s = 0; s < MSM_SPLIT_MAX; s++) msm[m][s] = -1;
     for (int it = 0; it <= reps; it++) {
         ZK_HIP(hipEventRecord(tb.e0, st));
         tb.sort.run(d_sc, n, c, st);
-        if (kind == 0) {
-            msm_launch(tb.sort, circ.A.p, false, tb.g1, tb.part, st, circ.A29.p);
-            msm_launch(tb.sort, circ.B1.p, false, tb.g1, tb.part, st, circ.B1_29.p);
-            msm_launch(tb.sort, circ.C.p, false, tb.g1, tb.part, st, circ.C29.p);
-            msm_launch(tb.sort, circ.B2.p, true, tb.g2, tb.part, st, circ.B2_29.p);
-        } else {
-            msm_launch(tb.sort, circ.H.p, false, tb.g1, tb.part, st, circ.H29.p);
-        }
         ZK_HIP(hipEventRecord(tb.e1, st));
-        ZK_HIP(hipEventSynchronize(tb.e1));
-        float ms = 0;
-        ZK_HIP(hipEventElapsedTime(&ms, tb.e0, tb.e1));
-        if (it > 0) best += ms / reps;
+        const float t_sort = elapsed(tb);
+        if (it > 0) sort_ms += t_sort / reps;
+        for (int m = 0; m < nm; m++)
+            for (int s = 1; s <= MSM_SPLIT_MAX; s++) {
+                if ((size_t)g.W * g.K * s > ((size_t)1 << 21) && s > 1) continue;
+                ZK_HIP(hipEventRecord(tb.e0, st));
+                msm_launch(tb.sort, ms[m].pts, ms[m].g2, ms[m].g2 ? tb.g2 : tb.g1, tb.part, st, ms[m].p29, s);
+                ZK_HIP(hipEventRecord(tb.e1, st));
+                const float t = elapsed(tb);
+                if (it > 0) msm[m][s - 1] = (msm[m][s - 1] < 0 ? 0 : msm[m][s - 1]) + t / reps;
+            }
     }
-    return best;
 }
 
 }  // namespace
@@ -95,7 +124,7 @@ void Context::autotune(const Circuit& circ, bool force, int reps, int chosen[2])
         const int cls = size_class(n);
         auto key = std::make_pair(kind, cls);
         if (!force && tuned.count(key)) {
-            chosen[kind] = tuned[key];
+            chosen[kind] = tuned[key].c;
             continue;
         }
         std::vector<uint64_t> sc(4 * n);
@@ -103,22 +132,37 @@ void Context::autotune(const Circuit& circ, bool force, int reps, int chosen[2])
         DevBuf d_sc;
         d_sc.ensure(n * 32);
         ZK_HIP(hipMemcpy(d_sc.p, sc.data(), n * 32, hipMemcpyHostToDevice));
-        int best_c = msm_default_c(n);
+        MsmChoice best{msm_default_c(n), 1, 1};
         float best_ms = -1;
-        const int hi = std::min(TUNE_C_MAX, std::max(TUNE_C_MIN, cls + 2));
-        for (int c = TUNE_C_MIN; c <= hi; c++) {
-            float ms = time_window(*this, circ, kind, c, d_sc.p, n, reps, tb);
-            if (best_ms < 0 || ms < best_ms) {
-                best_ms = ms;
-                best_c = c;
+        // candidate windows around the size: wide windows for large MSMs,
+        // narrow ones for small
+        const int hi = std::min(TUNE_C_MAX, std::max(TUNE_C_MIN, cls - 2));
+        const int lo = std::max(TUNE_C_MIN, std::min(cls - 7, hi - 4));
+        for (int c = lo; c <= hi; c++) {
+            float sort_ms, msm[4][MSM_SPLIT_MAX];
+            time_window(*this, circ, kind, c, d_sc.p, n, reps, tb, sort_ms, msm);
+            float total = sort_ms;
+            int sp[4] = {1, 1, 1, 1};
+            for (int m = 0; m < (kind == 0 ? 4 : 1); m++) {
+                for (int s = 2; s <= MSM_SPLIT_MAX; s++)
+                    if (msm[m][s - 1] >= 0 && msm[m][s - 1] < msm[m][sp[m] - 1]) sp[m] = s;
+                total += msm[m][sp[m] - 1];
+            }
+            if (best_ms < 0 || total < best_ms) {
+                best_ms = total;
+                // G1 split: the one most of the G1 MSMs of this kind chose
+                int g1 = sp[0];
+                if (kind == 0 && sp[1] == sp[2]) g1 = sp[1];
+                best = {c, g1, kind == 0 ? sp[3] : 1};
             }
         }
-        tuned[key] = best_c;
-        chosen[kind] = best_c;
+        tuned[key] = best;
+        chosen[kind] = best.c;
     }
 }
 
-// text table, one entry per line: "<arch> <kind> <log2 n> <c>"; '#' starts a comment
+// text table, one entry per line: "<arch> <kind> <log2 n> <c> [<split G1> <split G2>]";
+// '#' starts a comment; an entry without splits uses msm_default_split
 int Context::tune_load(const std::string& path) {
     std::ifstream f(path);
     ZK_REQUIRE(f.good(), ZK_E_IO, "cannot open tuning table " + path);
@@ -129,12 +173,14 @@ int Context::tune_load(const std::string& path) {
         if (line.empty() || line[0] == '#') continue;
         std::istringstream is(line);
         std::string a;
-        int kind, cls, c;
+        int kind, cls, c, s1 = 0, s2 = 0;
         ZK_REQUIRE(bool(is >> a >> kind >> cls >> c), ZK_E_FORMAT, "bad tuning table line: " + line);
-        ZK_REQUIRE(kind >= 0 && kind <= 1 && cls >= 0 && cls <= 40 && c >= 4 && c <= 22, ZK_E_FORMAT,
-                   "bad tuning table entry: " + line);
+        if (is >> s1) ZK_REQUIRE(bool(is >> s2), ZK_E_FORMAT, "bad tuning table line: " + line);
+        ZK_REQUIRE(kind >= 0 && kind <= 1 && cls >= 0 && cls <= 40 && c >= 4 && c <= 22 && s1 >= 0 &&
+                       s1 <= MSM_SPLIT_MAX && s2 >= 0 && s2 <= MSM_SPLIT_MAX,
+                   ZK_E_FORMAT, "bad tuning table entry: " + line);
         if (a != me) continue;
-        tuned[{kind, cls}] = c;
+        tuned[{kind, cls}] = {c, s1, s2};
         n++;
     }
     return n;
@@ -143,9 +189,11 @@ int Context::tune_load(const std::string& path) {
 void Context::tune_save(const std::string& path) const {
     std::ofstream f(path, std::ios::trunc);
     ZK_REQUIRE(f.good(), ZK_E_IO, "cannot write tuning table " + path);
-    f << "# zyklop_amd MSM windows: arch kind(0 witness, 1 h) ceil-log2(n) c\n";
+    f << "# zyklop_amd MSM windows: arch kind(0 witness, 1 h) ceil-log2(n) c split-G1 split-G2\n";
     const std::string me = arch();
-    for (auto& kv : tuned) f << me << ' ' << kv.first.first << ' ' << kv.first.second << ' ' << kv.second << '\n';
+    for (auto& kv : tuned)
+        f << me << ' ' << kv.first.first << ' ' << kv.first.second << ' ' << kv.second.c << ' ' << kv.second.split_g1
+          << ' ' << kv.second.split_g2 << '\n';
     ZK_REQUIRE(f.good(), ZK_E_IO, "write failed: " + path);
 }
 
