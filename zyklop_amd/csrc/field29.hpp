@@ -116,6 +116,46 @@ ZK_DEV F29 f29_mul(const F29& a, const F29& b) {
     return t;
 }
 
+// sum of N Montgomery products, sum_k a_k*b_k/R' mod p, with one reduction
+// (product scanning as in f29_mul). Operands normalized (limbs < 2^29); a
+// column then sums at most (2N + 1)*9 products below 2^58 plus the carry,
+// under 2^64 for N <= 4. With operands below 6p and N*36p^2 < 2^261*p/2 the
+// result is below 2p. A difference a*b - c*d is the sum with c replaced by
+// f29_neg(c).
+template <int N>
+ZK_DEV F29 f29_mulsum(const F29 (&a)[N], const F29 (&b)[N]) {
+    static_assert(N >= 1 && N <= 4, "column sums must stay below 2^64");
+    using namespace f29c;
+    uint32_t m[9];
+    F29 t;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < 17; k++) {
+#pragma unroll
+        for (int q = 0; q < N; q++)
+#pragma unroll
+            for (int i = 0; i < 9; i++) {
+                const int j = k - i;
+                if (j >= 0 && j < 9) acc += (uint64_t)a[q].l[i] * b[q].l[j];
+            }
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            const int j = k - i;
+            if (i < k && j >= 0 && j < 9) acc += (uint64_t)m[i] * P[j];
+        }
+        if (k < 9) {
+            m[k] = f29_qword((uint32_t)acc);
+            acc += (uint64_t)m[k] * P[0];
+        } else {
+            t.l[k - 9] = (uint32_t)acc & MASK;
+        }
+        acc >>= 29;
+        asm volatile("" : "+v"(acc));
+    }
+    t.l[8] = (uint32_t)acc;
+    return t;
+}
+
 // carry propagation: limbs < 2^32 -> limbs < 2^29 (same value)
 ZK_DEV F29 f29_norm(const F29& a) {
     F29 r;
@@ -268,10 +308,11 @@ ZK_DEV void f29_madd(Xyzz29& A, const F29& x2, const F29& y2) {
 #pragma unroll
     for (int i = 0; i < 9; i++) x3.l[i] = (P12[i] - PPP.l[i] - 2 * Q.l[i]) + RR.l[i];
     const F29 X3 = f29_reduce(f29_norm(x3));
+    // Y3 = R*(Q - X3) - Y1*PPP with one reduction: R, Q - X3 < 6p, Y1 and
+    // -PPP <= 2p, so the sum of products stays under the f29_mulsum bound
     const F29 T = f29_norm(f29_sub4(Q, X3));
-    const F29 m1 = f29_mul(R_, T);
-    const F29 m2 = f29_mul(A.Y, PPP);
-    A.Y = f29_reduce(f29_norm(f29_sub4(m1, m2)));
+    const F29 ya[2] = {R_, A.Y}, yb[2] = {T, f29_neg(PPP)};
+    A.Y = f29_mulsum<2>(ya, yb);
     A.X = X3;
     A.ZZ = f29_mul(A.ZZ, PP);
     A.ZZZ = f29_mul(A.ZZZ, PPP);
@@ -356,15 +397,20 @@ ZK_DEV void f29q_load_point(const void* base, uint32_t idx, uint32_t n, F29q& x,
     x.c1 = f29_load_packed(px + 2);
 }
 
-// A G2 XYZZ accumulator kept in LDS: 4 coordinates x 18 limbs, laid out
-// [coordinate][limb][lane] over a block of NT threads (consecutive lanes hit
-// consecutive banks). Coordinates are loaded where they are used, so the
-// accumulator does not occupy 72 registers for the whole loop.
+// A G2 XYZZ accumulator: X, Y and ZZZ in LDS, 3 coordinates x 18 limbs laid
+// out [coordinate][limb][lane] over a block of NT threads (consecutive lanes
+// hit consecutive banks), and ZZ in registers. The LDS coordinates are loaded
+// where they are used, so they do not occupy 54 registers for the whole loop;
+// with ZZ in registers the LDS share per thread is 216 B, which leaves room
+// for more waves per CU than 288 B would.
 template <int NT>
 struct AccLds {
     uint32_t* base;   // &lds[0][0][lane]
-    enum { X = 0, Y = 1, ZZ = 2, ZZZ = 3 };
+    F29q zz;
+    enum { X = 0, Y = 1, ZZZ = 2, ZZ = 3 };
+    static constexpr int LDS_WORDS = 3 * 18 * NT;
     ZK_DEV F29q ld(int c) const {
+        if (c == ZZ) return zz;
         asm volatile("" ::: "memory");
         F29q v;
 #pragma unroll
@@ -374,7 +420,11 @@ struct AccLds {
         }
         return v;
     }
-    ZK_DEV void st(int c, const F29q& v) const {
+    ZK_DEV void st(int c, const F29q& v) {
+        if (c == ZZ) {
+            zz = v;
+            return;
+        }
 #pragma unroll
         for (int i = 0; i < 9; i++) {
             base[(c * 18 + i) * NT] = v.c0.l[i];
@@ -384,11 +434,11 @@ struct AccLds {
     }
 };
 
-// madd-2008-s over Fp2 on an LDS accumulator, acc += (x2, y2) with components
+// madd-2008-s over Fp2 on that accumulator, acc += (x2, y2) with components
 // of x2 < p and y2 < 2p (normalized); inf tracks the empty/infinite sum. The
 // equal-x cases go through the standard-form formulas.
 template <int NT>
-ZK_DEV void f29q_madd(const AccLds<NT>& A, bool& inf, const F29q& x2, const F29q& y2) {
+ZK_DEV void f29q_madd(AccLds<NT>& A, bool& inf, const F29q& x2, const F29q& y2) {
     using namespace f29c;
     using L = AccLds<NT>;
     if (inf) {
@@ -438,9 +488,15 @@ ZK_DEV void f29q_madd(const AccLds<NT>& A, bool& inf, const F29q& x2, const F29q
         X3 = {f29_reduce(f29_norm(x0)), f29_reduce(f29_norm(x1))};
     }
     A.st(L::X, X3);
-    const F29q m1 = f29q_mul(f29q_sub_nr(Q, X3), R_);
-    const F29q m2 = f29q_mul(A.ld(L::Y), PPP);
-    A.st(L::Y, f29q_sub(m1, m2));
+    // Y3 = (Q - X3)*R - Y1*PPP, each Fp2 component as one sum of four
+    // products with one reduction: T = Q - X3 < 6p, R, Y1, PPP and their
+    // negations <= 2p, so the products sum below 32p^2 (f29_mulsum bound)
+    const F29q T = f29q_sub_nr(Q, X3);
+    const F29q Y1 = A.ld(L::Y);
+    const F29 nr1 = f29_neg(R_.c1), ny0 = f29_neg(Y1.c0), ny1 = f29_neg(Y1.c1);
+    const F29 a0[4] = {T.c0, T.c1, ny0, Y1.c1}, b0[4] = {R_.c0, nr1, PPP.c0, PPP.c1};
+    const F29 a1[4] = {T.c0, T.c1, ny0, ny1}, b1[4] = {R_.c1, R_.c0, PPP.c1, PPP.c0};
+    A.st(L::Y, F29q{f29_mulsum<4>(a0, b0), f29_mulsum<4>(a1, b1)});
 }
 
 }  // namespace zk

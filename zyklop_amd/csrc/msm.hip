@@ -34,8 +34,8 @@ int msm_default_c(size_t n) {
 
 int msm_default_split(size_t n, int c, bool g2) {
     const MsmGeom g = msm_geom(n, c, g2);
-    // resident waves per SIMD: G1 accumulation 6 (80 VGPRs), G2 2 (LDS)
-    const size_t target = (size_t)1024 * 64 * 2 * (g2 ? 2 : 6);
+    // resident waves per SIMD: G1 accumulation 6 (80 VGPRs), G2 about 3 (LDS)
+    const size_t target = (size_t)1024 * 64 * 2 * (g2 ? 3 : 6);
     const size_t nb = (size_t)g.W * g.K;
     return (int)std::max<size_t>(1, std::min<size_t>(3, target / nb));
 }
@@ -44,8 +44,12 @@ MsmGeom msm_geom(size_t n, int c, bool g2) {
     if (c <= 0) c = msm_default_c(n);
     ZK_REQUIRE(c >= 4 && c <= 22, ZK_E_ARG, "MSM window must be in [4, 22]");
     MsmGeom g;
-    g.c = c;
+    // W windows of the requested width at most, then the 128 bits shared
+    // evenly: nlo windows of c - 1 bits, the others of c bits
     g.W = (GLV_BITS + c - 1) / c;
+    c = (GLV_BITS + g.W - 1) / g.W;
+    g.c = c;
+    g.nlo = g.W * c - GLV_BITS;
     g.K = 1u << (c - 1);
     g.l1 = std::min<uint32_t>(L1_MAX, g.K);
     g.nseg = g.K / g.l1;
@@ -72,30 +76,34 @@ __global__ void k_glv_split(const uint32_t* __restrict__ sc, size_t n, uint4* __
 }
 
 // ---------------------------------------------------------------- digits
-// Signed c-bit digits of a half scalar, least significant window first:
-// v = window bits + carry; v > 2^(c-1) becomes v - 2^c with a carry into the
-// next window (include/msm_g1.cuh:271-329 uses the same recoding). The
-// magnitude is below 2^127, so W*c >= 128 leaves no carry out of the top
-// window. A negative half scalar flips the sign of every digit.
+// Signed digits of a half scalar, least significant window first, window w
+// being c - 1 bits wide for w < nlo and c bits above (128 bits in all):
+// v = window bits + carry; v > 2^(width-1) becomes v - 2^width with a carry
+// into the next window (include/msm_g1.cuh:271-329 uses the same recoding
+// with one width). The magnitude is below 2^127, so the top window's v stays
+// at most 2^(width-1): no carry out. Every magnitude fits the 2^(c-1) buckets
+// of a window. A negative half scalar flips the sign of every digit.
 struct DigitIter {
     uint32_t s[4];
-    uint32_t carry, mask, half, c;
+    uint32_t carry, c, left;
     bool flip;
-    ZK_DEV void init(const uint32_t* hs, size_t i, int cc) {
+    ZK_DEV void init(const uint32_t* hs, size_t i, int cc, int nlo) {
         const uint4 a = reinterpret_cast<const uint4*>(hs)[i];
         s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w & 0x7fffffffu;
         flip = (a.w >> 31) != 0;
         carry = 0;
         c = (uint32_t)cc;
-        mask = (1u << cc) - 1;
-        half = 1u << (cc - 1);
+        left = (uint32_t)nlo;
     }
     // returns magnitude (0 = skip) and sign
     ZK_DEV uint32_t next(bool& neg) {
+        const uint32_t wd = left ? c - 1 : c;
+        left -= left ? 1u : 0u;
+        const uint32_t mask = (1u << wd) - 1, half = 1u << (wd - 1);
         uint32_t v = (s[0] & mask) + carry;
 #pragma unroll
-        for (int k = 0; k < 3; k++) s[k] = __builtin_amdgcn_alignbit(s[k + 1], s[k], c);
-        s[3] >>= c;
+        for (int k = 0; k < 3; k++) s[k] = __builtin_amdgcn_alignbit(s[k + 1], s[k], wd);
+        s[3] >>= wd;
         if (v > half) {
             carry = 1;
             neg = !flip;
@@ -116,7 +124,7 @@ struct DigitIter {
 // Pass B, per group: LDS counts of its 2^s buckets give the bucket offsets,
 // then the entries go to offset + LDS rank.
 __global__ void __launch_bounds__(1024) k_sort_tile_hist(const uint32_t* __restrict__ sc, size_t n, size_t tile, int c,
-                                                         int W, uint32_t K, int s, uint32_t G, uint32_t nt,
+                                                         int nlo, int W, uint32_t K, int s, uint32_t G, uint32_t nt,
                                                          uint32_t* __restrict__ matrix) {
     extern __shared__ uint32_t lh[];
     for (uint32_t k = threadIdx.x; k < G; k += blockDim.x) lh[k] = 0;
@@ -124,7 +132,7 @@ __global__ void __launch_bounds__(1024) k_sort_tile_hist(const uint32_t* __restr
     const size_t i0 = (size_t)blockIdx.x * tile, i1 = i0 + tile < n ? i0 + tile : n;
     for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         DigitIter it;
-        it.init(sc, i, c);
+        it.init(sc, i, c, nlo);
         for (int w = 0; w < W; w++) {
             bool ng;
             uint32_t mag = it.next(ng);
@@ -136,7 +144,7 @@ __global__ void __launch_bounds__(1024) k_sort_tile_hist(const uint32_t* __restr
 }
 
 __global__ void __launch_bounds__(1024) k_sort_tile_scatter(const uint32_t* __restrict__ sc, size_t n, size_t tile, int c,
-                                                            int W, uint32_t K, int s, uint32_t G, uint32_t nt,
+                                                            int nlo, int W, uint32_t K, int s, uint32_t G, uint32_t nt,
                                                             const uint32_t* __restrict__ mscan, uint2* __restrict__ pairs) {
     extern __shared__ uint32_t lds[];
     uint32_t* base = lds;       // this tile's run start in each group
@@ -149,7 +157,7 @@ __global__ void __launch_bounds__(1024) k_sort_tile_scatter(const uint32_t* __re
     const size_t i0 = (size_t)blockIdx.x * tile, i1 = i0 + tile < n ? i0 + tile : n;
     for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         DigitIter it;
-        it.init(sc, i, c);
+        it.init(sc, i, c, nlo);
         for (int w = 0; w < W; w++) {
             bool ng;
             uint32_t mag = it.next(ng);
@@ -268,7 +276,7 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
     uint32_t* mat = matrix.as<uint32_t>();
     uint32_t* msc = mscan.as<uint32_t>();
     if (n) {
-        k_sort_tile_hist<<<nt, 1024, G * 4, st>>>(hs, n2, tile, g.c, g.W, g.K, sh, G, nt, mat);
+        k_sort_tile_hist<<<nt, 1024, G * 4, st>>>(hs, n2, tile, g.c, g.nlo, g.W, g.K, sh, G, nt, mat);
         ZK_HIP(hipGetLastError());
         fill_u32(mat + mlen - 1, 0u, 1, st);
     } else {
@@ -276,7 +284,7 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
     }
     exclusive_scan(mat, msc, msc + mlen, mlen, st);
     if (n) {
-        k_sort_tile_scatter<<<nt, 1024, 2 * G * 4, st>>>(hs, n2, tile, g.c, g.W, g.K, sh, G, nt, msc,
+        k_sort_tile_scatter<<<nt, 1024, 2 * G * 4, st>>>(hs, n2, tile, g.c, g.nlo, g.W, g.K, sh, G, nt, msc,
                                                           pairs.as<uint2>());
         ZK_HIP(hipGetLastError());
     }
@@ -348,7 +356,7 @@ ZK_DEV bool acc_range(uint32_t tid, uint32_t nb, uint32_t split, const uint32_t*
     return true;
 }
 
-__global__ void __launch_bounds__(256) k_msm_accum29(const uint32_t* __restrict__ entries,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_msm_accum29(const uint32_t* __restrict__ entries,
                                                      const uint32_t* __restrict__ offsets,
                                                      const uint32_t* __restrict__ order, uint32_t nb, uint32_t split,
                                                      uint32_t lmax,
@@ -376,10 +384,11 @@ __global__ void __launch_bounds__(256) k_msm_accum29(const uint32_t* __restrict_
 }
 
 // G2 accumulation in the 29-bit-limb form: the Fp2 state does not fit the
-// register file at two waves per SIMD, so each thread's bucket accumulator
-// lives in LDS (field29.hpp AccLds) and its coordinates are loaded where used.
-constexpr int ACC29Q_THREADS = 128;
-__global__ void __launch_bounds__(ACC29Q_THREADS) k_msm_accum29q(const uint32_t* __restrict__ entries,
+// register file next to the mixed add, so three coordinates of each thread's
+// bucket accumulator live in LDS (field29.hpp AccLds) and are loaded where
+// used. One-wave blocks let the LDS share set the waves per CU.
+constexpr int ACC29Q_THREADS = 64;
+__global__ void __launch_bounds__(ACC29Q_THREADS) __attribute__((amdgpu_waves_per_eu(2))) k_msm_accum29q(const uint32_t* __restrict__ entries,
                                                                  const uint32_t* __restrict__ offsets,
                                                                  const uint32_t* __restrict__ order, uint32_t nb,
                                                                  uint32_t split, uint32_t lmax,
@@ -388,12 +397,12 @@ __global__ void __launch_bounds__(ACC29Q_THREADS) k_msm_accum29q(const uint32_t*
                                                                  uint32_t* __restrict__ counters,
                                                                  ChunkRec* __restrict__ chunks,
                                                                  LargeRec* __restrict__ large) {
-    __shared__ uint32_t lds[4 * 18 * ACC29Q_THREADS];
+    __shared__ uint32_t lds[AccLds<ACC29Q_THREADS>::LDS_WORDS];
     uint32_t st, en, slot;
     if (!acc_range<Fq2>(blockIdx.x * blockDim.x + threadIdx.x, nb, split, order, offsets, lmax, buckets, counters, chunks,
                        large, st, en, slot))
         return;
-    const AccLds<ACC29Q_THREADS> A{lds + threadIdx.x};
+    AccLds<ACC29Q_THREADS> A{lds + threadIdx.x, {}};
     bool inf = true;
     for (uint32_t e = st; e < en; e++) {
         uint32_t ref = entries[e];
@@ -769,9 +778,11 @@ static zkh::Xyzz<HT> finish_t(const MsmPartials& p, int tasks) {
             }));
         for (auto& f : fs) f.get();
     }
+    // total = sum_w 2^(offset of w) S_w; window w is c - 1 bits wide below nlo
     zkh::Xyzz<HT> total = zkh::Xyzz<HT>::infinity();
     for (int w = g.W - 1; w >= 0; w--) {
-        for (int d = 0; d < g.c; d++) total = zkh::xyzz_dbl(total);
+        const int width = w < g.nlo ? g.c - 1 : g.c;
+        for (int d = 0; d < width; d++) total = zkh::xyzz_dbl(total);
         total = zkh::xyzz_add(total, S[w]);
     }
     return total;

@@ -32,8 +32,9 @@ namespace zk {
 constexpr int GLV_BITS = 128;  // signed half scalars: magnitude < 2^127
 
 struct MsmGeom {
-    int c;          // window bits
-    int W;          // windows = ceil(GLV_BITS / c)
+    int c;          // window bits: windows w < nlo have c - 1 bits, the others c
+    int W;          // windows = ceil(GLV_BITS / requested c); then c = ceil(GLV_BITS / W)
+    int nlo;        // W*c - GLV_BITS
     uint32_t K;     // buckets per window = 2^(c-1)
     uint32_t l1;    // buckets per level-1 reduction thread
     uint32_t nseg;  // level-1 segments per window = K / l1

@@ -18,6 +18,7 @@
 
 #include <fstream>
 #include <sstream>
+#include <vector>
 
 #include "prover.hpp"
 #include "synth.hpp"
@@ -134,11 +135,17 @@ void Context::autotune(const Circuit& circ, bool force, int reps, int chosen[2])
         ZK_HIP(hipMemcpy(d_sc.p, sc.data(), n * 32, hipMemcpyHostToDevice));
         MsmChoice best{msm_default_c(n), 1, 1};
         float best_ms = -1;
-        // candidate windows around the size: wide windows for large MSMs,
-        // narrow ones for small
+        // candidate windows around the size (wide for large MSMs, narrow for
+        // small), one per window count: widths with the same count give the
+        // same geometry (msm_geom)
         const int hi = std::min(TUNE_C_MAX, std::max(TUNE_C_MIN, cls - 2));
-        const int lo = std::max(TUNE_C_MIN, std::min(cls - 7, hi - 4));
+        const int lo = std::max(TUNE_C_MIN, std::min(cls - 8, hi - 5));
+        std::vector<int> cands;
         for (int c = lo; c <= hi; c++) {
+            const MsmGeom g = msm_geom(n, c, false);
+            if (cands.empty() || msm_geom(n, cands.back(), false).W != g.W) cands.push_back(g.c);
+        }
+        for (int c : cands) {
             float sort_ms, msm[4][MSM_SPLIT_MAX];
             time_window(*this, circ, kind, c, d_sc.p, n, reps, tb, sort_ms, msm);
             float total = sort_ms;
