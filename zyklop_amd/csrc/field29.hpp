@@ -353,16 +353,15 @@ ZK_DEV F29 f29_add(const F29& a, const F29& b) {
     return r;
 }
 
-// a: components normalized with value < 6p; b: components N. Karatsuba: three
-// products, c0 = a0b0 - a1b1, c1 = (a0+a1)(b0+b1) - a0b0 - a1b1; result N.
+// a: components normalized with value < 6p; b: components N. Each component
+// is a sum of two products with one reduction (f29_mulsum): c0 = a0*b0 +
+// a1*(-b1), c1 = a0*b1 + a1*b0; result N. Four products and two reductions
+// cost fewer instructions than Karatsuba's three products, three reductions
+// and the normalisations of its differences.
 ZK_DEV F29q f29q_mul(const F29q& a, const F29q& b) {
-    const F29 t0 = f29_mul(a.c0, b.c0);
-    const F29 t1 = f29_mul(a.c1, b.c1);
-    const F29 t2 = f29_mul(f29_add(a.c0, a.c1), f29_add(b.c0, b.c1));   // 12p x 4p < R'p
-    F29 c1;
-#pragma unroll
-    for (int i = 0; i < 9; i++) c1.l[i] = t2.l[i] + f29c::P8[i] - t0.l[i] - t1.l[i];
-    return {f29_reduce(f29_norm(f29_sub4(t0, t1))), f29_reduce(f29_norm(c1))};
+    const F29 x[2] = {a.c0, a.c1};
+    const F29 y0[2] = {b.c0, f29_neg(b.c1)}, y1[2] = {b.c1, b.c0};
+    return {f29_mulsum<2>(x, y0), f29_mulsum<2>(x, y1)};
 }
 
 // a: components N. (a0+a1)(a0-a1) + (2 a0 a1) u; result N.
