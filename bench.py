@@ -44,9 +44,10 @@ MADD_FPMUL = 10                         # XYZZ mixed add (madd-2008-s): 8M + 2S
 MADD_FPMUL_G2 = 30                      # the same in Fp2: 10 Fp2 products x 3 Fp products (Karatsuba)
 CALIB_BYTES = 384 << 20                 # device copy before the probes (profiler counter calibration)
 # HBM bytes per entry of the G1 accumulation kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE
-# runs of the probe (c = 17, 15,790,005 entries: 3.010 GB per launch), scaled by
-# the calibration copy; profiles/msm_accum_m2p20_hbm_counters.md
-HBM_BYTES_PER_ENTRY_G1 = 3.010e9 / 15790005
+# runs of the probe (c = 16 over the GLV halves, 16,842,520 entries: 3.068 GB per
+# launch), scaled by the calibration copy; profiles/msm_accum_m2p20_hbm_counters.md
+HBM_BYTES_PER_ENTRY_G1 = 3.068e9 / 16842520
+ALGO_BYTES_PER_ENTRY_G1 = 68            # y and x (or beta*x) of the base, 32 B each packed, + the 4 B entry
 
 
 def log(msg):
@@ -264,9 +265,9 @@ def section_probes(args, zk, ctx, circ):
         "achieved": round(achieved, 3), "peak": round(PEAK_MAC_TS, 2), "unit": "TMAC/s",
         "frac": round(achieved / PEAK_MAC_TS, 4), "traffic": int(units * HBM_BYTES_PER_ENTRY_G1),
         "traffic_note": ("HBM bytes per launch: per-entry figure from the profiler's FETCH_SIZE + WRITE_SIZE runs "
-                         "of this probe (calibrated by a 384 MiB copy) x entries; algorithmic bytes 84 per entry "
-                         "(80 B point + 4 B entry)"),
-        "algorithmic_bytes": int(units * 84),
+                         "of this probe (calibrated by a 384 MiB copy) x entries; algorithmic bytes 68 per entry "
+                         "(64 B of packed base coordinates + 4 B entry)"),
+        "algorithmic_bytes": int(units * ALGO_BYTES_PER_ENTRY_G1),
         "avg_launch_ms": round(t_launch * 1e3, 4), "units_per_launch": int(units),
         "unit_def": ("one signed-digit (point, bucket) entry = one XYZZ mixed add = 10 Fp products x 136 32x32 "
                      "MACs (the algorithmic count of a 256-bit Montgomery product, whatever the limb size)"),
