@@ -13,8 +13,8 @@ proofs of the same inputs):
                   (zk_prove_batch_device) and as one replayed HIP graph
                   (zk_batch_graph_*); `value` is the faster of the two.
       Kernel probes (alone on the device): the G1 and G2 witness MSMs over the
-      resident bases (roofline of the bucket-accumulation kernel) and NTTs of
-      2^22 (forward, inverse, coset).
+      resident bases (roofline of the bucket-accumulation kernel), NTTs of
+      2^22 (forward, inverse, coset) and a G1 MSM of 2^22 points.
   mixed: 64 proofs, m log-uniform in [2^14, 2^20], 1024 public inputs; half
       over 4 shared circuits (shared CRS), half over 32 circuits of their own;
       windows from the autotuner; plain vs one captured graph vs streams.
@@ -26,6 +26,7 @@ Prints one JSON line.
 """
 import argparse
 import concurrent.futures as cf
+import ctypes
 import json
 import math
 import os
@@ -290,7 +291,48 @@ def section_probes(args, zk, ctx, circ):
         "ms": {k: r3(v) for k, v in ntt.items()},
         "GBps": {k: round(v, 1) for k, v in gbs.items()},
     }
-    return roofline, roofline_ntt
+    msm22 = probe_msm_2p22(args, zk, ctx, circ)
+    return roofline, roofline_ntt, msm22
+
+
+def probe_msm_2p22(args, zk, ctx, circ):
+    """The G1 MSM of N = 2^22 points alone on the device: the circuit's resident
+    H, A' and B1' bases end to end (distinct points, already in HBM) with uniform
+    scalars. Priced by the implementation-independent Fp-mul count F_G1(N) x 136."""
+    N = 1 << 22
+    d_pts = ctx.dev_alloc(N * 64)
+    off = 0
+    for which in (4, 0, 1):
+        p, cnt = circ.bases(which)
+        take = min(cnt, N - off)
+        ctx.dev_copy(ctypes.c_void_p(d_pts.value + off * 64), p, take * 64)
+        off += take
+        if off == N:
+            break
+    assert off == N
+    sc = zk.synth_uniform(args.seed + 22, N)
+    d_sc = ctx.dev_alloc(sc.nbytes)
+    ctx.upload(d_sc, sc)
+    rows = []
+    for i in range(2 + args.probe_reps):
+        ctx.msm_g1_device(d_pts, d_sc, N, 0)
+        if i >= 2:
+            rows.append(ctx.last_timing())
+    ctx.dev_free(d_sc)
+    ctx.dev_free(d_pts)
+    t = np.array(rows).mean(axis=0)
+    macs = f_g1(N) * MACS_PER_FPMUL
+    achieved = macs / (t[0] / 1e3) / 1e12
+    log(f"[bench] G1 MSM 2^22 alone: {t[0]:.3f} ms (sort {t[1]:.3f}, accumulation {t[2]:.3f} ms, "
+        f"{int(t[3])} entries) -> {achieved:.2f} T MAC/s algorithmic")
+    return {
+        "n": N, "ms": r3(t[0]), "sort_ms": r3(t[1]), "accumulation_ms": r3(t[2]), "entries": int(t[3]),
+        "msm_per_s": round(1e3 / t[0], 2), "achieved": round(achieved, 3), "peak": round(PEAK_MAC_TS, 2),
+        "unit": "TMAC/s", "frac": round(achieved / PEAK_MAC_TS, 4),
+        "work_def": "F_G1(N) = min_c ceil(254/c)(11N + 16*2^c) Fp-mul equivalents x 136 MACs, device time "
+                    "(GLV split + sort + accumulation + reduction), host finish excluded",
+        "points": "resident H, A', B1' bases of the 2^20 circuit, end to end; uniform scalars",
+    }
 
 
 # ---------------------------------------------------------------- mixed batch
@@ -517,7 +559,7 @@ def main():
     ctx = zk.Context(0)
     ctx.set_batch_slots(args.slots)
     circ, tuned = setup_headline(args, zk, ctx)
-    roofline, roofline_ntt = section_probes(args, zk, ctx, circ)
+    roofline, roofline_ntt, msm22 = section_probes(args, zk, ctx, circ)
     m, n_pub, K = 1 << args.log_m, args.n_pub, args.steps
     result = {
         "metric": "Groth16 proofs/s + p50/p99 latency at 2^20 constraints, batched vs plain",
@@ -545,6 +587,7 @@ def main():
         result.update(head)
     result["roofline"] = roofline
     result["roofline_ntt"] = roofline_ntt
+    result["msm_g1_2p22"] = msm22
     circ.free()
     if "mixed" in sections:
         result["mixed_batch"] = section_mixed(args, zk, ctx)

@@ -22,7 +22,7 @@ template <> struct RedBlk<Fp> { static constexpr int V = 256; };
 template <> struct RedBlk<Fq2> { static constexpr int V = 128; };
 
 struct ChunkRec { uint32_t bucket, start, len, pad; };
-struct LargeRec { uint32_t slot, first, count, pad; };
+struct LargeRec { uint32_t bucket, first, count, pad; };
 
 int msm_default_c(size_t n) {
     int lg = ilog2(n ? n : 1);
@@ -245,6 +245,26 @@ __global__ void __launch_bounds__(1024) k_load_scatter(const uint32_t* __restric
     if (b < nb) order[lh[key] + rank] = b;
 }
 
+// Chunk lists of the oversized buckets (load key 0, first in the accumulation
+// order): one thread per oversized bucket, grid-stride; hist[0] is their count.
+__global__ void __launch_bounds__(256) k_chunk_list(const uint32_t* __restrict__ order, const uint32_t* __restrict__ off,
+                                                    const uint32_t* __restrict__ hist, uint32_t* __restrict__ counters,
+                                                    ChunkRec* __restrict__ chunks, LargeRec* __restrict__ large) {
+    const uint32_t n_over = hist[0];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_over; i += gridDim.x * blockDim.x) {
+        const uint32_t b = order[i];
+        const uint32_t s0 = off[b], e0 = off[b + 1];
+        const uint32_t nch = (e0 - s0 + CHUNK - 1) / CHUNK;
+        const uint32_t first = atomicAdd(&counters[0], nch);
+        for (uint32_t j = 0; j < nch; j++) {
+            const uint32_t c0 = s0 + j * CHUNK;
+            chunks[first + j] = {b, c0, min(CHUNK, e0 - c0), 0u};
+        }
+        const uint32_t li = atomicAdd(&counters[1], 1u);
+        large[li] = {b, first, nch, 0u};
+    }
+}
+
 void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
     ZK_REQUIRE(n_ < 0x40000000ull, ZK_E_ARG, "MSM size must be < 2^30");
     n = n_;
@@ -309,6 +329,16 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
     exclusive_scan(hist, cur, cur + nkey, nkey, st);
     k_load_scatter<<<nblk_o, 1024, nkey * 4, st>>>(off, (uint32_t)nb, lmax, qs, nkey, cur, order.as<uint32_t>());
     ZK_HIP(hipGetLastError());
+    // chunk lists of the oversized buckets, shared by every MSM over this sort
+    const size_t max_chunks = ne / CHUNK + ne / L_MIN_LARGE + 2, max_large = ne / L_MIN_LARGE + 2;
+    chunks.ensure(max_chunks * sizeof(ChunkRec));
+    large.ensure(max_large * sizeof(LargeRec));
+    counters.ensure(64);
+    n_chunk_max = max_chunks;
+    fill_u32(counters.as<uint32_t>(), 0u, 16, st);
+    k_chunk_list<<<64, 256, 0, st>>>(order.as<uint32_t>(), off, hist, counters.as<uint32_t>(), chunks.as<ChunkRec>(),
+                                     large.as<LargeRec>());
+    ZK_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------- buckets
@@ -321,15 +351,15 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
 // Thread tid of an accumulation launch over nb buckets, each split into
 // `split` parts of consecutive entries (bucket b's parts go to slots
 // b*split + h, summed by the level-1 reduction). Returns false when the thread
-// has nothing to accumulate: past the end, or a bucket above lmax entries,
-// which goes to the chunked path (part 0 lists its chunks, whose merged sum
-// lands in slot b*split; the other parts store infinity). Otherwise [st, en)
-// are its entries and slot its bucket slot.
+// has nothing to accumulate: past the end, or a bucket above lmax entries.
+// Such a bucket is summed by the chunked path before this launch
+// (k_msm_chunks, k_msm_merge into slot b*split); its other parts store
+// infinity here. Otherwise [st, en) are the thread's entries and slot its
+// bucket slot.
 template <class T>
 ZK_DEV bool acc_range(uint32_t tid, uint32_t nb, uint32_t split, const uint32_t* __restrict__ order,
-                      const uint32_t* __restrict__ offsets, uint32_t lmax, void* __restrict__ buckets,
-                      uint32_t* __restrict__ counters, ChunkRec* __restrict__ chunks, LargeRec* __restrict__ large,
-                      uint32_t& st, uint32_t& en, uint32_t& slot) {
+                      const uint32_t* __restrict__ offsets, uint32_t lmax, void* __restrict__ buckets, uint32_t& st,
+                      uint32_t& en, uint32_t& slot) {
     if (tid >= nb * split) return false;
     const uint32_t ob = tid / split, h = tid - ob * split;
     const uint32_t b = order[ob];
@@ -337,18 +367,7 @@ ZK_DEV bool acc_range(uint32_t tid, uint32_t nb, uint32_t split, const uint32_t*
     const uint32_t cnt = e0 - s0;
     slot = b * split + h;
     if (cnt > lmax) {
-        if (h == 0) {
-            const uint32_t nch = (cnt + CHUNK - 1) / CHUNK;
-            const uint32_t first = atomicAdd(&counters[0], nch);
-            for (uint32_t j = 0; j < nch; j++) {
-                const uint32_t c0 = s0 + j * CHUNK;
-                chunks[first + j] = {b, c0, min(CHUNK, e0 - c0), 0u};
-            }
-            const uint32_t li = atomicAdd(&counters[1], 1u);
-            large[li] = {slot, first, nch, 0u};
-        } else {
-            PointIO<T>::store_xyzz(buckets, slot, xyzz_inf<T>());
-        }
+        if (h != 0) PointIO<T>::store_xyzz(buckets, slot, xyzz_inf<T>());
         return false;
     }
     st = s0 + (uint32_t)(((uint64_t)cnt * h) / split);
@@ -361,12 +380,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                                                      const uint32_t* __restrict__ order, uint32_t nb, uint32_t split,
                                                      uint32_t lmax,
                                                      const void* __restrict__ points29, uint32_t nbase,
-                                                     void* __restrict__ buckets,
-                                                     uint32_t* __restrict__ counters, ChunkRec* __restrict__ chunks,
-                                                     LargeRec* __restrict__ large) {
+                                                     void* __restrict__ buckets) {
     uint32_t st, en, slot;
-    if (!acc_range<Fp>(blockIdx.x * blockDim.x + threadIdx.x, nb, split, order, offsets, lmax, buckets, counters, chunks,
-                       large, st, en, slot))
+    if (!acc_range<Fp>(blockIdx.x * blockDim.x + threadIdx.x, nb, split, order, offsets, lmax, buckets, st, en, slot))
         return;
     Xyzz29 acc;
     acc.inf = true;
@@ -393,14 +409,10 @@ __global__ void __launch_bounds__(ACC29Q_THREADS) __attribute__((amdgpu_waves_pe
                                                                  const uint32_t* __restrict__ order, uint32_t nb,
                                                                  uint32_t split, uint32_t lmax,
                                                                  const void* __restrict__ points29,
-                                                                 uint32_t nbase, void* __restrict__ buckets,
-                                                                 uint32_t* __restrict__ counters,
-                                                                 ChunkRec* __restrict__ chunks,
-                                                                 LargeRec* __restrict__ large) {
+                                                                 uint32_t nbase, void* __restrict__ buckets) {
     __shared__ uint32_t lds[AccLds<ACC29Q_THREADS>::LDS_WORDS];
     uint32_t st, en, slot;
-    if (!acc_range<Fq2>(blockIdx.x * blockDim.x + threadIdx.x, nb, split, order, offsets, lmax, buckets, counters, chunks,
-                       large, st, en, slot))
+    if (!acc_range<Fq2>(blockIdx.x * blockDim.x + threadIdx.x, nb, split, order, offsets, lmax, buckets, st, en, slot))
         return;
     AccLds<ACC29Q_THREADS> A{lds + threadIdx.x, {}};
     bool inf = true;
@@ -537,7 +549,8 @@ __global__ void __launch_bounds__(256) k_msm_chunks(const uint32_t* __restrict__
 // one workgroup per oversized bucket: sum its chunk partials
 template <class T>
 __global__ void __launch_bounds__(256) k_msm_merge(const uint32_t* __restrict__ counters, const LargeRec* __restrict__ large,
-                                                   const void* __restrict__ partials, void* __restrict__ buckets) {
+                                                   const void* __restrict__ partials, uint32_t split,
+                                                   void* __restrict__ buckets) {
     __shared__ Xyzz<T> sh[256];
     const uint32_t nl = counters[1];
     for (uint32_t li = blockIdx.x; li < nl; li += gridDim.x) {
@@ -546,7 +559,7 @@ __global__ void __launch_bounds__(256) k_msm_merge(const uint32_t* __restrict__ 
         for (uint32_t j = threadIdx.x; j < lr.count; j += blockDim.x)
             acc = xyzz_add(acc, PointIO<T>::load_xyzz(partials, lr.first + j));
         Xyzz<T> s = block_sum<T>(acc, sh);
-        if (threadIdx.x == 0) PointIO<T>::store_xyzz(buckets, lr.slot, s);
+        if (threadIdx.x == 0) PointIO<T>::store_xyzz(buckets, (size_t)lr.bucket * split, s);
     }
 }
 
@@ -660,7 +673,7 @@ MsmScratch::~MsmScratch() {
     if (ev_end) (void)hipEventDestroy(ev_end);
 }
 
-void MsmScratch::ensure(const MsmGeom& g, size_t n_entries, bool g2, int split) {
+void MsmScratch::ensure(const MsmGeom& g, size_t max_chunks, bool g2, int split) {
     size_t xb = g2 ? 256 : 128;
     size_t nb = (size_t)g.W * g.K;
     buckets.ensure(nb * split * xb);
@@ -668,44 +681,45 @@ void MsmScratch::ensure(const MsmGeom& g, size_t n_entries, bool g2, int split) 
     t1.ensure((size_t)g.W * g.nseg * xb);
     r2.ensure((size_t)g.W * g.nblk * xb);
     t2.ensure((size_t)g.W * g.nblk * xb);
-    size_t max_chunks = n_entries / CHUNK + n_entries / L_MIN_LARGE + 2;
-    size_t max_large = n_entries / L_MIN_LARGE + 2;
-    chunks.ensure(max_chunks * sizeof(ChunkRec));
-    large.ensure(max_large * sizeof(LargeRec));
     partials.ensure(max_chunks * xb);
-    counters.ensure(64);
+}
+
+// The chunked path of the oversized buckets (lists built by the sort): their
+// sums land in bucket slot b*split before the accumulation. Queued ahead of
+// the accumulation so that these short launches run while the GPU has room,
+// instead of waiting behind the large accumulation launches of other streams.
+template <class T>
+static void launch_pre_t(const MsmSort& sort, const void* d_points, MsmScratch& scr, hipStream_t st, int split) {
+    const bool g2 = std::is_same<T, Fq2>::value;
+    const MsmGeom g = msm_geom(sort.n, sort.g.c, g2);
+    scr.ensure(g, sort.n_chunk_max, g2, split);
+    scr.split = split;
+    k_msm_chunks<T><<<512, 256, 0, st>>>(sort.entries.as<uint32_t>(), d_points, (uint32_t)sort.n,
+                                          sort.counters.as<uint32_t>(), sort.chunks.as<ChunkRec>(), scr.partials.p);
+    ZK_HIP(hipGetLastError());
+    k_msm_merge<T><<<256, 256, 0, st>>>(sort.counters.as<uint32_t>(), sort.large.as<LargeRec>(), scr.partials.p,
+                                         (uint32_t)split, scr.buckets.p);
+    ZK_HIP(hipGetLastError());
 }
 
 template <class T>
-static void launch_t(const MsmSort& sort, const void* d_points, const void* d_points29, MsmScratch& scr,
-                     MsmPartials& out, hipStream_t st, int split) {
+static void launch_main_t(const MsmSort& sort, const void* d_points29, MsmScratch& scr, MsmPartials& out,
+                          hipStream_t st) {
     const bool g2 = std::is_same<T, Fq2>::value;
     const MsmGeom g = msm_geom(sort.n, sort.g.c, g2);
-    scr.ensure(g, (size_t)g.W * 2 * sort.n, g2, split);
+    const int split = scr.split;
     const uint32_t nb = (uint32_t)((size_t)g.W * g.K), nt = nb * (uint32_t)split;
-    fill_u32(scr.counters.p, 0u, 16, st);
     if (scr.ev_acc0) ZK_HIP(hipEventRecord(scr.ev_acc0, st));
     if (g2)
         k_msm_accum29q<<<cdiv(nt, ACC29Q_THREADS), ACC29Q_THREADS, 0, st>>>(
             sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(), sort.order.as<uint32_t>(), nb, (uint32_t)split,
-            sort.lmax,
-            d_points29, (uint32_t)sort.n, scr.buckets.p, scr.counters.as<uint32_t>(), scr.chunks.as<ChunkRec>(),
-            scr.large.as<LargeRec>());
+            sort.lmax, d_points29, (uint32_t)sort.n, scr.buckets.p);
     else
         k_msm_accum29<<<cdiv(nt, 128), 128, 0, st>>>(sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(),
                                                      sort.order.as<uint32_t>(), nb, (uint32_t)split, sort.lmax,
-                                                     d_points29,
-                                                     (uint32_t)sort.n, scr.buckets.p,
-                                                     scr.counters.as<uint32_t>(), scr.chunks.as<ChunkRec>(),
-                                                     scr.large.as<LargeRec>());
+                                                     d_points29, (uint32_t)sort.n, scr.buckets.p);
     ZK_HIP(hipGetLastError());
     if (scr.ev_acc1) ZK_HIP(hipEventRecord(scr.ev_acc1, st));
-    k_msm_chunks<T><<<512, 256, 0, st>>>(sort.entries.as<uint32_t>(), d_points, (uint32_t)sort.n,
-                                          scr.counters.as<uint32_t>(), scr.chunks.as<ChunkRec>(), scr.partials.p);
-    ZK_HIP(hipGetLastError());
-    k_msm_merge<T><<<256, 256, 0, st>>>(scr.counters.as<uint32_t>(), scr.large.as<LargeRec>(), scr.partials.p,
-                                         scr.buckets.p);
-    ZK_HIP(hipGetLastError());
     uint32_t nseg_all = (uint32_t)g.W * g.nseg;
     k_msm_reduce1<T><<<cdiv(nseg_all, 128), 128, 0, st>>>(scr.buckets.p, g.K, g.W, g.l1, (uint32_t)split, scr.r1.p,
                                                            scr.t1.p);
@@ -724,12 +738,24 @@ static void launch_t(const MsmSort& sort, const void* d_points, const void* d_po
     if (scr.ev_end) ZK_HIP(hipEventRecord(scr.ev_end, st));
 }
 
+void msm_launch_pre(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, hipStream_t st, int split) {
+    ZK_REQUIRE(split >= 1 && split <= MSM_SPLIT_MAX, ZK_E_ARG, "bucket split must be in [1, 4]");
+    if (g2) launch_pre_t<Fq2>(sort, d_points, scr, st, split);
+    else launch_pre_t<Fp>(sort, d_points, scr, st, split);
+}
+
+void msm_launch_main(const MsmSort& sort, bool g2, MsmScratch& scr, MsmPartials& out, hipStream_t st,
+                     const void* d_points29) {
+    ZK_REQUIRE(sort.n == 0 || d_points29, ZK_E_INTERNAL, "MSM launch without the 29-bit copy of its points");
+    ZK_REQUIRE(scr.split >= 1, ZK_E_INTERNAL, "MSM accumulation queued before its chunked path");
+    if (g2) launch_main_t<Fq2>(sort, d_points29, scr, out, st);
+    else launch_main_t<Fp>(sort, d_points29, scr, out, st);
+}
+
 void msm_launch(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, MsmPartials& out, hipStream_t st,
                 const void* d_points29, int split) {
-    ZK_REQUIRE(sort.n == 0 || d_points29, ZK_E_INTERNAL, "MSM launch without the 29-bit copy of its points");
-    ZK_REQUIRE(split >= 1 && split <= MSM_SPLIT_MAX, ZK_E_ARG, "bucket split must be in [1, 4]");
-    if (g2) launch_t<Fq2>(sort, d_points, d_points29, scr, out, st, split);
-    else launch_t<Fp>(sort, d_points, d_points29, scr, out, st, split);
+    msm_launch_pre(sort, d_points, g2, scr, st, split);
+    msm_launch_main(sort, g2, scr, out, st, d_points29);
 }
 
 // ---------------------------------------------------------------- host finish

@@ -55,6 +55,9 @@ struct MsmSort {
     uint32_t lmax = 0, nkey = 0;
     int qs = 0;
     DevBuf half, matrix, mscan, pairs, offsets, entries, order, lkeys;
+    // chunk lists of the oversized buckets (above lmax entries), built by run()
+    DevBuf chunks, large, counters;
+    size_t n_chunk_max = 0;
     // sorts canonical scalars (n x 32 B, device, n < 2^30) into bucket order;
     // an entry is the index i < 2n of a base (msm_points_to29 order) with the
     // digit's sign in bit 31
@@ -70,13 +73,14 @@ struct MsmPartials {
 
 // Device scratch for the bucket phase of one MSM.
 struct MsmScratch {
-    DevBuf buckets, r1, t1, r2, t2, chunks, large, partials, counters;
+    DevBuf buckets, r1, t1, r2, t2, partials;
+    int split = 0;     // threads per bucket of the queued MSM (set by msm_launch_pre)
     size_t cap_buckets = 0;
     // optional events around the bucket-accumulation kernel (profiling)
     hipEvent_t ev_acc0 = nullptr, ev_acc1 = nullptr, ev_end = nullptr;
     void enable_timing();
     ~MsmScratch();
-    void ensure(const MsmGeom& g, size_t n_entries, bool g2, int split);
+    void ensure(const MsmGeom& g, size_t max_chunks, bool g2, int split);
 };
 
 // Launch bucket accumulation + GPU reduction for points (affine, Montgomery,
@@ -93,6 +97,13 @@ constexpr int MSM_SPLIT_MAX = 4;
 int msm_default_split(size_t n, int c, bool g2);
 void msm_launch(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, MsmPartials& out,
                 hipStream_t st, const void* d_points29, int split = 1);
+// msm_launch in two parts: the chunked path of the oversized buckets, then
+// the accumulation, reductions and the copy of the partials. Queueing every
+// MSM's first part before any accumulation keeps those short launches from
+// waiting behind the accumulations of other streams.
+void msm_launch_pre(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, hipStream_t st, int split);
+void msm_launch_main(const MsmSort& sort, bool g2, MsmScratch& scr, MsmPartials& out, hipStream_t st,
+                     const void* d_points29);
 
 // n affine points (standard form, G1 64 B / G2 128 B each) -> n slots of the
 // 29-bit-limb form (G1 128 B, G2 256 B; layout in field29.hpp), each holding

@@ -269,11 +269,10 @@ ProveSlot::~ProveSlot() {
 size_t ProveSlot::device_bytes() const {
     size_t b = scal.bytes + aw.bytes + bw.bytes + cw.bytes + t0.bytes + t1.bytes + t2.bytes + h.bytes;
     for (const MsmSort* q : {&sortW, &sortH})
-        b += q->matrix.bytes + q->mscan.bytes + q->pairs.bytes + q->offsets.bytes + q->entries.bytes + q->order.bytes +
-             q->lkeys.bytes;
+        b += q->half.bytes + q->matrix.bytes + q->mscan.bytes + q->pairs.bytes + q->offsets.bytes + q->entries.bytes +
+             q->order.bytes + q->lkeys.bytes + q->chunks.bytes + q->large.bytes + q->counters.bytes;
     for (const MsmScratch* s : {&scrA, &scrB1, &scrC, &scrB2, &scrH})
-        b += s->buckets.bytes + s->r1.bytes + s->t1.bytes + s->r2.bytes + s->t2.bytes + s->chunks.bytes + s->large.bytes +
-             s->partials.bytes;
+        b += s->buckets.bytes + s->r1.bytes + s->t1.bytes + s->r2.bytes + s->t2.bytes + s->partials.bytes;
     return b;
 }
 
@@ -338,10 +337,14 @@ void ProveSlot::body(Context& ctx, const Circuit& c, ProofParts& out) {
     ZK_HIP(hipEventRecord(ev_done1, s1));
     if (timed) ZK_HIP(hipEventRecord(ev[5], s1));
     for (hipStream_t p : {s2, s3, s4}) ZK_HIP(hipStreamWaitEvent(p, ev_done1, 0));
-    msm_launch(sortW, c.B2.p, true, scrB2, out.b2, s4, c.B2_29.p, sp_w2);
-    msm_launch(sortW, c.A.p, false, scrA, out.a, s1, c.A29.p, sp_w1);
-    msm_launch(sortW, c.B1.p, false, scrB1, out.b1, s2, c.B1_29.p, sp_w1);
-    msm_launch(sortW, c.C.p, false, scrC, out.c, s3, c.C29.p, sp_w1);
+    msm_launch_pre(sortW, c.B2.p, true, scrB2, s4, sp_w2);
+    msm_launch_pre(sortW, c.A.p, false, scrA, s1, sp_w1);
+    msm_launch_pre(sortW, c.B1.p, false, scrB1, s2, sp_w1);
+    msm_launch_pre(sortW, c.C.p, false, scrC, s3, sp_w1);
+    msm_launch_main(sortW, true, scrB2, out.b2, s4, c.B2_29.p);
+    msm_launch_main(sortW, false, scrA, out.a, s1, c.A29.p);
+    msm_launch_main(sortW, false, scrB1, out.b1, s2, c.B1_29.p);
+    msm_launch_main(sortW, false, scrC, out.c, s3, c.C29.p);
     // H pipeline on s0
     spmv_abc(c.rowA.as<uint32_t>(), c.colA.as<uint32_t>(), c.valA.p, c.n_rowsA, c.rowB.as<uint32_t>(),
              c.colB.as<uint32_t>(), c.valB.p, c.n_rowsB, scal.p, n, aw.p, bw.p, cw.p, s0);

@@ -116,6 +116,8 @@ void time_window(Context& ctx, const Circuit& circ, int kind, int c, const void*
 
 }  // namespace
 
+constexpr float SPLIT_MARGIN = 0.97f;
+
 void Context::autotune(const Circuit& circ, bool force, int reps, int chosen[2]) {
     ZK_HIP(hipSetDevice(device));
     const size_t sizes[2] = {(size_t)circ.n_vars + 4, (size_t)circ.n};
@@ -150,9 +152,12 @@ void Context::autotune(const Circuit& circ, bool force, int reps, int chosen[2])
             time_window(*this, circ, kind, c, d_sc.p, n, reps, tb, sort_ms, msm);
             float total = sort_ms;
             int sp[4] = {1, 1, 1, 1};
+            // a larger split adds level-1 work (one addition per extra part
+            // and bucket) that a GPU shared with other proofs pays in full:
+            // it must beat the smaller split alone by SPLIT_MARGIN
             for (int m = 0; m < (kind == 0 ? 4 : 1); m++) {
                 for (int s = 2; s <= MSM_SPLIT_MAX; s++)
-                    if (msm[m][s - 1] >= 0 && msm[m][s - 1] < msm[m][sp[m] - 1]) sp[m] = s;
+                    if (msm[m][s - 1] >= 0 && msm[m][s - 1] < SPLIT_MARGIN * msm[m][sp[m] - 1]) sp[m] = s;
                 total += msm[m][sp[m] - 1];
             }
             if (best_ms < 0 || total < best_ms) {
