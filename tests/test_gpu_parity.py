@@ -79,7 +79,8 @@ def test_msm_g2_golden(ctx, ref_vectors):
             assert np.array_equal(out, exp), v["label"]
 
 
-@pytest.mark.parametrize("n,window", [(1, 0), (2, 4), (100, 0), (1000, 8), (3000, 0), (3000, 13)])
+@pytest.mark.parametrize("n,window", [(1, 0), (2, 4), (100, 0), (1000, 8), (3000, 0), (3000, 13), (2000, 9), (2000, 19),
+                                      (700, 22)])
 def test_msm_g1_random_vs_oracle(ctx, n, window):
     rng = np.random.default_rng(n + window)
     base = [int(x) for x in rng.integers(1, 2**62, size=min(n, 700))]
@@ -158,14 +159,14 @@ def test_msm_rejects_noncanonical_scalar(ctx):
     assert e.value.code == 1
 
 
-@pytest.mark.parametrize("n", [1, 40, 400])
-def test_msm_g2_random_vs_oracle(ctx, n):
+@pytest.mark.parametrize("n,window", [(1, 0), (40, 0), (400, 0), (400, 9), (400, 13)])
+def test_msm_g2_random_vs_oracle(ctx, n, window):
     rng = np.random.default_rng(100 + n)
     base = [int(x) for x in rng.integers(1, 2**62, size=min(n, 60))]
     ks = [base[i % len(base)] for i in range(n)]
     pts = mont_pts_g2(ks)
     sc = O.arr([int.from_bytes(rng.bytes(32), "little") % pyref.R for _ in range(n)])
-    out, inf = ctx.msm_g2(pts, sc)
+    out, inf = ctx.msm_g2(pts, sc, window)
     exp, einf = oracle_msm_g2(pts, sc)
     assert inf == einf and np.array_equal(out, exp)
 
