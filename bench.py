@@ -117,6 +117,17 @@ def upload_all(ctx, arrays):
 
 
 # ---------------------------------------------------------------- headline section
+def tuned_table(ctx):
+    """The context's tuning table as rows (kind, ceil log2 n, c, G1 split, G2 split), via zk_tune_save."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "windows.txt")
+        ctx.tune_save(path)
+        rows = [ln.split() for ln in open(path) if ln.strip() and not ln.startswith("#")]
+    return [{"kind": "witness" if r[1] == "0" else "h", "log2_n": int(r[2]), "c": int(r[3]),
+             "split_g1": int(r[4]), "split_g2": int(r[5])} for r in rows]
+
+
 def setup_headline(args, zk, ctx):
     m, n_pub = 1 << args.log_m, args.n_pub
     t0 = time.time()
@@ -203,6 +214,7 @@ def section_headline(args, zk, ctx, circ, tuned):
         "speedup_batched_vs_plain": round(batch_ps / plain_ps, 4),
         "phases_ms": {k: round(v, 4) for k, v in mean.items() if k.endswith("_ms")},
         "tuned_windows": tuned,
+        "tuned_table": tuned_table(ctx),
     }
 
 

@@ -91,12 +91,14 @@ int zk_set_windows(zk_ctx* ctx, int window_witness, int window_h);
 /* MSM window autotuner (no reference counterpart: the reference's window is a
  * compile-time constant, include/msm_g1.cuh:11). For circuit c it measures on
  * the device, over c's resident bases and uniform scalars, the digit sort plus
- * the MSMs that share it for every window in [11, 18] (clipped to at most
- * ceil(log2 n) + 2): kind 0 = the witness sort over n_vars + 4 scalars and the
- * A', B1', C' (G1) and B2' (G2) MSMs; kind 1 = the sort of h and the H MSM
- * (n scalars). The fastest window is recorded per (kind, ceil log2 n) in the
- * context's table, which every later proof on ctx uses unless zk_set_windows
- * forces one. force = 0 keeps size classes already in the table.
+ * the MSMs that share it, for one window per window count in [11, 19] around
+ * the size (at most ceil(log2 n) - 2) and every bucket split 1..4: kind 0 =
+ * the witness sort over n_vars + 4 scalars and the A', B1', C' (G1) and B2'
+ * (G2) MSMs; kind 1 = the sort of h and the H MSM (n scalars). The fastest
+ * window, with its G1 and G2 splits, is recorded per (kind, ceil log2 n) in
+ * the context's table, which every later proof on ctx uses unless
+ * zk_set_windows forces one. force = 0 keeps size classes already in the
+ * table.
  * windows (may be NULL) receives the (witness, h) windows now in effect. */
 int zk_autotune(zk_ctx* ctx, const zk_circuit* c, int force, int windows[2]);
 /* window a proof on ctx would use for an MSM of kind (0 witness, 1 h) and size n */

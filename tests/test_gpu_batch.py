@@ -99,7 +99,7 @@ def test_autotune_table(tmp_path):
     n_w = c.n_vars + 4
     before = ctx.prove(c, w, r, s).words()
     cw, ch = ctx.autotune(c)
-    assert 11 <= cw <= 18 and 11 <= ch <= 18
+    assert 11 <= cw <= 19 and 11 <= ch <= 19
     assert (ctx.tuned_window(0, n_w), ctx.tuned_window(1, c.n)) == (cw, ch)
     assert ctx.autotune(c) == (cw, ch)        # cached size classes are kept
     after = ctx.prove(c, w, r, s).words()

@@ -24,11 +24,13 @@ template <> struct RedBlk<Fq2> { static constexpr int V = 128; };
 struct ChunkRec { uint32_t bucket, start, len, pad; };
 struct LargeRec { uint32_t bucket, first, count, pad; };
 
+// window for an untuned size: the 2n GLV halves of n scalars favour about
+// log2(n) - 3 bits (2^22 points: c = 19, 7 windows, measured 7% faster than 8)
 int msm_default_c(size_t n) {
     int lg = ilog2(n ? n : 1);
-    int c = lg - 4;
+    int c = lg - 3;
     if (c < 4) c = 4;
-    if (c > 18) c = 18;
+    if (c > 19) c = 19;
     return c;
 }
 

@@ -107,7 +107,7 @@ struct ProveSlot {
     size_t device_bytes() const;
 };
 
-constexpr int TUNE_C_MIN = 11, TUNE_C_MAX = 18;  // autotuner window range
+constexpr int TUNE_C_MIN = 11, TUNE_C_MAX = 19;  // autotuner window range (c = 19: 7 windows)
 int size_class(size_t n);                         // ceil(log2 n)
 
 struct Context {
