@@ -400,7 +400,7 @@ def section_mixed(args, zk, ctx):
         graph_note = str(e)
     # streams
     ctx.set_batch_slots(args.mixed_slots)
-    ctx.prove_batch_device(plan[:4], d_ws[:4], R[:4], S[:4])
+    ctx.prove_batch_device(plan, d_ws, R[::-1], S[::-1])     # warm pass: slots sized for every circuit
     t0 = time.perf_counter()
     out = ctx.prove_batch_device(plan, d_ws, R, S)
     streams_s = time.perf_counter() - t0

@@ -2,6 +2,7 @@
 // status codes plus a per-thread message.
 #include <string.h>
 
+#include <set>
 #include <string>
 
 #include "../../include/zk_abi.h"
@@ -95,6 +96,13 @@ void run_batch(zk_ctx* ctx, int count, const zk_circuit* const* circuits, const 
     for (int i = 0; i < count; i++) need(circuits[i] && witnesses[i], "NULL circuit or witness");
     int next = 0;
     const int inflight = std::min(count, nslots);
+    // every slot sized for every circuit of the batch before any proof is
+    // queued: a buffer that grew mid-batch would free and allocate device
+    // memory, which waits for the work of all the other slots
+    std::set<const zk::Circuit*> distinct;
+    for (int i = 0; i < count; i++) distinct.insert(circuits[i]->impl.get());
+    for (int k = 0; k < inflight; k++)
+        for (const zk::Circuit* c : distinct) ctx->batch_slots[k]->reserve(ctx->impl, *c, !on_dev);
     for (; next < inflight; next++)
         ctx->batch_slots[next % nslots]->enqueue(ctx->impl, *circuits[next]->impl, witnesses[next], on_dev,
                                                  r ? r + 4 * next : nullptr, s ? s + 4 * next : nullptr);

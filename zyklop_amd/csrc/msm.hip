@@ -267,34 +267,79 @@ __global__ void __launch_bounds__(256) k_chunk_list(const uint32_t* __restrict__
     }
 }
 
-void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
-    ZK_REQUIRE(n_ < 0x40000000ull, ZK_E_ARG, "MSM size must be < 2^30");
-    n = n_;
-    g = msm_geom(n, c, false);
+// sizes of one digit sort over n scalars at window c (shared by reserve and run)
+struct SortShape {
+    MsmGeom g;
+    size_t n2, nb, tile, mlen, ne, max_chunks, max_large;
+    uint32_t G, nt, lmax, nkey;
+    int sh, qs;
+};
+
+static SortShape sort_shape(size_t n, int c) {
+    ZK_REQUIRE(n < 0x40000000ull, ZK_E_ARG, "MSM size must be < 2^30");
+    SortShape z;
+    z.g = msm_geom(n, c, false);
     // the sort runs over the 2n half scalars of the GLV split
-    const size_t n2 = 2 * n;
-    half.ensure(std::max<size_t>(1, n2) * 16);
+    z.n2 = 2 * n;
+    z.nb = (size_t)z.g.W * z.g.K;
+    // group size 2^s: at most 4096 groups (LDS histograms of pass A)
+    z.sh = 7;
+    while (((z.nb + (1ull << z.sh) - 1) >> z.sh) > 4096) z.sh++;
+    z.G = (uint32_t)((z.nb + (1ull << z.sh) - 1) >> z.sh);
+    // tiles of pass A: multiples of 4096 scalars (long runs per group for the
+    // pair writes), at most 2048 tiles
+    z.tile = (size_t)4096 * std::max<size_t>(1, (z.n2 + 4096 * 2048 - 1) / (4096 * 2048));
+    z.nt = (uint32_t)std::max<size_t>(1, (z.n2 + z.tile - 1) / z.tile);
+    z.mlen = (size_t)z.G * z.nt + 1;
+    z.ne = std::max<size_t>(1, (size_t)z.g.W * z.n2);
+    // accumulation order. Oversized threshold: 8x the mean bucket load
+    // (uniform scalars put about 2n/K entries in a bucket).
+    z.lmax = std::max<uint32_t>(L_MIN_LARGE, 8u * (uint32_t)((z.n2 + z.g.K - 1) / z.g.K));
+    z.qs = 0;
+    while ((z.lmax >> z.qs) + 2 > 4096) z.qs++;
+    z.nkey = (z.lmax >> z.qs) + 2;
+    z.max_chunks = z.ne / CHUNK + z.ne / L_MIN_LARGE + 2;
+    z.max_large = z.ne / L_MIN_LARGE + 2;
+    return z;
+}
+
+static void sort_reserve(MsmSort& s, const SortShape& z) {
+    s.half.ensure(std::max<size_t>(1, z.n2) * 16);
+    s.matrix.ensure(z.mlen * 4);
+    s.mscan.ensure(z.mlen * 4 + (size_t)4 * (cdiv(z.mlen, SCAN_TILE) + 1));
+    s.pairs.ensure(z.ne * 8);
+    s.entries.ensure(z.ne * 4);
+    s.offsets.ensure((z.nb + 1) * 4);
+    s.order.ensure(z.nb * 4);
+    s.lkeys.ensure((size_t)4 * (2 * z.nkey + cdiv(z.nkey, SCAN_TILE) + 1));
+    s.chunks.ensure(z.max_chunks * sizeof(ChunkRec));
+    s.large.ensure(z.max_large * sizeof(LargeRec));
+    s.counters.ensure(64);
+}
+
+size_t MsmSort::reserve(size_t n_, int c) {
+    const SortShape z = sort_shape(n_, c);
+    sort_reserve(*this, z);
+    return z.max_chunks;
+}
+
+void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
+    const SortShape z = sort_shape(n_, c);
+    sort_reserve(*this, z);
+    n = n_;
+    g = z.g;
+    lmax = z.lmax;
+    qs = z.qs;
+    nkey = z.nkey;
+    n_chunk_max = z.max_chunks;
+    const size_t n2 = z.n2, nb = z.nb, tile = z.tile, mlen = z.mlen;
+    const uint32_t G = z.G, nt = z.nt;
+    const int sh = z.sh;
     if (n) {
         k_glv_split<<<cdiv(n, 256), 256, 0, st>>>((const uint32_t*)d_scalars, n, half.as<uint4>());
         ZK_HIP(hipGetLastError());
     }
     const uint32_t* hs = half.as<uint32_t>();
-    size_t nb = (size_t)g.W * g.K;
-    // group size 2^s: at most 4096 groups (LDS histograms of pass A)
-    int sh = 7;
-    while (((nb + (1ull << sh) - 1) >> sh) > 4096) sh++;
-    const uint32_t G = (uint32_t)((nb + (1ull << sh) - 1) >> sh);
-    // tiles of pass A: multiples of 4096 scalars (long runs per group for the
-    // pair writes), at most 2048 tiles
-    const size_t tile = (size_t)4096 * std::max<size_t>(1, (n2 + 4096 * 2048 - 1) / (4096 * 2048));
-    const uint32_t nt = (uint32_t)std::max<size_t>(1, (n2 + tile - 1) / tile);
-    const size_t mlen = (size_t)G * nt + 1;
-    const size_t ne = std::max<size_t>(1, (size_t)g.W * n2);
-    matrix.ensure(mlen * 4);
-    mscan.ensure(mlen * 4 + (size_t)4 * (cdiv(mlen, SCAN_TILE) + 1));
-    pairs.ensure(ne * 8);
-    entries.ensure(ne * 4);
-    offsets.ensure((nb + 1) * 4);
     uint32_t* mat = matrix.as<uint32_t>();
     uint32_t* msc = mscan.as<uint32_t>();
     if (n) {
@@ -314,14 +359,6 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
                                              entries.as<uint32_t>());
     ZK_HIP(hipGetLastError());
     const uint32_t* off = offsets.as<uint32_t>();
-    // accumulation order. Oversized threshold: 8x the mean bucket load
-    // (uniform scalars put about 2n/K entries in a bucket).
-    lmax = std::max<uint32_t>(L_MIN_LARGE, 8u * (uint32_t)((n2 + g.K - 1) / g.K));
-    qs = 0;
-    while ((lmax >> qs) + 2 > 4096) qs++;
-    nkey = (lmax >> qs) + 2;
-    order.ensure(nb * 4);
-    lkeys.ensure((size_t)4 * (2 * nkey + cdiv(nkey, SCAN_TILE) + 1));
     uint32_t* hist = lkeys.as<uint32_t>();
     uint32_t* cur = hist + nkey;
     fill_u32(hist, 0u, nkey, st);
@@ -332,15 +369,14 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
     k_load_scatter<<<nblk_o, 1024, nkey * 4, st>>>(off, (uint32_t)nb, lmax, qs, nkey, cur, order.as<uint32_t>());
     ZK_HIP(hipGetLastError());
     // chunk lists of the oversized buckets, shared by every MSM over this sort
-    const size_t max_chunks = ne / CHUNK + ne / L_MIN_LARGE + 2, max_large = ne / L_MIN_LARGE + 2;
-    chunks.ensure(max_chunks * sizeof(ChunkRec));
-    large.ensure(max_large * sizeof(LargeRec));
-    counters.ensure(64);
-    n_chunk_max = max_chunks;
     fill_u32(counters.as<uint32_t>(), 0u, 16, st);
     k_chunk_list<<<64, 256, 0, st>>>(order.as<uint32_t>(), off, hist, counters.as<uint32_t>(), chunks.as<ChunkRec>(),
                                      large.as<LargeRec>());
     ZK_HIP(hipGetLastError());
+}
+
+void msm_reserve_partials(MsmPartials& out, const MsmGeom& g, bool g2) {
+    out.host.ensure(2 * (size_t)g.W * g.nblk * (g2 ? 256 : 128));
 }
 
 // ---------------------------------------------------------------- buckets

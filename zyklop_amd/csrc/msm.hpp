@@ -58,6 +58,9 @@ struct MsmSort {
     // chunk lists of the oversized buckets (above lmax entries), built by run()
     DevBuf chunks, large, counters;
     size_t n_chunk_max = 0;
+    // sizes every buffer a sort of n scalars at window c needs (no launch);
+    // returns the chunk-list capacity for MsmScratch::ensure
+    size_t reserve(size_t n, int c);
     // sorts canonical scalars (n x 32 B, device, n < 2^30) into bucket order;
     // an entry is the index i < 2n of a base (msm_points_to29 order) with the
     // digit's sign in bit 31
@@ -110,6 +113,9 @@ void msm_launch_main(const MsmSort& sort, bool g2, MsmScratch& scr, MsmPartials&
 // P_i and the x of lambda*P_i: the 2n bases of the GLV split.
 constexpr size_t PT29_BYTES = 128, PT29Q_BYTES = 256;
 void msm_points_to29(const void* d_points, size_t n, void* d_out, bool g2, hipStream_t st);
+
+// size out's host buffer for an MSM of geometry g (no launch)
+void msm_reserve_partials(MsmPartials& out, const MsmGeom& g, bool g2);
 
 // Host finish: combine the partials into the MSM value.
 zkh::Xyzz<zkh::Fp> msm_finish_g1(const MsmPartials& p);

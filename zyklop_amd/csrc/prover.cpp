@@ -295,6 +295,22 @@ void ProveSlot::prepare(Context& ctx, const Circuit& c) {
     for (DevBuf* b : {&aw, &bw, &cw, &t0, &t1, &t2, &h}) b->ensure(n * 32);
 }
 
+void ProveSlot::reserve(Context& ctx, const Circuit& c, bool host_witness) {
+    const size_t nS = (size_t)c.n_vars + 4, n = c.n;
+    prepare(ctx, c);
+    if (host_witness) staging.ensure((size_t)c.n_vars * 32);
+    const int cw = ctx.window_for(0, nS), ch = ctx.window_for(1, n);
+    const size_t kw = sortW.reserve(nS, cw), kh = sortH.reserve(n, ch);
+    const int s1 = ctx.split_for(0, nS, false), s2 = ctx.split_for(0, nS, true), sh = ctx.split_for(1, n, false);
+    const MsmGeom gw1 = msm_geom(nS, cw, false), gw2 = msm_geom(nS, cw, true), gh = msm_geom(n, ch, false);
+    for (MsmScratch* q : {&scrA, &scrB1, &scrC}) q->ensure(gw1, kw, false, s1);
+    scrB2.ensure(gw2, kw, true, s2);
+    scrH.ensure(gh, kh, false, sh);
+    for (MsmPartials* q : {&parts.a, &parts.b1, &parts.c}) msm_reserve_partials(*q, gw1, false);
+    msm_reserve_partials(parts.b2, gw2, true);
+    msm_reserve_partials(parts.h, gh, false);
+}
+
 void ProveSlot::enqueue(Context& ctx, const Circuit& c, const uint64_t* witness, bool on_dev, const uint64_t r_[4],
                         const uint64_t s_[4]) {
     circ = &c;

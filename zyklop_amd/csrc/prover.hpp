@@ -92,6 +92,11 @@ struct ProveSlot {
                  const uint64_t s[4]);
     // size every device buffer a proof of c needs in this slot
     void prepare(Context& ctx, const Circuit& c);
+    // size every buffer a proof of c needs in this slot, sorts, MSM scratch
+    // and host partials included, without queueing work; a batch reserves its
+    // slots for all of its circuits first, so that no buffer grows (and no
+    // allocation waits for the device) while proofs are in flight
+    void reserve(Context& ctx, const Circuit& c, bool host_witness);
     // the device work of one proof once scal holds [w | 1, r, s, -rs] (queued
     // on s0 before the call): everything up to the partials in out, ending
     // with s1..s4 joined into s0. Stream-capturable after prepare().
