@@ -46,9 +46,9 @@ MADD_FPMUL_G2 = 30                      # the same in Fp2: 10 Fp2 products x 3 F
 CALIB_BYTES = 384 << 20                 # device copy before the probes (profiler counter calibration)
 # HBM bytes per entry of the G1 accumulation kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE
 # runs of the probe (8 windows of 16 bits over the GLV halves, 16,842,520 entries:
-# 3.031 GB per launch), scaled by the calibration copy;
+# 3.029 GB per launch), scaled by the calibration copy;
 # profiles/msm_accum_m2p20_hbm_counters.md
-HBM_BYTES_PER_ENTRY_G1 = 3.031e9 / 16842520
+HBM_BYTES_PER_ENTRY_G1 = 3.029e9 / 16842520
 ALGO_BYTES_PER_ENTRY_G1 = 68            # y and x (or beta*x) of the base, 32 B each packed, + the 4 B entry
 
 
