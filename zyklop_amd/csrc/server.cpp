@@ -131,8 +131,10 @@ void Server::stats(double out[8]) {
 }
 
 // Admit queued requests into slots, oldest first. A free slot whose workspace
-// already fits is used as is; otherwise a free slot may grow, or a new slot be
-// created, only if free HBM stays above the reserve afterwards.
+// already fits is used as is; otherwise a new slot is created, or else a free
+// slot grows, only if free HBM stays above the reserve afterwards. A new slot
+// comes first: growing frees device memory, which waits for the proofs the
+// other slots have in flight.
 bool Server::try_admit(std::unique_lock<std::mutex>& lk) {
     bool any = false;
     while (!queue_.empty()) {
@@ -144,14 +146,14 @@ bool Server::try_admit(std::unique_lock<std::mutex>& lk) {
         if (pick < 0) {
             size_t free_b = 0, total_b = 0;
             ZK_HIP(hipMemGetInfo(&free_b, &total_b));
-            for (size_t i = 0; i < slots_.size() && pick < 0; i++)
-                if (!running_[i] && free_b >= need - std::min(need, cap_[i]) + reserve_) pick = (int)i;
-            if (pick < 0 && (int)slots_.size() < max_slots_ && free_b >= need + reserve_) {
+            if ((int)slots_.size() < max_slots_ && free_b >= need + reserve_) {
                 slots_.push_back(std::make_unique<ProveSlot>());
                 running_.push_back(nullptr);
                 cap_.push_back(0);
                 pick = (int)slots_.size() - 1;
             }
+            for (size_t i = 0; i < slots_.size() && pick < 0; i++)
+                if (!running_[i] && free_b >= need - std::min(need, cap_[i]) + reserve_) pick = (int)i;
         }
         if (pick < 0) break;  // wait for a slot or for memory
         queue_.pop_front();
