@@ -155,9 +155,17 @@ int zk_prove_stats(zk_ctx* ctx, double* out, int n);
  * A worker thread admits submitted requests into workspace slots while other
  * proofs run (continuous batching), creating or growing a slot only if free
  * HBM stays above reserve_bytes. Circuits must outlive their requests.
- * zk_server_destroy finishes every submitted request first. */
+ * zk_server_destroy finishes every submitted request first. Destroying the
+ * context first does the same for each of its servers, after which their
+ * handles only accept zk_server_destroy (other calls return ZK_E_STATE). */
 int zk_server_create(zk_ctx* ctx, int max_slots, uint64_t reserve_bytes, zk_server** out);
 void zk_server_destroy(zk_server* s);
+/* On an idle server (nothing queued or running; else ZK_E_STATE): create up
+ * to max_slots workspace slots, while free HBM stays above the reserve, and
+ * size each for every one of the count circuits, so that requests on them
+ * never allocate device memory while other proofs run. *n_slots (may be
+ * NULL) receives the slot count. */
+int zk_server_prepare(zk_server* s, const zk_circuit* const* circuits, int count, int* n_slots);
 /* witness: host, canonical, copied at submit; r, s NULL = 0 */
 int zk_server_submit(zk_server* s, const zk_circuit* c, const uint64_t* witness, uint32_t n_witness, const uint64_t r[4],
                      const uint64_t sb[4], uint64_t* ticket);

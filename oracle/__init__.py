@@ -62,6 +62,9 @@ def lib():
         L.or_prove_trapdoor_w.argtypes = [ctypes.c_void_p, _u64p, _u64p, _u64p, _u64p]
         L.or_verify_trapdoor.argtypes = [ctypes.c_void_p, _u64p, _u64p, _u64p, _u64p]
         L.or_to_dec.argtypes = [_u64p, ctypes.c_char_p, ctypes.c_size_t]
+        L.or_g1_progression.argtypes = [_u64p, _u64p, ctypes.c_size_t, _u64p]
+        L.or_fr_poly_eval.argtypes = [_u64p, ctypes.c_size_t, _u64p, ctypes.c_size_t, _u64p]
+        L.or_fr_dot.argtypes = [_u64p, _u64p, ctypes.c_size_t, _u64p]
         _lib = L
     return _lib
 
@@ -159,6 +162,34 @@ def g2_gen():
     g = np.zeros(16, dtype=np.uint64)
     lib().or_g2_generator(ptr(g))
     return g
+
+
+def g1_progression(k0, d, n):
+    """(n, 8) Montgomery affine AoS points (k0 + i*d)*G: bases with known logs."""
+    out = np.zeros((n, 8), np.uint64)
+    k = np.array(to_limbs(k0 % R), np.uint64)
+    dd = np.array(to_limbs(d % R), np.uint64)
+    assert lib().or_g1_progression(ptr(k), ptr(dd), n, ptr(out)) == 0
+    return out
+
+
+def poly_eval(coeffs, zs):
+    """sum_j coeffs[j] * z^j mod r for each python-int z (coefficient form carries through)."""
+    c = np.ascontiguousarray(coeffs, np.uint64).reshape(-1, 4)
+    z = arr([int(v) % R for v in zs])
+    out = np.zeros_like(z)
+    assert lib().or_fr_poly_eval(ptr(c), c.shape[0], ptr(z), z.shape[0], ptr(out)) == 0
+    return [from_limbs(o) for o in out]
+
+
+def fr_dot(a, b):
+    """sum_i a_i * b_i mod r (canonical (n, 4) arrays) as a python int."""
+    a = np.ascontiguousarray(a, np.uint64).reshape(-1, 4)
+    b = np.ascontiguousarray(b, np.uint64).reshape(-1, 4)
+    assert a.shape == b.shape
+    out = np.zeros(4, np.uint64)
+    assert lib().or_fr_dot(ptr(a), ptr(b), a.shape[0], ptr(out)) == 0
+    return from_limbs(out)
 
 
 class Circuit:

@@ -1049,3 +1049,115 @@ int or_verify_trapdoor(const or_circuit* c, const uint64_t* wv, const uint64_t r
     or_prove_trapdoor_w(c, wv, rr, ss, ref);
     return memcmp(ref, proof, sizeof ref) == 0;
 }
+
+/* ====================================================================
+ * Helpers for checks at full size (2^20 .. 2^22 elements).
+ * ==================================================================== */
+
+/* Points with known discrete logs: out[i] = (k0 + i*d)*G for i < n, Montgomery
+ * affine AoS (x[4], y[4]; infinity all zero). Point i = a*B + b is A_a + D_b with
+ * A_a = (k0 + a*B*d)*G and D_b = (b*d)*G; every block of B sums shares one
+ * inversion (Montgomery's simultaneous-inversion trick), blocks run in parallel.
+ * Pinned by: the exponents are known, so any sample point equals a textbook
+ * double-and-add product (tests/test_oracle.py). */
+#define PROG_B 4096
+int or_g1_progression(const uint64_t k0v[4], const uint64_t dv[4], size_t n, uint64_t* out) {
+    or_init();
+    if (!n) return 0;
+    fe k0, d;
+    memcpy(k0.v, k0v, 32); memcpy(d.v, dv, 32);
+    f_reduce(&k0, &k0, &R_MOD); f_reduce(&d, &d, &R_MOD);
+    g1_aff G = G1_GEN();
+    g1_jac jg = g1_from_aff(&G);
+    size_t nblk = (n + PROG_B - 1) / PROG_B;
+    g1_aff* D = (g1_aff*)malloc(sizeof(g1_aff) * PROG_B);
+    g1_aff* A = (g1_aff*)malloc(sizeof(g1_aff) * nblk);
+    g1_jac jd = g1_mul_jac(&jg, &d), acc = g1_jac_inf();
+    for (size_t b = 0; b < PROG_B; b++) { D[b] = g1_to_aff(&acc); acc = g1_add(&acc, &jd); }
+    fe bd, bb = fe_u64(PROG_B);
+    RMUL(&bd, &bb, &d);
+    g1_jac step = g1_mul_jac(&jg, &bd);
+    acc = g1_mul_jac(&jg, &k0);
+    for (size_t a = 0; a < nblk; a++) { A[a] = g1_to_aff(&acc); acc = g1_add(&acc, &step); }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (size_t a = 0; a < nblk; a++) {
+        fe* den = (fe*)malloc(sizeof(fe) * PROG_B);
+        fe* pre = (fe*)malloc(sizeof(fe) * PROG_B);
+        unsigned char gen[PROG_B];   /* 1 = generic affine sum, 0 = handled by the Jacobian path */
+        size_t i0 = a * PROG_B, cnt = n - i0 < PROG_B ? n - i0 : PROG_B;
+        fe prod = fe_u64(1);
+        for (size_t b = 0; b < cnt; b++) {
+            gen[b] = !A[a].inf && !D[b].inf && !fe_eq(&A[a].x, &D[b].x);
+            if (gen[b]) PSUB(&den[b], &D[b].x, &A[a].x); else den[b] = fe_u64(1);
+            pre[b] = prod;
+            PMUL(&prod, &prod, &den[b]);
+        }
+        fe inv;
+        f_inv(&inv, &prod, &P_MOD);
+        for (size_t b = cnt; b-- > 0;) {
+            fe di;
+            PMUL(&di, &inv, &pre[b]);
+            PMUL(&inv, &inv, &den[b]);
+            g1_aff o;
+            if (gen[b]) {
+                fe lam, t, x3, y3;
+                PSUB(&t, &D[b].y, &A[a].y); PMUL(&lam, &t, &di);
+                PMUL(&x3, &lam, &lam); PSUB(&x3, &x3, &A[a].x); PSUB(&x3, &x3, &D[b].x);
+                PSUB(&t, &A[a].x, &x3); PMUL(&y3, &lam, &t); PSUB(&y3, &y3, &A[a].y);
+                o.x = x3; o.y = y3; o.inf = 0;
+            } else {
+                g1_jac p = g1_from_aff(&A[a]), q = g1_from_aff(&D[b]);
+                g1_jac s = g1_add(&p, &q);
+                o = g1_to_aff(&s);
+            }
+            uint64_t* dst = out + 8 * (i0 + b);
+            g1_to_mont(&o, dst, dst + 4);
+        }
+        free(den); free(pre);
+    }
+    free(D); free(A);
+    return 0;
+}
+
+/* Horner evaluation of sum_j c_j z^j mod r at nz points z (canonical) in one
+ * pass per point, points in parallel. The coefficients' representation carries
+ * through (Montgomery coefficients give the Montgomery value), so NTT output
+ * rows are checked by definition: X_k = sum_j x_j w^(jk). */
+int or_fr_poly_eval(const uint64_t* coeffs, size_t n, const uint64_t* zs, size_t nz, uint64_t* out) {
+    or_init();
+#pragma omp parallel for schedule(dynamic, 1)
+    for (size_t q = 0; q < nz; q++) {
+        fe z, acc = fe_zero(), c;
+        memcpy(z.v, zs + 4 * q, 32);
+        for (size_t j = n; j-- > 0;) {
+            memcpy(c.v, coeffs + 4 * j, 32);
+            RMUL(&acc, &acc, &z);
+            RADD(&acc, &acc, &c);
+        }
+        memcpy(out + 4 * q, acc.v, 32);
+    }
+    return 0;
+}
+
+/* sum_i a_i b_i mod r (canonical in and out), in parallel chunks */
+int or_fr_dot(const uint64_t* a, const uint64_t* b, size_t n, uint64_t out[4]) {
+    or_init();
+    enum { CH = 1 << 14 };
+    size_t nch = (n + CH - 1) / CH;
+    fe* part = (fe*)calloc(nch ? nch : 1, sizeof(fe));
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < nch; k++) {
+        fe acc = fe_zero(), x, y, t;
+        size_t e = (k + 1) * CH < n ? (k + 1) * CH : n;
+        for (size_t i = k * CH; i < e; i++) {
+            memcpy(x.v, a + 4 * i, 32); memcpy(y.v, b + 4 * i, 32);
+            RMUL(&t, &x, &y); RADD(&acc, &acc, &t);
+        }
+        part[k] = acc;
+    }
+    fe tot = fe_zero();
+    for (size_t k = 0; k < nch; k++) RADD(&tot, &tot, &part[k]);
+    memcpy(out, tot.v, 32);
+    free(part);
+    return 0;
+}

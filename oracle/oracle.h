@@ -101,6 +101,14 @@ int or_verify_trapdoor(const or_circuit* c, const uint64_t* w, const uint64_t r[
 /* exponents (canonical Fr) of the toxic waste: tau, alpha, beta, gamma, delta */
 void or_circuit_toxic(const or_circuit* c, uint64_t out[20]);
 
+/* ---- helpers for checks at full size ---- */
+/* out[i] = (k0 + i*d)*G, i < n: Montgomery affine AoS x[4], y[4] (known logs) */
+int or_g1_progression(const uint64_t k0[4], const uint64_t d[4], size_t n, uint64_t* out);
+/* out[q] = sum_j coeffs[j] * zs[q]^j mod r (z canonical; coefficient form carries through) */
+int or_fr_poly_eval(const uint64_t* coeffs, size_t n, const uint64_t* zs, size_t nz, uint64_t* out);
+/* sum_i a_i * b_i mod r, canonical */
+int or_fr_dot(const uint64_t* a, const uint64_t* b, size_t n, uint64_t out[4]);
+
 /* decimal string of a canonical 256-bit value */
 void or_to_dec(const uint64_t a[4], char* buf, size_t buf_size);
 

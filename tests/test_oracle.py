@@ -342,3 +342,30 @@ def test_zkey_writer_layout(tmp_path):
     assert ncoef == c.nnz_a + c.nnz_b and secs[4][1] == 4 + 44 * ncoef
     w = open(wt, "rb").read()
     assert w[:4] == b"wtns" and len(w) == 12 + 12 + 40 + 12 + 32 * c.n_vars
+
+
+def test_full_size_helpers_pinned():
+    """The oracle helpers behind the full-size GPU checks, against textbook
+    definitions: known-log points vs affine double-and-add (including block
+    boundaries of the batched construction), Horner evaluation vs the DFT
+    definition, and the Fr dot product vs Python big ints."""
+    rng = np.random.default_rng(17)
+    k0, d = int.from_bytes(rng.bytes(32), "little") % pyref.R, int.from_bytes(rng.bytes(32), "little") % pyref.R
+    n = 9000
+    pts = O.g1_progression(k0, d, n)
+    for i in (0, 1, 2, 4095, 4096, 4097, 8191, 8192, n - 1):
+        x, y = pyref.ec_mul(pyref.G1, (k0 + i * d) % pyref.R, pyref.Fp1)
+        assert [O.from_limbs(pts[i, :4]), O.from_limbs(pts[i, 4:])] == [x * pyref.RM % pyref.P, y * pyref.RM % pyref.P], i
+    # exponent 0 at i = 0 gives the all-zero infinity encoding
+    z = O.g1_progression(0, 5, 3)
+    assert not z[0].any() and O.from_limbs(z[2, :4]) == pyref.ec_mul(pyref.G1, 10, pyref.Fp1)[0] * pyref.RM % pyref.P
+    # Horner evaluation == DFT rows
+    log_n = 6
+    xs = [int.from_bytes(rng.bytes(32), "little") % pyref.R for _ in range(1 << log_n)]
+    w = pyref.root(log_n)
+    dft = pyref.dft(xs, w)
+    ks = [0, 1, 5, 63]
+    assert O.poly_eval(O.arr(xs), [pow(w, k, pyref.R) for k in ks]) == [dft[k] for k in ks]
+    a = [int.from_bytes(rng.bytes(32), "little") % pyref.R for _ in range(40000)]
+    b = [int.from_bytes(rng.bytes(32), "little") % pyref.R for _ in range(40000)]
+    assert O.fr_dot(O.arr(a), O.arr(b)) == sum(x * y for x, y in zip(a, b)) % pyref.R

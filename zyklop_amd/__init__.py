@@ -80,6 +80,7 @@ ABI = {
     "zk_prove_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
     "zk_server_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "zk_server_destroy": (None, [_vp]),
+    "zk_server_prepare": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "zk_server_submit": (ctypes.c_int, [_vp, _vp, _u64p, ctypes.c_uint32, _u64p, _u64p, ctypes.POINTER(ctypes.c_uint64)]),
     "zk_server_submit_device": (ctypes.c_int, [_vp, _vp, _vp, _u64p, _u64p, ctypes.POINTER(ctypes.c_uint64)]),
     "zk_server_poll": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(Proof),
@@ -417,6 +418,14 @@ class Server:
 
     def __del__(self):
         self.close()
+
+    def prepare(self, circuits):
+        """Size every slot for every circuit up front (idle server); returns the slot count."""
+        n = len(circuits)
+        ch = (_vp * n)(*[c.h for c in circuits])
+        k = ctypes.c_int(0)
+        _check(lib().zk_server_prepare(self.h, ch, n, ctypes.byref(k)))
+        return k.value
 
     def submit(self, circuit, witness, r=None, s=None):
         w = np.ascontiguousarray(witness, dtype=np.uint64)

@@ -15,9 +15,15 @@
 #include "synth.hpp"
 #include "zkey.hpp"
 
+struct zk_server;
+
 struct zk_ctx {
     zk::Context impl;
     std::vector<std::unique_ptr<zk::ProveSlot>> batch_slots;
+    // servers created on this context: destroying the context finishes and
+    // detaches them first (their worker threads use the context)
+    std::mutex servers_mu;
+    std::set<zk_server*> servers;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     float last_ms[4] = {0, 0, 0, 0};
     bool profiling = false;
@@ -40,7 +46,8 @@ struct zk_circuit {
 };
 
 struct zk_server {
-    std::unique_ptr<zk::Server> impl;
+    zk_ctx* ctx = nullptr;               // null once the context was destroyed
+    std::unique_ptr<zk::Server> impl;    // null once the context was destroyed
 };
 
 struct zk_batch {
@@ -103,17 +110,24 @@ void run_batch(zk_ctx* ctx, int count, const zk_circuit* const* circuits, const 
     for (int i = 0; i < count; i++) distinct.insert(circuits[i]->impl.get());
     for (int k = 0; k < inflight; k++)
         for (const zk::Circuit* c : distinct) ctx->batch_slots[k]->reserve(ctx->impl, *c, !on_dev);
-    for (; next < inflight; next++)
-        ctx->batch_slots[next % nslots]->enqueue(ctx->impl, *circuits[next]->impl, witnesses[next], on_dev,
-                                                 r ? r + 4 * next : nullptr, s ? s + 4 * next : nullptr);
-    for (int i = 0; i < count; i++) {
-        zk::ProveSlot& sl = *ctx->batch_slots[i % nslots];
-        sl.finish(&out[i]);
-        if (next < count) {
-            sl.enqueue(ctx->impl, *circuits[next]->impl, witnesses[next], on_dev, r ? r + 4 * next : nullptr,
-                       s ? s + 4 * next : nullptr);
-            next++;
+    try {
+        for (; next < inflight; next++)
+            ctx->batch_slots[next % nslots]->enqueue(ctx->impl, *circuits[next]->impl, witnesses[next], on_dev,
+                                                     r ? r + 4 * next : nullptr, s ? s + 4 * next : nullptr);
+        for (int i = 0; i < count; i++) {
+            zk::ProveSlot& sl = *ctx->batch_slots[i % nslots];
+            sl.finish(&out[i]);
+            if (next < count) {
+                sl.enqueue(ctx->impl, *circuits[next]->impl, witnesses[next], on_dev, r ? r + 4 * next : nullptr,
+                           s ? s + 4 * next : nullptr);
+                next++;
+            }
         }
+    } catch (...) {
+        // drain what the other slots still have queued before reporting, so
+        // no device work of this batch outlives the call
+        for (int k = 0; k < inflight; k++) ctx->batch_slots[k]->drain();
+        throw;
     }
 }
 
@@ -184,7 +198,18 @@ int zk_ctx_create(int device, zk_ctx** out) {
     });
 }
 
-void zk_ctx_destroy(zk_ctx* ctx) { delete ctx; }
+void zk_ctx_destroy(zk_ctx* ctx) {
+    if (!ctx) return;
+    {
+        std::lock_guard<std::mutex> lk(ctx->servers_mu);
+        for (zk_server* s : ctx->servers) {
+            s->impl.reset();   // finishes every request the server accepted
+            s->ctx = nullptr;
+        }
+        ctx->servers.clear();
+    }
+    delete ctx;
+}
 
 int zk_circuit_load(zk_ctx* ctx, const char* zkey_path, zk_circuit** out) {
     return guard([&] {
@@ -425,25 +450,54 @@ int zk_server_create(zk_ctx* ctx, int max_slots, uint64_t reserve_bytes, zk_serv
             delete s;
             throw;
         }
+        s->ctx = ctx;
+        std::lock_guard<std::mutex> lk(ctx->servers_mu);
+        ctx->servers.insert(s);
         *out = s;
     });
 }
 
-void zk_server_destroy(zk_server* s) { delete s; }
+void zk_server_destroy(zk_server* s) {
+    if (!s) return;
+    if (s->ctx) {
+        std::lock_guard<std::mutex> lk(s->ctx->servers_mu);
+        s->ctx->servers.erase(s);
+    }
+    delete s;
+}
+
+static zk::Server& live(zk_server* s) {
+    need(s != nullptr, "NULL server");
+    if (!s->impl) throw zk::Error(ZK_E_STATE, "the server's context was destroyed");
+    return *s->impl;
+}
+
+int zk_server_prepare(zk_server* s, const zk_circuit* const* circuits, int count, int* n_slots) {
+    return guard([&] {
+        need(circuits != nullptr, "NULL circuits");
+        std::vector<const zk::Circuit*> cs;
+        for (int i = 0; i < count; i++) {
+            need(circuits[i] != nullptr, "NULL circuit");
+            cs.push_back(circuits[i]->impl.get());
+        }
+        int k = live(s).prepare(cs.data(), count);
+        if (n_slots) *n_slots = k;
+    });
+}
 
 int zk_server_submit(zk_server* s, const zk_circuit* c, const uint64_t* witness, uint32_t n_witness, const uint64_t r[4],
                      const uint64_t sb[4], uint64_t* ticket) {
     return guard([&] {
-        need(s && c && witness && ticket, "NULL argument");
-        *ticket = s->impl->submit(*c->impl, witness, n_witness, r, sb);
+        need(c && witness && ticket, "NULL argument");
+        *ticket = live(s).submit(*c->impl, witness, n_witness, r, sb);
     });
 }
 
 int zk_server_submit_device(zk_server* s, const zk_circuit* c, const void* d_witness, const uint64_t r[4],
                             const uint64_t sb[4], uint64_t* ticket) {
     return guard([&] {
-        need(s && c && d_witness && ticket, "NULL argument");
-        *ticket = s->impl->submit_device(*c->impl, d_witness, r, sb);
+        need(c && d_witness && ticket, "NULL argument");
+        *ticket = live(s).submit_device(*c->impl, d_witness, r, sb);
     });
 }
 
@@ -454,35 +508,34 @@ static int server_result(int status, const std::string& err) {
 
 int zk_server_poll(zk_server* s, uint64_t ticket, int* done, zk_proof* out, double times[3]) {
     return guard([&] {
-        need(s && done, "NULL argument");
+        need(done != nullptr, "NULL argument");
         int st = ZK_OK;
         std::string err;
-        *done = s->impl->poll(ticket, out, times, &st, &err) ? 1 : 0;
+        *done = live(s).poll(ticket, out, times, &st, &err) ? 1 : 0;
         if (*done) server_result(st, err);
     });
 }
 
 int zk_server_wait(zk_server* s, uint64_t ticket, zk_proof* out, double times[3]) {
     return guard([&] {
-        need(s != nullptr, "NULL server");
         int st = ZK_OK;
         std::string err;
-        s->impl->wait(ticket, out, times, &st, &err);
+        live(s).wait(ticket, out, times, &st, &err);
         server_result(st, err);
     });
 }
 
 int zk_server_cancel(zk_server* s, uint64_t ticket, int* cancelled) {
     return guard([&] {
-        need(s && cancelled, "NULL argument");
-        *cancelled = s->impl->cancel(ticket) ? 1 : 0;
+        need(cancelled != nullptr, "NULL argument");
+        *cancelled = live(s).cancel(ticket) ? 1 : 0;
     });
 }
 
 int zk_server_stats(zk_server* s, double out[8]) {
     return guard([&] {
-        need(s && out, "NULL argument");
-        s->impl->stats(out);
+        need(out != nullptr, "NULL argument");
+        live(s).stats(out);
     });
 }
 

@@ -69,6 +69,19 @@ struct DevBuf {
     template <class T> T* as() const { return reinterpret_cast<T*>(p); }
 };
 
+// Device bytes a sizing pass (DevBuf::ensure on every buffer) would add: each
+// requirement is counted against the buffer's current size, or in full when
+// `fresh` (a workspace not yet created). Admission control uses it to know
+// whether queueing work would allocate.
+struct GrowCount {
+    bool fresh = false;
+    size_t bytes = 0;
+    void need(const DevBuf& d, size_t n) {
+        const size_t cur = fresh ? 0 : d.bytes;
+        if (n > cur) bytes += n - cur;
+    }
+};
+
 // Owning pinned host allocation (hipHostMalloc).
 struct HostBuf {
     void* p = nullptr;

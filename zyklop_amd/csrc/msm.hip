@@ -303,23 +303,35 @@ static SortShape sort_shape(size_t n, int c) {
     return z;
 }
 
+// every buffer of a sort with its size for shape z: f(buffer, bytes)
+template <class S, class F>
+static void sort_buffers(S& s, const SortShape& z, F&& f) {
+    f(s.half, std::max<size_t>(1, z.n2) * 16);
+    f(s.matrix, z.mlen * 4);
+    f(s.mscan, z.mlen * 4 + (size_t)4 * (cdiv(z.mlen, SCAN_TILE) + 1));
+    f(s.pairs, z.ne * 8);
+    f(s.entries, z.ne * 4);
+    f(s.offsets, (z.nb + 1) * 4);
+    f(s.order, z.nb * 4);
+    f(s.lkeys, (size_t)4 * (2 * z.nkey + cdiv(z.nkey, SCAN_TILE) + 1));
+    f(s.chunks, z.max_chunks * sizeof(ChunkRec));
+    f(s.large, z.max_large * sizeof(LargeRec));
+    f(s.counters, (size_t)64);
+}
+
 static void sort_reserve(MsmSort& s, const SortShape& z) {
-    s.half.ensure(std::max<size_t>(1, z.n2) * 16);
-    s.matrix.ensure(z.mlen * 4);
-    s.mscan.ensure(z.mlen * 4 + (size_t)4 * (cdiv(z.mlen, SCAN_TILE) + 1));
-    s.pairs.ensure(z.ne * 8);
-    s.entries.ensure(z.ne * 4);
-    s.offsets.ensure((z.nb + 1) * 4);
-    s.order.ensure(z.nb * 4);
-    s.lkeys.ensure((size_t)4 * (2 * z.nkey + cdiv(z.nkey, SCAN_TILE) + 1));
-    s.chunks.ensure(z.max_chunks * sizeof(ChunkRec));
-    s.large.ensure(z.max_large * sizeof(LargeRec));
-    s.counters.ensure(64);
+    sort_buffers(s, z, [](DevBuf& b, size_t n) { b.ensure(n); });
 }
 
 size_t MsmSort::reserve(size_t n_, int c) {
     const SortShape z = sort_shape(n_, c);
     sort_reserve(*this, z);
+    return z.max_chunks;
+}
+
+size_t MsmSort::count_reserve(size_t n_, int c, GrowCount& gc) const {
+    const SortShape z = sort_shape(n_, c);
+    sort_buffers(*this, z, [&](const DevBuf& b, size_t n) { gc.need(b, n); });
     return z.max_chunks;
 }
 
@@ -711,15 +723,24 @@ MsmScratch::~MsmScratch() {
     if (ev_end) (void)hipEventDestroy(ev_end);
 }
 
+template <class S, class F>
+static void scratch_buffers(S& q, const MsmGeom& g, size_t max_chunks, bool g2, int split, F&& f) {
+    const size_t xb = g2 ? 256 : 128;
+    const size_t nb = (size_t)g.W * g.K;
+    f(q.buckets, nb * split * xb);
+    f(q.r1, (size_t)g.W * g.nseg * xb);
+    f(q.t1, (size_t)g.W * g.nseg * xb);
+    f(q.r2, (size_t)g.W * g.nblk * xb);
+    f(q.t2, (size_t)g.W * g.nblk * xb);
+    f(q.partials, max_chunks * xb);
+}
+
 void MsmScratch::ensure(const MsmGeom& g, size_t max_chunks, bool g2, int split) {
-    size_t xb = g2 ? 256 : 128;
-    size_t nb = (size_t)g.W * g.K;
-    buckets.ensure(nb * split * xb);
-    r1.ensure((size_t)g.W * g.nseg * xb);
-    t1.ensure((size_t)g.W * g.nseg * xb);
-    r2.ensure((size_t)g.W * g.nblk * xb);
-    t2.ensure((size_t)g.W * g.nblk * xb);
-    partials.ensure(max_chunks * xb);
+    scratch_buffers(*this, g, max_chunks, g2, split, [](DevBuf& b, size_t n) { b.ensure(n); });
+}
+
+void MsmScratch::count_ensure(const MsmGeom& g, size_t max_chunks, bool g2, int split, GrowCount& gc) const {
+    scratch_buffers(*this, g, max_chunks, g2, split, [&](const DevBuf& b, size_t n) { gc.need(b, n); });
 }
 
 // The chunked path of the oversized buckets (lists built by the sort): their

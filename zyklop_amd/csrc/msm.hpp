@@ -61,6 +61,8 @@ struct MsmSort {
     // sizes every buffer a sort of n scalars at window c needs (no launch);
     // returns the chunk-list capacity for MsmScratch::ensure
     size_t reserve(size_t n, int c);
+    // the device bytes reserve(n, c) would add now (into gc); same return value
+    size_t count_reserve(size_t n, int c, GrowCount& gc) const;
     // sorts canonical scalars (n x 32 B, device, n < 2^30) into bucket order;
     // an entry is the index i < 2n of a base (msm_points_to29 order) with the
     // digit's sign in bit 31
@@ -84,6 +86,8 @@ struct MsmScratch {
     void enable_timing();
     ~MsmScratch();
     void ensure(const MsmGeom& g, size_t max_chunks, bool g2, int split);
+    // the device bytes ensure() would add now (into gc)
+    void count_ensure(const MsmGeom& g, size_t max_chunks, bool g2, int split, GrowCount& gc) const;
 };
 
 // Launch bucket accumulation + GPU reduction for points (affine, Montgomery,

@@ -231,13 +231,10 @@ static zkh::Fr root_of_unity(int log_n) {
     return zkh::Fr::from_u64(5).pow(e);
 }
 
-void NttPlan::build(int ln, hipStream_t st) {
-    ZK_REQUIRE(ln >= 1 && ln <= 27, ZK_E_ARG, "NTT size must be 2^1..2^27");
-    log_n = ln;
-    n = (size_t)1 << ln;
-    // radix-8 passes; a radix-16 pass absorbs one leftover bit, radix-4/2 the rest
-    np = 0;
-    int rem = ln;
+// pass radices of a size-2^ln transform: radix-8 passes; a radix-16 pass
+// absorbs one leftover bit, radix-4/2 the rest. Returns the pass count.
+static int ntt_layout(int ln, int lr[NTT_MAX_PASSES]) {
+    int np = 0, rem = ln;
     while (rem > 0) {
         int take = rem >= 3 ? 3 : rem;
         lr[np++] = take;
@@ -245,6 +242,33 @@ void NttPlan::build(int ln, hipStream_t st) {
     }
     if (np >= 2 && lr[np - 1] == 1) { lr[np - 2] += 1; np--; }  // ...3,1 -> ...4
     for (int j = np; j < NTT_MAX_PASSES; j++) lr[j] = 0;
+    return np;
+}
+
+// entries of the concatenated pass twiddle tables (passes 0..np-2), and each pass's offset
+static size_t ntt_tw_entries(int ln, int np, const int lr[NTT_MAX_PASSES], size_t* off) {
+    size_t tot = 0;
+    int consumed = 0;
+    for (int j = 0; j < np; j++) {
+        if (off) off[j] = tot;
+        if (j < np - 1) tot += (size_t)1 << (ln - consumed);
+        consumed += lr[j];
+    }
+    return tot;
+}
+
+size_t NttPlan::bytes_for(int ln) {
+    int lr_[NTT_MAX_PASSES];
+    const int np_ = ntt_layout(ln, lr_);
+    const size_t tot = ntt_tw_entries(ln, np_, lr_, nullptr);
+    return 2 * std::max<size_t>(tot, 1) * 32 + 4 * ((size_t)1 << ln) * 32;
+}
+
+void NttPlan::build(int ln, hipStream_t st) {
+    ZK_REQUIRE(ln >= 1 && ln <= 27, ZK_E_ARG, "NTT size must be 2^1..2^27");
+    log_n = ln;
+    n = (size_t)1 << ln;
+    np = ntt_layout(ln, lr);
 
     // base power tables for w_2n and w_2n^-1 (cover every w_M, w_R and the coset factors)
     zkh::Fr w2n = root_of_unity(ln + 1), w2n_inv = w2n.inv();
@@ -271,16 +295,10 @@ void NttPlan::build(int ln, hipStream_t st) {
         }
     }
     // pass twiddle tables w_M^(c k), layout [k][c], for passes 0..np-2
-    size_t tot = 0;
-    int consumed = 0;
-    for (int j = 0; j < np; j++) {
-        tw_off[j] = tot;
-        if (j < np - 1) tot += (size_t)1 << (ln - consumed);
-        consumed += lr[j];
-    }
+    const size_t tot = ntt_tw_entries(ln, np, lr, tw_off);
     tw_fwd.ensure(std::max<size_t>(tot, 1) * 32);
     tw_inv.ensure(std::max<size_t>(tot, 1) * 32);
-    consumed = 0;
+    int consumed = 0;
     for (int j = 0; j < np - 1; j++) {
         int logM = ln - consumed;
         int logS = logM - lr[j];

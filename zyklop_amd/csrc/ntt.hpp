@@ -39,6 +39,8 @@ struct NttPlan {
     DevBuf inv_plain;   // n^-1             (plain inverse)
     DevBuf inv_uncoset; // n^-1 * w_2n^-i   (inverse of the coset NTT)
     void build(int log_n, hipStream_t st);
+    // device bytes a plan of size 2^log_n holds
+    static size_t bytes_for(int log_n);
 };
 
 // in -> out (in is clobbered unless in == out and np == 1). Each of the

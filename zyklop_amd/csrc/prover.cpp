@@ -382,6 +382,11 @@ void ProveSlot::body(Context& ctx, const Circuit& c, ProofParts& out) {
     }
 }
 
+void ProveSlot::drain() noexcept {
+    for (hipStream_t p : {s0, s1, s2, s3, s4})
+        if (p) (void)hipStreamSynchronize(p);
+}
+
 bool ProveSlot::device_done() const {
     for (hipStream_t p : {s0, s1, s2, s3, s4}) {
         hipError_t e = hipStreamQuery(p);
@@ -391,20 +396,39 @@ bool ProveSlot::device_done() const {
     return true;
 }
 
-size_t ProveSlot::bytes_needed(const Circuit& c) {
+// The buffers reserve(ctx, c, ...) sizes, counted instead of allocated: sl
+// null counts a slot not yet created. The NTT plan is the context's (shared
+// by every slot) and counts while it does not exist.
+size_t ProveSlot::count_bytes(const Context& ctx, const Circuit& c, const ProveSlot* sl) {
+    static const DevBuf no_buf;
+    static const MsmSort no_sort;
+    static const MsmScratch no_scr;
+    GrowCount g;
+    g.fresh = sl == nullptr;
     const size_t nS = (size_t)c.n_vars + 4, n = c.n;
-    MsmGeom gw = msm_geom(nS, 0, false), gh = msm_geom(n, 0, false);
-    size_t b = nS * 32 + 7 * n * 32;                                    // scalars + Fr arrays
-    b += 2 * nS * 16 + 2 * n * 16;                                      // GLV half scalars
-    b += 3 * ((size_t)gw.W * 2 * nS * 4 + (size_t)gh.W * 2 * n * 4);    // sorted entries + (entry, bucket) pairs
-    b += 4 * 4 * ((size_t)gw.W * gw.K + (size_t)gh.W * gh.K);           // offsets / order
-    b += 2 * 4 * (size_t)4096 * (std::min<size_t>(2048, nS / 1024 + 1) + std::min<size_t>(2048, n / 1024 + 1));  // group matrices
-    size_t bw = (size_t)gw.W * gw.K, bh = (size_t)gh.W * gh.K;
-    b += MSM_SPLIT_MAX * (3 * bw * 128 + bw * 256 + bh * 128);          // bucket slots A, B1, C, B2, H (max split)
-    b += (3 * bw + bw * 2 + bh) * 128 / 4;                              // reduction partials (upper bound)
-    b += ((size_t)gw.W * 2 * nS / 64) * (16 + 128) * 4 + ((size_t)gh.W * 2 * n / 64) * (16 + 128);  // large-bucket lists
-    return b;
+    g.need(sl ? sl->scal : no_buf, nS * 32);
+    if (sl) {
+        for (const DevBuf* b : {&sl->aw, &sl->bw, &sl->cw, &sl->t0, &sl->t1, &sl->t2, &sl->h}) g.need(*b, n * 32);
+    } else {
+        g.bytes += 7 * n * 32;
+    }
+    const int cw = ctx.window_for(0, nS), ch = ctx.window_for(1, n);
+    const size_t kw = (sl ? sl->sortW : no_sort).count_reserve(nS, cw, g);
+    const size_t kh = (sl ? sl->sortH : no_sort).count_reserve(n, ch, g);
+    const int s1 = ctx.split_for(0, nS, false), s2 = ctx.split_for(0, nS, true), sh = ctx.split_for(1, n, false);
+    const MsmGeom gw1 = msm_geom(nS, cw, false), gw2 = msm_geom(nS, cw, true), gh = msm_geom(n, ch, false);
+    (sl ? sl->scrA : no_scr).count_ensure(gw1, kw, false, s1, g);
+    (sl ? sl->scrB1 : no_scr).count_ensure(gw1, kw, false, s1, g);
+    (sl ? sl->scrC : no_scr).count_ensure(gw1, kw, false, s1, g);
+    (sl ? sl->scrB2 : no_scr).count_ensure(gw2, kw, true, s2, g);
+    (sl ? sl->scrH : no_scr).count_ensure(gh, kh, false, sh, g);
+    if (!ctx.ntt_plans.count(c.log_n)) g.bytes += NttPlan::bytes_for(c.log_n);
+    return g.bytes;
 }
+
+size_t ProveSlot::bytes_needed(const Context& ctx, const Circuit& c) { return count_bytes(ctx, c, nullptr); }
+
+size_t ProveSlot::grow_bytes(const Context& ctx, const Circuit& c) const { return count_bytes(ctx, c, this); }
 
 template <class T>
 static void put_aff(const zkh::Xyzz<T>& p, uint64_t* dst) {

@@ -105,11 +105,20 @@ struct ProveSlot {
     void finish(zk_proof* out);
     // true when every stream of the slot has drained (non-blocking)
     bool device_done() const;
-    // device bytes a proof of c needs in a slot (workspace estimate for admission)
-    static size_t bytes_needed(const Circuit& c);
+    // device bytes a proof of c needs in a new slot (its whole workspace at
+    // the windows and splits ctx would use, plus the NTT plan if missing)
+    static size_t bytes_needed(const Context& ctx, const Circuit& c);
+    // device bytes reserve(ctx, c, ...) would add to this slot now (0: a
+    // proof of c queues without allocating)
+    size_t grow_bytes(const Context& ctx, const Circuit& c) const;
+    // wait for every stream of the slot, ignoring errors (cleanup after a failure)
+    void drain() noexcept;
     // number of per-proof statistics in stats[] (layout in zk_abi.h: zk_prove_stats)
     static constexpr int N_STATS = 24;
     size_t device_bytes() const;
+
+  private:
+    static size_t count_bytes(const Context& ctx, const Circuit& c, const ProveSlot* sl);
 };
 
 constexpr int TUNE_C_MIN = 11, TUNE_C_MAX = 19;  // autotuner window range (c = 19: 7 windows)

@@ -51,7 +51,12 @@ uint64_t Server::admit_request(std::shared_ptr<ServerRequest> req, const Circuit
     ZK_REQUIRE(zkh::Fr::is_canonical(req->r) && zkh::Fr::is_canonical(req->s), ZK_E_ARG, "blinding must be canonical");
     size_t free_b = 0, total_b = 0;
     ZK_HIP(hipMemGetInfo(&free_b, &total_b));
-    if (ProveSlot::bytes_needed(c) + reserve_ > total_b) {
+    size_t need;
+    {
+        std::lock_guard<std::mutex> g(ctx_.mu);
+        need = ProveSlot::bytes_needed(ctx_, c);
+    }
+    if (need + reserve_ > total_b) {
         std::lock_guard<std::mutex> lk(mu_);
         n_rejected_++;
         throw Error(ZK_E_NOMEM, "request needs more device memory than the GPU has");
@@ -68,6 +73,44 @@ uint64_t Server::admit_request(std::shared_ptr<ServerRequest> req, const Circuit
     }
     cv_work_.notify_one();
     return t;
+}
+
+int Server::prepare(const Circuit* const* cs, int count) {
+    ZK_REQUIRE(count >= 1, ZK_E_ARG, "no circuits to prepare for");
+    std::vector<const Circuit*> order;
+    for (int i = 0; i < count; i++) {
+        ZK_REQUIRE(cs[i] != nullptr, ZK_E_ARG, "NULL circuit");
+        order.push_back(cs[i]);
+    }
+    // largest first: its reserve allocates the largest buffers once, and the
+    // smaller circuits then find them big enough
+    std::sort(order.begin(), order.end(), [](const Circuit* a, const Circuit* b) {
+        return (size_t)a->n + a->n_vars > (size_t)b->n + b->n_vars;
+    });
+    std::lock_guard<std::mutex> lk(mu_);
+    for (auto& r : running_) ZK_REQUIRE(!r, ZK_E_STATE, "zk_server_prepare needs an idle server");
+    ZK_REQUIRE(queue_.empty(), ZK_E_STATE, "zk_server_prepare needs an idle server");
+    std::lock_guard<std::mutex> g(ctx_.mu);
+    ZK_HIP(hipSetDevice(ctx_.device));
+    for (int k = 0; k < max_slots_; k++) {
+        if (k == (int)slots_.size()) {
+            size_t need = 0;
+            for (const Circuit* c : order) need = std::max(need, ProveSlot::bytes_needed(ctx_, *c));
+            size_t free_b = 0, total_b = 0;
+            ZK_HIP(hipMemGetInfo(&free_b, &total_b));
+            if (k > 0 && free_b < need + reserve_) break;
+            slots_.push_back(std::make_unique<ProveSlot>());
+            running_.push_back(nullptr);
+        }
+        for (const Circuit* c : order) {
+            size_t free_b = 0, total_b = 0;
+            ZK_HIP(hipMemGetInfo(&free_b, &total_b));
+            ZK_REQUIRE(free_b >= slots_[k]->grow_bytes(ctx_, *c) + reserve_, ZK_E_NOMEM,
+                       "not enough free device memory to prepare a slot above the reserve");
+            slots_[k]->reserve(ctx_, *c, true);
+        }
+    }
+    return (int)slots_.size();
 }
 
 static void copy_out(const ServerRequest& q, zk_proof* out, double times[3], int* status, std::string* err) {
@@ -130,35 +173,44 @@ void Server::stats(double out[8]) {
     out[7] = peak_inflight_;
 }
 
-// Admit queued requests into slots, oldest first. A free slot whose workspace
-// already fits is used as is; otherwise a new slot is created, or else a free
-// slot grows, only if free HBM stays above the reserve afterwards. A new slot
-// comes first: growing frees device memory, which waits for the proofs the
-// other slots have in flight.
+// Admit queued requests into slots, oldest first. A free slot whose buffers
+// already fit the request (grow_bytes == 0: nothing is allocated) is used as
+// is; otherwise a new slot is created, or else a free slot grows, only if free
+// HBM stays above the reserve afterwards. A new slot comes first: growing
+// frees device memory, which waits for the proofs the other slots have in
+// flight. zk_server_prepare sizes every slot up front, so that admission
+// never allocates while proofs run.
 bool Server::try_admit(std::unique_lock<std::mutex>& lk) {
     bool any = false;
     while (!queue_.empty()) {
         auto req = queue_.front();
-        const size_t need = ProveSlot::bytes_needed(*req->circ);
+        // the sizes depend on the context's windows and NTT plans: read under its lock
+        auto grow = [&](size_t i) {
+            std::lock_guard<std::mutex> g(ctx_.mu);
+            return slots_[i]->grow_bytes(ctx_, *req->circ);
+        };
         int pick = -1;
         for (size_t i = 0; i < slots_.size() && pick < 0; i++)
-            if (!running_[i] && cap_[i] >= need) pick = (int)i;
+            if (!running_[i] && grow(i) == 0) pick = (int)i;
         if (pick < 0) {
             size_t free_b = 0, total_b = 0;
             ZK_HIP(hipMemGetInfo(&free_b, &total_b));
+            size_t need;
+            {
+                std::lock_guard<std::mutex> g(ctx_.mu);
+                need = ProveSlot::bytes_needed(ctx_, *req->circ);
+            }
             if ((int)slots_.size() < max_slots_ && free_b >= need + reserve_) {
                 slots_.push_back(std::make_unique<ProveSlot>());
                 running_.push_back(nullptr);
-                cap_.push_back(0);
                 pick = (int)slots_.size() - 1;
             }
             for (size_t i = 0; i < slots_.size() && pick < 0; i++)
-                if (!running_[i] && free_b >= need - std::min(need, cap_[i]) + reserve_) pick = (int)i;
+                if (!running_[i] && free_b >= grow(i) + reserve_) pick = (int)i;
         }
         if (pick < 0) break;  // wait for a slot or for memory
         queue_.pop_front();
         running_[pick] = req;
-        cap_[pick] = std::max(cap_[pick], need);
         double inflight = 0;
         for (auto& r : running_) inflight += r ? 1 : 0;
         peak_inflight_ = std::max(peak_inflight_, inflight);
@@ -179,11 +231,7 @@ bool Server::try_admit(std::unique_lock<std::mutex>& lk) {
             st = ZK_E_INTERNAL;
             err = e.what();
         }
-        if (st != ZK_OK) {
-            // drop whatever part of the proof was queued before the failure
-            for (hipStream_t p : {slots_[pick]->s0, slots_[pick]->s1, slots_[pick]->s2, slots_[pick]->s3, slots_[pick]->s4})
-                (void)hipStreamSynchronize(p);
-        }
+        if (st != ZK_OK) slots_[pick]->drain();  // drop whatever part of the proof was queued before the failure
         lk.lock();
         if (st != ZK_OK) {
             req->status = st;

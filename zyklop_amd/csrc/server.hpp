@@ -52,6 +52,10 @@ class Server {
     bool poll(uint64_t ticket, zk_proof* out, double times[3], int* status, std::string* err);
     void wait(uint64_t ticket, zk_proof* out, double times[3], int* status, std::string* err);
     bool cancel(uint64_t ticket);
+    // on an idle server: create up to max_slots slots (while free HBM stays
+    // above the reserve) and size each for every circuit of cs, so that no
+    // request on them allocates device memory later; returns the slot count
+    int prepare(const Circuit* const* cs, int count);
     void stats(double out[8]);
 
   private:
@@ -63,7 +67,6 @@ class Server {
     const size_t reserve_;
     std::vector<std::unique_ptr<ProveSlot>> slots_;
     std::vector<std::shared_ptr<ServerRequest>> running_;  // per slot, null when free
-    std::vector<size_t> cap_;                              // workspace bytes a slot already holds
     std::deque<std::shared_ptr<ServerRequest>> queue_;
     std::unordered_map<uint64_t, std::shared_ptr<ServerRequest>> reqs_;
     std::mutex mu_;

@@ -127,7 +127,10 @@ ZkeyView::ZkeyView(const std::string& path) : file(path) {
     B2 = bs.at(file, 7, &sz); ZK_REQUIRE(sz >= 128 * nv, ZK_E_FORMAT, "section 7 too short");
     C = bs.at(file, 8, &sz);  ZK_REQUIRE(sz >= 64 * nc, ZK_E_FORMAT, "section 8 too short");
     H = bs.at(file, 9, &sz);  ZK_REQUIRE(sz >= 64 * nh, ZK_E_FORMAT, "section 9 too short");
-    if (bs.sec.count(3)) IC = bs.at(file, 3, nullptr);
+    if (bs.sec.count(3)) {
+        IC = bs.at(file, 3, &sz);
+        ZK_REQUIRE(sz >= 64ull * (hdr.n_public + 1), ZK_E_FORMAT, "section 3 shorter than nPublic + 1 points");
+    }
 }
 
 std::vector<uint64_t> read_wtns(const std::string& path, uint32_t* n_vars) {
