@@ -256,6 +256,26 @@ int zk_synth_uniform(uint64_t seed, size_t count, uint64_t* out);
 /* n_vars / domain of the synthetic circuit (m, n_pub) */
 int zk_synth_sizes(uint32_t m, uint32_t n_pub, uint32_t* n_vars, uint32_t* domain);
 
+/* ---- Groth16 verification (host; no GPU) ----
+ * Replaces `snarkjs groth16 verify` (the reference's check, Makefile:135):
+ * the optimal ate pairing on BN254, e(A, B) == e(alpha1, beta2) e(vk_x, gamma2)
+ * e(C, delta2) with vk_x = IC_0 + sum_i pub_i IC_i, after checking that A and
+ * C lie on G1 and B on the twist in the order-r subgroup. *ok = 1 when the
+ * proof verifies, 0 when it does not; the status is nonzero only for bad
+ * arguments or files. publics: n_public canonical Fr values (4 words each),
+ * the statement's public inputs w_1..w_l; the proof in the zk_proof layout
+ * (Montgomery affine). The verification key comes from a resident circuit,
+ * from zkey §2/§3, or, with public.json and proof.json texts as written by
+ * zk_public_json / zk_proof_to_json, from a zkey file. */
+int zk_verify(const zk_circuit* c, const uint64_t* publics, uint32_t n_public, const zk_proof* proof, int* ok);
+int zk_verify_zkey(const char* zkey_path, const uint64_t* publics, uint32_t n_public, const zk_proof* proof, int* ok);
+int zk_verify_json(const char* zkey_path, const char* public_json, const char* proof_json, int* ok);
+/* e(P, Q) for P in G1 (8 words) and Q in G2 (16 words), Montgomery affine:
+ * the Fp12 value as 6 Fp2 coefficients c0.c0, c0.c1, c0.c2, c1.c0, c1.c1,
+ * c1.c2 of Fp12 = Fp6[w]/(w^2 - v), Fp6 = Fp2[v]/(v^3 - (9 + u)), each Fp2
+ * as (a0, a1) Montgomery (48 words). For tests of the pairing itself. */
+int zk_pairing(const uint64_t p[8], const uint64_t q[16], uint64_t out[48]);
+
 /* ---- host-side check of the MSM's scalar split (no GPU) ---- */
 /* No counterpart in the reference, whose MSMs run over full 254-bit scalars
  * (include/msm_g1.cuh:271-329). The GLV split the MSM kernels apply to every

@@ -48,6 +48,15 @@ def test_cli_matches_oracle(tmp_path):
     words_b = c.prove(rr, ss)
     canon_b = O.field_op(O.FP, O.FROM_MONT, words_b.reshape(8, 4))
     assert json.loads(open(pj).read())["pi_c"][:2] == [dec(canon_b[6]), dec(canon_b[7])]
+    # the CLI's own pairing check accepts its outputs and rejects a changed statement
+    v = subprocess.run([CLI, "verify", zkey, pub, pj], capture_output=True, text=True, timeout=120)
+    assert v.returncode == 0 and v.stdout.strip() == "OK", v.stderr
+    vals = json.loads(open(pub).read())
+    vals[0] = str((int(vals[0]) + 1) % O.R)
+    bad_pub = str(tmp_path / "bad_public.json")
+    open(bad_pub, "w").write(json.dumps(vals))
+    v = subprocess.run([CLI, "verify", zkey, bad_pub, pj], capture_output=True, text=True, timeout=120)
+    assert v.returncode == 2 and v.stdout.strip() == "INVALID"
 
 
 def test_cli_reports_bad_input(tmp_path):

@@ -6,13 +6,16 @@ Each test names the workload it covers:
   * the full prove at m = 2^20 constraints with 4096 public inputs;
   * the 64-proof mixed batch (m log-uniform in [2^14, 2^20], half over shared
     circuits), stream-batched;
-  * the served mix over resident 2^18 and 2^20 circuits.
+  * the served mix over resident 2^18 and 2^20 circuits, and the 2^24 circuit
+    (a size-2^25 coset NTT) served on prepared slots.
 
 What the oracle can afford at these sizes decides the check: MSM results
 against the discrete-log closed form (bases with known logs, result
 (sum s_i k_i)*G); NTT rows against direct polynomial evaluation at the row's
 point; proofs bit for bit against the trapdoor closed form of the oracle's
-circuit, plus the Groth16 equation in the exponent. Every comparison is exact.
+circuit, plus the Groth16 equation in the exponent. Proofs also pass the
+pairing check (zk_verify), which at 2^24 stands in for the closed form.
+Every comparison is exact.
 """
 import time
 
@@ -155,6 +158,9 @@ def test_prove_2p20_device_vs_trapdoor(ctx, circ2):
     assert np.array_equal(got, ref.prove_trapdoor(r, s))
     assert ref.verify(got, r, s)
     log(f"[prove 2^20] device proof == closed form, verified ({time.time() - t0:.1f} s oracle)")
+    t0 = time.time()
+    assert zk.verify(circ, w[1:L2 + 1], got)
+    log(f"[prove 2^20] pairing check passes ({time.time() - t0:.2f} s)")
 
 
 def test_prove_2p20_batch_vs_trapdoor(ctx, circ2):
@@ -259,5 +265,36 @@ def test_served_2p18_2p20(ctx, circ2):
         c, w, r, s = reqs[k]
         assert np.array_equal(served[k], ref.prove_trapdoor(r, s, w=w)), k
         assert ref.verify(served[k], r, s, w=w), k
+    for k, (c, w, r, s) in enumerate(reqs):
+        assert zk.verify(c, w[1:c.n_public + 1], served[k]), k
     log(f"[served] 12 requests == alone, peak in flight {st['peak_in_flight']:.0f}; 2^18 and 2^20 verified")
     c18.free()
+
+
+def test_served_2p24_pairing_verified(ctx):
+    """The largest served circuit, m = 2^24 with 1024 inputs (n = 2^25, a
+    size-2^25 coset NTT): the server, with its slots prepared for the circuit,
+    proves two requests (fresh blinding each); each equals the proof made
+    alone and passes the pairing check against the circuit's verification key.
+    The oracle's closed form is not run at this size (minutes of host work);
+    the pairing check does not need the trapdoor."""
+    t0 = time.time()
+    c = ctx.synth_circuit(242424, 1 << 24, 1024)
+    w = c.witness(4242)
+    log(f"[served 2^24] circuit and witness in {time.time() - t0:.1f} s, {c.device_bytes / 2**30:.1f} GiB resident")
+    rs = [fresh_rs(2400 + k) for k in range(2)]
+    srv = zk.Server(ctx, max_slots=2)
+    try:
+        slots = srv.prepare([c])
+        t0 = time.time()
+        tickets = [srv.submit(c, w, r, s) for r, s in rs]
+        served = [srv.wait(t)[0].words() for t in tickets]
+        log(f"[served 2^24] 2 requests on {slots} prepared slots in {time.time() - t0:.2f} s")
+    finally:
+        srv.close()
+    for k, (r, s) in enumerate(rs):
+        assert np.array_equal(served[k], ctx.prove(c, w, r, s).words()), k
+        assert zk.verify(c, w[1:1025], served[k]), k
+    assert not np.array_equal(served[0], served[1])
+    log("[served 2^24] both == alone, pairing check passes")
+    c.free()

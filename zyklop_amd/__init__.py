@@ -121,6 +121,11 @@ ABI = {
     "zk_synth_uniform": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_size_t, _u64p]),
     "zk_glv_split": (ctypes.c_int, [_u64p, ctypes.c_size_t, _u64p]),
     "zk_synth_sizes": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p]),
+    "zk_verify": (ctypes.c_int, [_vp, _u64p, ctypes.c_uint32, ctypes.POINTER(Proof), ctypes.POINTER(ctypes.c_int)]),
+    "zk_verify_zkey": (ctypes.c_int, [ctypes.c_char_p, _u64p, ctypes.c_uint32, ctypes.POINTER(Proof),
+                                      ctypes.POINTER(ctypes.c_int)]),
+    "zk_verify_json": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
+    "zk_pairing": (ctypes.c_int, [_u64p, _u64p, _u64p]),
 }
 
 
@@ -599,4 +604,50 @@ def glv_split(scalars):
     k = np.ascontiguousarray(scalars, dtype=np.uint64).reshape(-1, 4)
     out = np.zeros_like(k)
     _check(lib().zk_glv_split(u64p(k), k.shape[0], u64p(out)))
+    return out
+
+
+def _proof_arg(proof):
+    if isinstance(proof, Proof):
+        return proof
+    w = np.ascontiguousarray(proof, dtype=np.uint64).reshape(32)
+    pr = Proof()
+    ctypes.memmove(ctypes.byref(pr), w.ctypes.data, 256)
+    return pr
+
+
+def _publics_arg(publics):
+    p = np.ascontiguousarray(publics, dtype=np.uint64).reshape(-1, 4)
+    return p, p.shape[0]
+
+
+def verify(circuit, publics, proof):
+    """Pairing check of a proof against a resident circuit's verification key."""
+    p, n = _publics_arg(publics)
+    ok = ctypes.c_int(0)
+    _check(lib().zk_verify(circuit.h, u64p(p), n, ctypes.byref(_proof_arg(proof)), ctypes.byref(ok)))
+    return bool(ok.value)
+
+
+def verify_zkey(zkey_path, publics, proof):
+    """Pairing check against the verification key of a .zkey file (sections 2 and 3)."""
+    p, n = _publics_arg(publics)
+    ok = ctypes.c_int(0)
+    _check(lib().zk_verify_zkey(zkey_path.encode(), u64p(p), n, ctypes.byref(_proof_arg(proof)), ctypes.byref(ok)))
+    return bool(ok.value)
+
+
+def verify_json(zkey_path, public_json, proof_json):
+    """Pairing check from public.json and proof.json texts (replaces snarkjs groth16 verify)."""
+    ok = ctypes.c_int(0)
+    _check(lib().zk_verify_json(zkey_path.encode(), public_json.encode(), proof_json.encode(), ctypes.byref(ok)))
+    return bool(ok.value)
+
+
+def pairing(p, q):
+    """e(P, Q) as 48 words (Fp12, Montgomery) for G1 P (8 words) and G2 Q (16 words)."""
+    pp = np.ascontiguousarray(p, dtype=np.uint64).reshape(8)
+    qq = np.ascontiguousarray(q, dtype=np.uint64).reshape(16)
+    out = np.zeros(48, np.uint64)
+    _check(lib().zk_pairing(u64p(pp), u64p(qq), u64p(out)))
     return out

@@ -9,10 +9,12 @@
 #include "glv.hpp"
 #include "msm.hpp"
 #include "ntt.hpp"
+#include "pairing.hpp"
 #include "prover.hpp"
 #include "qap.hpp"
 #include "server.hpp"
 #include "synth.hpp"
+#include "verify.hpp"
 #include "zkey.hpp"
 
 struct zk_server;
@@ -880,3 +882,58 @@ int zk_synth_sizes(uint32_t m, uint32_t n_pub, uint32_t* n_vars, uint32_t* domai
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- verification
+static void check_points_words(const uint64_t* w, size_t n, const char* what) {
+    for (size_t i = 0; i < n; i++)
+        if (!zkh::Fp::is_canonical(w + 4 * i)) throw zk::Error(ZK_E_ARG, std::string(what) + " coordinate not below p");
+}
+
+int zk_verify(const zk_circuit* c, const uint64_t* publics, uint32_t n_public, const zk_proof* proof, int* ok) {
+    return guard([&] {
+        need(c && proof && ok && (publics || n_public == 0), "NULL argument");
+        check_points_words(&proof->pi_a[0], 8, "proof");
+        const zk::Circuit& ci = *c->impl;
+        *ok = zk::verify_proof(ci.hdr, ci.ic.data(), publics, n_public, *proof) ? 1 : 0;
+    });
+}
+
+int zk_verify_zkey(const char* zkey_path, const uint64_t* publics, uint32_t n_public, const zk_proof* proof, int* ok) {
+    return guard([&] {
+        need(zkey_path && proof && ok && (publics || n_public == 0), "NULL argument");
+        check_points_words(&proof->pi_a[0], 8, "proof");
+        zk::ZkeyView zv(zkey_path);
+        ZK_REQUIRE(zv.IC != nullptr, ZK_E_FORMAT, "zkey has no section 3 (IC)");
+        std::vector<uint64_t> ic((const uint64_t*)zv.IC, (const uint64_t*)zv.IC + 8ull * (zv.hdr.n_public + 1));
+        *ok = zk::verify_proof(zv.hdr, ic.data(), publics, n_public, *proof) ? 1 : 0;
+    });
+}
+
+int zk_verify_json(const char* zkey_path, const char* public_json, const char* proof_json, int* ok) {
+    return guard([&] {
+        need(zkey_path && public_json && proof_json && ok, "NULL argument");
+        const std::vector<uint64_t> pub = zk::parse_public_json(public_json);
+        const zk_proof p = zk::parse_proof_json(proof_json);
+        zk::ZkeyView zv(zkey_path);
+        ZK_REQUIRE(zv.IC != nullptr, ZK_E_FORMAT, "zkey has no section 3 (IC)");
+        std::vector<uint64_t> ic((const uint64_t*)zv.IC, (const uint64_t*)zv.IC + 8ull * (zv.hdr.n_public + 1));
+        *ok = zk::verify_proof(zv.hdr, ic.data(), pub.data(), (uint32_t)(pub.size() / 4), p) ? 1 : 0;
+    });
+}
+
+int zk_pairing(const uint64_t p[8], const uint64_t q[16], uint64_t out[48]) {
+    return guard([&] {
+        need(p && q && out, "NULL argument");
+        check_points_words(p, 2, "G1");
+        check_points_words(q, 4, "G2");
+        const zkh::Aff<zkh::Fp> P = zkh::g1_from_words(p);
+        const zkh::Aff<zkh::Fp2> Q = zkh::g2_from_words(q);
+        need(zkh::g1_on_curve(P) && zkh::g2_on_curve(Q), "point not on its curve");
+        const zkh::Fp12 e = zkh::pairing(P, Q);
+        const zkh::Fp2* c[6] = {&e.c0.c0, &e.c0.c1, &e.c0.c2, &e.c1.c0, &e.c1.c1, &e.c1.c2};
+        for (int k = 0; k < 6; k++) {
+            memcpy(out + 8 * k, c[k]->c0.v, 32);
+            memcpy(out + 8 * k + 4, c[k]->c1.v, 32);
+        }
+    });
+}

@@ -4,6 +4,10 @@
 // writes proof.json (snarkjs JSON, also printed) and public.json (w[1..nPublic]).
 // Optional: --blind <seed> uses snarkjs blinding with r, s derived from seed
 // (the reference proves unblinded, r = s = 0, which stays the default).
+//   zkprover verify <circuit.zkey> <public.json> <proof.json>
+// checks a proof with the pairing (the reference's `snarkjs groth16 verify`,
+// Makefile:135), reading the verification key from the zkey; prints OK or
+// INVALID and exits 0 or 2.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -17,7 +21,34 @@ static int fail(const char* what) {
     return 1;
 }
 
+static bool slurp(const char* path, std::string& out) {
+    FILE* f = fopen(path, "rb");
+    if (!f) return false;
+    char buf[65536];
+    size_t k;
+    while ((k = fread(buf, 1, sizeof buf, f)) > 0) out.append(buf, k);
+    fclose(f);
+    return true;
+}
+
+static int verify_main(int argc, char** argv) {
+    if (argc != 5) {
+        fprintf(stderr, "Usage:   %s verify <circuit.zkey> <public.json> <proof.json>\n", argv[0]);
+        return 1;
+    }
+    std::string pub, prf;
+    if (!slurp(argv[3], pub) || !slurp(argv[4], prf)) {
+        fprintf(stderr, "[zkprover] cannot read %s or %s\n", argv[3], argv[4]);
+        return 1;
+    }
+    int ok = 0;
+    if (zk_verify_json(argv[2], pub.c_str(), prf.c_str(), &ok) != ZK_OK) return fail("verify");
+    printf("%s\n", ok ? "OK" : "INVALID");
+    return ok ? 0 : 2;
+}
+
 int main(int argc, char** argv) {
+    if (argc >= 2 && strcmp(argv[1], "verify") == 0) return verify_main(argc, argv);
     if (argc < 5) {
         fprintf(stderr, "Usage:   %s <circuit.zkey> <witness.wtns> <proof.json> <public.json> [--blind SEED]\n", argv[0]);
         return 1;
