@@ -1,0 +1,85 @@
+// G2 bucket phase of the Pippenger MSM (design in msm.hpp): accumulation in
+// the 29-bit-limb form over Fp2 with the accumulator partly in LDS, base-slot
+// conversion, and the Fp2 instances of the shared chunk / reduction kernels
+// (msm_kernels.hpp).
+#include "msm_kernels.hpp"
+
+namespace zk {
+
+// G2 accumulation in the 29-bit-limb form: the Fp2 state does not fit the
+// register file next to the mixed add, so three coordinates of each thread's
+// bucket accumulator live in LDS (field29.hpp AccLds) and are loaded where
+// used. One-wave blocks let the LDS share set the waves per CU.
+constexpr int ACC29Q_THREADS = 64;
+__global__ void __launch_bounds__(ACC29Q_THREADS) __attribute__((amdgpu_waves_per_eu(2))) k_msm_accum29q(const uint32_t* __restrict__ entries,
+                                                                 const uint32_t* __restrict__ offsets,
+                                                                 const uint32_t* __restrict__ order, uint32_t nb,
+                                                                 uint32_t split, uint32_t lmax,
+                                                                 const void* __restrict__ points29,
+                                                                 uint32_t nbase, void* __restrict__ buckets) {
+    __shared__ uint32_t lds[AccLds<ACC29Q_THREADS>::LDS_WORDS];
+    uint32_t st, en, slot;
+    if (!acc_range<Fq2>(blockIdx.x * blockDim.x + threadIdx.x, nb, split, order, offsets, lmax, buckets, st, en, slot))
+        return;
+    AccLds<ACC29Q_THREADS> A{lds + threadIdx.x, {}};
+    bool inf = true;
+    for (uint32_t e = st; e < en; e++) {
+        uint32_t ref = entries[e];
+        F29q x, y;
+        f29q_load_point(points29, ref & 0x7fffffffu, nbase, x, y);
+        if (f29_is_zero(x.c0) && f29_is_zero(x.c1) && f29_is_zero(y.c0) && f29_is_zero(y.c1)) continue;
+        if (ref >> 31) y = {f29_neg(y.c0), f29_neg(y.c1)};
+        f29q_madd(A, inf, x, y);
+    }
+    using L = AccLds<ACC29Q_THREADS>;
+    Xyzz<Fq2> o = xyzz_inf<Fq2>();
+    if (!inf) o = {f29q_to_std(A.ld(L::X)), f29q_to_std(A.ld(L::Y)), f29q_to_std(A.ld(L::ZZ)), f29q_to_std(A.ld(L::ZZZ))};
+    PointIO<Fq2>::store_xyzz(buckets, slot, o);
+}
+
+// packed words of the 29-bit form of x (std Montgomery), at w[0..7]
+ZK_DEV void pack29(const Fp& x, uint32_t* w) { f29_pack(f29_from_std(x), w); }
+
+// base slots of msm_points_to29 (layout in field29.hpp); infinity stays all zero
+__global__ void k_points_to29q(const void* __restrict__ pts, uint32_t n, void* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Aff<Fq2> a = PointIO<Fq2>::load_aff(pts, i);
+    uint32_t w[48] = {};
+    if (!aff_is_inf(a)) {
+        const Fp b = beta_mont(glv::BETA_G2_MONT);
+        pack29(a.y.c0, w);
+        pack29(a.y.c1, w + 8);
+        pack29(a.x.c0, w + 16);
+        pack29(a.x.c1, w + 24);
+        pack29(mul<FpCfg>(a.x.c0, b), w + 32);
+        pack29(mul<FpCfg>(a.x.c1, b), w + 40);
+    }
+    uint4* o = reinterpret_cast<uint4*>(out) + PT29Q_SLOT_UINT4 * i;
+#pragma unroll
+    for (int k = 0; k < PT29Q_SLOT_UINT4; k++)
+        o[k] = k < 12 ? make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]) : make_uint4(0u, 0u, 0u, 0u);
+}
+
+template <>
+void launch_accum<Fq2>(const MsmSort& sort, uint32_t nb, int split, const void* d_points29, void* buckets, hipStream_t st) {
+    const uint32_t nt = nb * (uint32_t)split;
+    k_msm_accum29q<<<cdiv(nt, ACC29Q_THREADS), ACC29Q_THREADS, 0, st>>>(
+        sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(), sort.order.as<uint32_t>(), nb, (uint32_t)split,
+        sort.lmax, d_points29, (uint32_t)sort.n, buckets);
+}
+
+void msm_launch_pre_g2(const MsmSort& sort, const void* d_points, MsmScratch& scr, hipStream_t st, int split) {
+    launch_pre_t<Fq2>(sort, d_points, scr, st, split);
+}
+
+void msm_launch_main_g2(const MsmSort& sort, MsmScratch& scr, MsmPartials& out, hipStream_t st, const void* d_points29) {
+    launch_main_t<Fq2>(sort, d_points29, scr, out, st);
+}
+
+void msm_points_to29_g2(const void* d_points, size_t n, void* d_out, hipStream_t st) {
+    k_points_to29q<<<cdiv(n, 256), 256, 0, st>>>(d_points, (uint32_t)n, d_out);
+    ZK_HIP(hipGetLastError());
+}
+
+}  // namespace zk

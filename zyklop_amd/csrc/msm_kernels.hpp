@@ -1,0 +1,308 @@
+// Device code shared by the per-curve MSM translation units (msm_g1.hip,
+// msm_g2.hip): constants of the bucket phase, the oversized-bucket chunk path,
+// the two reduction levels, and the launch sequence of one MSM's bucket phase,
+// templated on the coordinate field. msm.hip holds the digit sort and the
+// host finish.
+#pragma once
+#include <algorithm>
+#include <type_traits>
+
+#include "curve.hpp"
+#include "field29.hpp"
+#include "glv.hpp"
+#include "msm.hpp"
+
+namespace zk {
+
+constexpr uint32_t L_MIN_LARGE = 64;  // floor of the oversized-bucket threshold
+constexpr uint32_t CHUNK = 256 * 16;  // entries per chunk of an oversized bucket
+constexpr uint32_t L1_MAX = 8;        // buckets per level-1 reduction thread (fewer if K is smaller)
+constexpr int L2 = 4;                 // level-1 segments per level-2 thread
+constexpr int LOG_L2 = 2;
+template <class T> struct RedBlk;
+template <> struct RedBlk<Fp> { static constexpr int V = 256; };
+template <> struct RedBlk<Fq2> { static constexpr int V = 128; };
+
+struct ChunkRec { uint32_t bucket, start, len, pad; };
+struct LargeRec { uint32_t bucket, first, count, pad; };
+
+// oversized-bucket threshold floor, chunk size and level sizes are shared with
+// the sort (msm.hip), which builds the chunk lists
+
+// ---------------------------------------------------------------- buckets
+// Bucket accumulation, G1. One thread per bucket, in the sort's accumulation
+// order (longest first, so a wave's lanes carry near-equal loads and the
+// launch has no long tail); buckets above lmax entries go to the chunked path.
+// Arithmetic in the 29-bit-limb form (field29.hpp): points come from the
+// converted copy of the bases, and each bucket is converted back to the
+// standard form when it is stored.
+// Thread tid of an accumulation launch over nb buckets, each split into
+// `split` parts of consecutive entries (bucket b's parts go to slots
+// b*split + h, summed by the level-1 reduction). Returns false when the thread
+// has nothing to accumulate: past the end, or a bucket above lmax entries.
+// Such a bucket is summed by the chunked path before this launch
+// (k_msm_chunks, k_msm_merge into slot b*split); its other parts store
+// infinity here. Otherwise [st, en) are the thread's entries and slot its
+// bucket slot.
+template <class T>
+ZK_DEV bool acc_range(uint32_t tid, uint32_t nb, uint32_t split, const uint32_t* __restrict__ order,
+                      const uint32_t* __restrict__ offsets, uint32_t lmax, void* __restrict__ buckets, uint32_t& st,
+                      uint32_t& en, uint32_t& slot) {
+    if (tid >= nb * split) return false;
+    const uint32_t ob = tid / split, h = tid - ob * split;
+    const uint32_t b = order[ob];
+    const uint32_t s0 = offsets[b], e0 = offsets[b + 1];
+    const uint32_t cnt = e0 - s0;
+    slot = b * split + h;
+    if (cnt > lmax) {
+        if (h != 0) PointIO<T>::store_xyzz(buckets, slot, xyzz_inf<T>());
+        return false;
+    }
+    st = s0 + (uint32_t)(((uint64_t)cnt * h) / split);
+    en = s0 + (uint32_t)(((uint64_t)cnt * (h + 1)) / split);
+    return true;
+}
+
+// endomorphism constant (Montgomery, 8 x u32) as a field element
+ZK_DEV Fp beta_mont(const uint32_t* c) {
+    Fp b;
+#pragma unroll
+    for (int k = 0; k < 8; k++) b.v[k] = c[k];
+    return b;
+}
+
+// base i of an MSM over the 2n points of the GLV split: P_i for i < n, else
+// lambda*P_(i-n) = (beta*x, y) (glv.hpp)
+template <class T>
+ZK_DEV Aff<T> load_base(const void* pts, uint32_t i, uint32_t n);
+template <>
+ZK_DEV Aff<Fp> load_base<Fp>(const void* pts, uint32_t i, uint32_t n) {
+    if (i < n) return PointIO<Fp>::load_aff(pts, i);
+    Aff<Fp> a = PointIO<Fp>::load_aff(pts, i - n);
+    if (aff_is_inf(a)) return a;
+    a.x = mul<FpCfg>(a.x, beta_mont(glv::BETA_G1_MONT));
+    return a;
+}
+template <>
+ZK_DEV Aff<Fq2> load_base<Fq2>(const void* pts, uint32_t i, uint32_t n) {
+    if (i < n) return PointIO<Fq2>::load_aff(pts, i);
+    Aff<Fq2> a = PointIO<Fq2>::load_aff(pts, i - n);
+    if (aff_is_inf(a)) return a;
+    const Fp b = beta_mont(glv::BETA_G2_MONT);
+    a.x.c0 = mul<FpCfg>(a.x.c0, b);
+    a.x.c1 = mul<FpCfg>(a.x.c1, b);
+    return a;
+}
+
+// tree-sum of one XYZZ per thread across the block (blockDim.x must be a power of 2)
+template <class T>
+__device__ Xyzz<T> block_sum(Xyzz<T> v, Xyzz<T>* sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    for (int h = blockDim.x >> 1; h >= 1; h >>= 1) {
+        __syncthreads();
+        if (t < h) sh[t] = xyzz_add(sh[t], sh[t + h]);
+    }
+    __syncthreads();
+    Xyzz<T> r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// one workgroup per chunk of an oversized bucket (grid-stride over the chunk list)
+template <class T>
+__global__ void __launch_bounds__(256) k_msm_chunks(const uint32_t* __restrict__ entries, const void* __restrict__ points,
+                                                    uint32_t nbase, const uint32_t* __restrict__ counters,
+                                                    const ChunkRec* __restrict__ chunks, void* __restrict__ partials) {
+    __shared__ Xyzz<T> sh[256];
+    const uint32_t nch = counters[0];
+    for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
+        ChunkRec cr = chunks[ci];
+        Xyzz<T> acc = xyzz_inf<T>();
+        for (uint32_t e = threadIdx.x; e < cr.len; e += blockDim.x) {
+            uint32_t ref = entries[cr.start + e];
+            Aff<T> P = load_base<T>(points, ref & 0x7fffffffu, nbase);
+            if (aff_is_inf(P)) continue;
+            if (ref >> 31) P.y = PointIO<T>::neg_y(P.y);
+            xyzz_madd(acc, P);
+        }
+        Xyzz<T> s = block_sum<T>(acc, sh);
+        if (threadIdx.x == 0) PointIO<T>::store_xyzz(partials, ci, s);
+    }
+}
+
+// one workgroup per oversized bucket: sum its chunk partials
+template <class T>
+__global__ void __launch_bounds__(256) k_msm_merge(const uint32_t* __restrict__ counters, const LargeRec* __restrict__ large,
+                                                   const void* __restrict__ partials, uint32_t split,
+                                                   void* __restrict__ buckets) {
+    __shared__ Xyzz<T> sh[256];
+    const uint32_t nl = counters[1];
+    for (uint32_t li = blockIdx.x; li < nl; li += gridDim.x) {
+        LargeRec lr = large[li];
+        Xyzz<T> acc = xyzz_inf<T>();
+        for (uint32_t j = threadIdx.x; j < lr.count; j += blockDim.x)
+            acc = xyzz_add(acc, PointIO<T>::load_xyzz(partials, lr.first + j));
+        Xyzz<T> s = block_sum<T>(acc, sh);
+        if (threadIdx.x == 0) PointIO<T>::store_xyzz(buckets, (size_t)lr.bucket * split, s);
+    }
+}
+
+// ---------------------------------------------------------------- reduction
+// The point additions here are inlined, each loop body holding one or two
+// addition sites (level 2 selects its operands per step; the step index is
+// uniform across the wave): out-of-line calls would pass the 128/256-byte
+// operands through scratch memory on every step of these latency-bound chains.
+//
+// Level 1: thread j owns buckets [j*L1, (j+1)*L1) of one window; with bucket
+// b holding digit b+1: R_j = sum B_b, T_j = sum (b - j*L1 + 1) B_b, so the
+// window value is sum_j (T_j + j*L1*R_j). B_b is the sum of the bucket's
+// `split` slots from the accumulation.
+template <class T>
+__global__ void __launch_bounds__(128) k_msm_reduce1(const void* __restrict__ buckets, uint32_t K, int W, uint32_t L1,
+                                                     uint32_t split, void* __restrict__ R, void* __restrict__ Tt) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t nseg = K / L1;
+    if (j >= (uint32_t)W * nseg) return;
+    uint32_t w = j / nseg, jj = j - w * nseg;
+    size_t base = (size_t)w * K + (size_t)jj * L1;
+    Xyzz<T> run = xyzz_inf<T>(), acc = xyzz_inf<T>();
+#pragma unroll 1
+    for (int b = (int)L1 - 1; b >= 0; b--) {
+#pragma unroll 1
+        for (uint32_t h = 0; h < split; h++) run = xyzz_add_i(run, PointIO<T>::load_xyzz(buckets, (base + b) * split + h));
+        acc = xyzz_add_i(acc, run);
+    }
+    PointIO<T>::store_xyzz(R, j, run);
+    PointIO<T>::store_xyzz(Tt, j, acc);
+}
+
+// Level 2: input per window is M pairs (R_j, T_j) worth sum_j (T_j + j*U*R_j),
+// U = 2^logU. Block blk covers j in [blk*BLK*L2, (blk+1)*BLK*L2); thread t owns
+// L2 consecutive j. Per thread: run = sum R_j, acc = sum (j - j0) R_j,
+// tsum = sum T_j. With S_t = sum_{t' >= t} run_t' (suffix scan over the block),
+// sum_t t*run_t = sum_{t >= 1} S_t, so the block's value relative to j_blk is
+// sum_t V_t with V_t = tsum_t + U*(acc_t + L2*[t >= 1]*S_t): one tree sum.
+// Output per block: R'' = S_0 and T'' = sum_t V_t, so the window value is
+// sum_blk (T''_blk + blk * (U*L2*BLK) * R''_blk).
+template <class T, int BLK>
+__global__ void __launch_bounds__(BLK) k_msm_reduce2(const void* __restrict__ Rin, const void* __restrict__ Tin, uint32_t M,
+                                                     int logU, uint32_t nblk, void* __restrict__ Rout,
+                                                     void* __restrict__ Tout) {
+    __shared__ Xyzz<T> sh[BLK];
+    const int t = threadIdx.x;
+    const uint32_t w = blockIdx.x / nblk, blk = blockIdx.x - w * nblk;
+    const uint32_t j0 = (blk * BLK + t) * L2;
+    Xyzz<T> run = xyzz_inf<T>(), acc = xyzz_inf<T>(), tsum = xyzz_inf<T>();
+    // per item, highest j first: acc += run; run += R_j; tsum += T_j
+#pragma unroll 1
+    for (int step = 0; step < 3 * L2; step++) {
+        const int q = L2 - 1 - step / 3, k = step % 3;
+        const uint32_t j = j0 + q;
+        if (j >= M) continue;
+        Xyzz<T> x = k == 0 ? acc : (k == 1 ? run : tsum);
+        Xyzz<T> y = k == 0 ? run : PointIO<T>::load_xyzz(k == 1 ? Rin : Tin, (size_t)w * M + j);
+        Xyzz<T> r = xyzz_add_i(x, y);
+        if (k == 0) acc = r;
+        else if (k == 1) run = r;
+        else tsum = r;
+    }
+    // suffix scan of run across the block: S_t = sum_{t' >= t} run_t'
+    sh[t] = run;
+#pragma unroll 1
+    for (int d = 1; d < BLK; d <<= 1) {
+        __syncthreads();
+        Xyzz<T> mine = sh[t];
+        Xyzz<T> other = (t + d < BLK) ? sh[t + d] : xyzz_inf<T>();
+        __syncthreads();
+        sh[t] = xyzz_add_i(mine, other);
+    }
+    __syncthreads();
+    const Xyzz<T> S = sh[t], total_r = sh[0];
+    __syncthreads();
+    // V_t = tsum + U * (acc + L2 * [t >= 1] * S_t): doublings and additions
+    // through one site each
+    Xyzz<T> v = t >= 1 ? S : xyzz_inf<T>();
+#pragma unroll 1
+    for (int step = 0; step < LOG_L2 + 1 + logU + 1; step++) {
+        if (step < LOG_L2 || (step > LOG_L2 && step <= LOG_L2 + logU)) {
+            v = xyzz_dbl_i(v);
+        } else {
+            v = xyzz_add_i(v, step == LOG_L2 ? acc : tsum);
+        }
+    }
+    // tree sum of V over the block
+    sh[t] = v;
+#pragma unroll 1
+    for (int h = BLK >> 1; h >= 1; h >>= 1) {
+        __syncthreads();
+        if (t < h) sh[t] = xyzz_add_i(sh[t], sh[t + h]);
+    }
+    __syncthreads();
+    if (t == 0) {
+        PointIO<T>::store_xyzz(Rout, blockIdx.x, total_r);
+        PointIO<T>::store_xyzz(Tout, blockIdx.x, sh[0]);
+    }
+}
+
+// The chunked path of the oversized buckets (lists built by the sort): their
+// sums land in bucket slot b*split before the accumulation. Queued ahead of
+// the accumulation so that these short launches run while the GPU has room,
+// instead of waiting behind the large accumulation launches of other streams.
+template <class T>
+static void launch_pre_t(const MsmSort& sort, const void* d_points, MsmScratch& scr, hipStream_t st, int split) {
+    const bool g2 = std::is_same<T, Fq2>::value;
+    const MsmGeom g = msm_geom(sort.n, sort.g.c, g2);
+    scr.ensure(g, sort.n_chunk_max, g2, split);
+    scr.split = split;
+    k_msm_chunks<T><<<512, 256, 0, st>>>(sort.entries.as<uint32_t>(), d_points, (uint32_t)sort.n,
+                                          sort.counters.as<uint32_t>(), sort.chunks.as<ChunkRec>(), scr.partials.p);
+    ZK_HIP(hipGetLastError());
+    k_msm_merge<T><<<256, 256, 0, st>>>(sort.counters.as<uint32_t>(), sort.large.as<LargeRec>(), scr.partials.p,
+                                         (uint32_t)split, scr.buckets.p);
+    ZK_HIP(hipGetLastError());
+}
+
+// the bucket-accumulation kernel of curve T (msm_g1.hip / msm_g2.hip)
+template <class T>
+void launch_accum(const MsmSort& sort, uint32_t nb, int split, const void* d_points29, void* buckets, hipStream_t st);
+
+template <class T>
+static void launch_main_t(const MsmSort& sort, const void* d_points29, MsmScratch& scr, MsmPartials& out,
+                          hipStream_t st) {
+    const bool g2 = std::is_same<T, Fq2>::value;
+    const MsmGeom g = msm_geom(sort.n, sort.g.c, g2);
+    const int split = scr.split;
+    const uint32_t nb = (uint32_t)((size_t)g.W * g.K), nt = nb * (uint32_t)split;
+    if (scr.ev_acc0) ZK_HIP(hipEventRecord(scr.ev_acc0, st));
+    launch_accum<T>(sort, nb, split, d_points29, scr.buckets.p, st);
+    ZK_HIP(hipGetLastError());
+    if (scr.ev_acc1) ZK_HIP(hipEventRecord(scr.ev_acc1, st));
+    uint32_t nseg_all = (uint32_t)g.W * g.nseg;
+    k_msm_reduce1<T><<<cdiv(nseg_all, 128), 128, 0, st>>>(scr.buckets.p, g.K, g.W, g.l1, (uint32_t)split, scr.r1.p,
+                                                           scr.t1.p);
+    ZK_HIP(hipGetLastError());
+    constexpr int BLK = RedBlk<T>::V;
+    k_msm_reduce2<T, BLK><<<(unsigned)(g.W * g.nblk), BLK, 0, st>>>(scr.r1.p, scr.t1.p, g.nseg, ilog2(g.l1), g.nblk,
+                                                                     scr.r2.p, scr.t2.p);
+    ZK_HIP(hipGetLastError());
+    size_t xb = g2 ? 256 : 128;
+    size_t half = (size_t)g.W * g.nblk * xb;
+    out.g = g;
+    out.g2 = g2;
+    out.host.ensure(2 * half);
+    ZK_HIP(hipMemcpyAsync(out.host.p, scr.r2.p, half, hipMemcpyDeviceToHost, st));
+    ZK_HIP(hipMemcpyAsync((char*)out.host.p + half, scr.t2.p, half, hipMemcpyDeviceToHost, st));
+    if (scr.ev_end) ZK_HIP(hipEventRecord(scr.ev_end, st));
+}
+
+
+// per-curve entry points, defined in msm_g1.hip and msm_g2.hip
+void msm_launch_pre_g1(const MsmSort& sort, const void* d_points, MsmScratch& scr, hipStream_t st, int split);
+void msm_launch_main_g1(const MsmSort& sort, MsmScratch& scr, MsmPartials& out, hipStream_t st, const void* d_points29);
+void msm_points_to29_g1(const void* d_points, size_t n, void* d_out, hipStream_t st);
+void msm_launch_pre_g2(const MsmSort& sort, const void* d_points, MsmScratch& scr, hipStream_t st, int split);
+void msm_launch_main_g2(const MsmSort& sort, MsmScratch& scr, MsmPartials& out, hipStream_t st, const void* d_points29);
+void msm_points_to29_g2(const void* d_points, size_t n, void* d_out, hipStream_t st);
+
+}  // namespace zk
