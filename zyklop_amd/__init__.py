@@ -129,9 +129,14 @@ ABI = {
 }
 
 
-def build():
-    """Compile libzkp.so for gfx950 in-tree (hipcc cross-compiles without a GPU)."""
-    subprocess.check_call(["make", "-s", "-j8", "-C", os.path.join(_HERE, "csrc")])
+def build(incremental=False):
+    """Compile libzkp.so for gfx950 in-tree (hipcc cross-compiles without a GPU).
+    Every object is rebuilt unless incremental (then make's header dependencies
+    decide)."""
+    cmd = ["make", "-s", "-j8", "-C", os.path.join(_HERE, "csrc")]
+    if not incremental:
+        cmd.insert(1, "-B")
+    subprocess.check_call(cmd)
 
 
 def lib():
