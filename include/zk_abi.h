@@ -233,6 +233,16 @@ int zk_msm_g1_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, s
                      int* is_inf);
 int zk_msm_g2_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t n, int window, uint64_t out[16],
                      int* is_inf);
+/* zk_msm_g1_device without any allocation. No counterpart in the reference:
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Its whole device workspace
+ * is a fixed 64 MiB array that the library holds on every device it runs on
+ * (enough for 65536 points at the default window); larger workspaces are
+ * refused with ZK_E_ARG. The host partials go to a fixed host array. Calls
+ * on one device share both arrays: calls that overlap on one device race.
+ * Results are identical to zk_msm_g1_device's. */
+int zk_msm_g1_device_noalloc(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t n, int window,
+                             uint64_t out[8], int* is_inf);
 /* device time of the last kernel-level call on ctx (HIP events on its stream):
  * MSM: [0] total ms [1] digit sort ms [2] bucket-accumulation kernel ms
  *      [3] number of (point, bucket) entries accumulated;

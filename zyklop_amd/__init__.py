@@ -112,6 +112,8 @@ ABI = {
                                         ctypes.POINTER(ctypes.c_int)]),
     "zk_msm_g2_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_int, _u64p,
                                         ctypes.POINTER(ctypes.c_int)]),
+    "zk_msm_g1_device_noalloc": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_int, _u64p,
+                                                ctypes.POINTER(ctypes.c_int)]),
     "zk_last_timing": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
     "zk_circuit_bases": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_uint64)]),
     "zk_synth_circuit": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_vp)]),
@@ -271,6 +273,13 @@ class Context:
         out = np.zeros(8, np.uint64)
         inf = ctypes.c_int(0)
         _check(lib().zk_msm_g1_device(self.h, d_points, d_scalars, n, window, u64p(out), ctypes.byref(inf)))
+        return out, bool(inf.value)
+
+    def msm_g1_device_noalloc(self, d_points, d_scalars, n, window=0):
+        """G1 MSM on device buffers with the library's fixed scratch (no allocation)."""
+        out = np.zeros(8, np.uint64)
+        inf = ctypes.c_int(0)
+        _check(lib().zk_msm_g1_device_noalloc(self.h, d_points, d_scalars, n, window, u64p(out), ctypes.byref(inf)))
         return out, bool(inf.value)
 
     def msm_g2_device(self, d_points, d_scalars, n, window=0):

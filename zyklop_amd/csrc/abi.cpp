@@ -683,6 +683,16 @@ int zk_msm_g1_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, s
     });
 }
 
+int zk_msm_g1_device_noalloc(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t n, int window,
+                             uint64_t out[8], int* is_inf) {
+    return guard([&] {
+        need(ctx && out && (n == 0 || (d_points && d_scalars)), "NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        put_result(zk::msm_g1_noalloc(ctx->impl.device, d_points, d_scalars, n, window, ctx->impl.stream), out, is_inf);
+    });
+}
+
 int zk_msm_g2_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t n, int window, uint64_t out[16],
                      int* is_inf) {
     return guard([&] {

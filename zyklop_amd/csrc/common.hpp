@@ -32,21 +32,34 @@ struct Error : std::runtime_error {
         if (!(cond)) throw ::zk::Error((code), (msg));      \
     } while (0)
 
-// Owning device allocation.
+// Device allocation: owning (hipMalloc), or borrowed from memory the caller
+// manages (borrow()), which never grows and is never freed here.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool borrowed = false;
     DevBuf() = default;
     explicit DevBuf(size_t n) { alloc(n); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), borrowed(o.borrowed) { o.p = nullptr; o.bytes = 0; }
     DevBuf& operator=(DevBuf&& o) noexcept {
-        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        if (this != &o) {
+            release();
+            p = o.p; bytes = o.bytes; borrowed = o.borrowed;
+            o.p = nullptr; o.bytes = 0;
+        }
         return *this;
     }
     ~DevBuf() { release(); }
+    void borrow(void* q, size_t n) {
+        release();
+        p = q;
+        bytes = n;
+        borrowed = true;
+    }
     void alloc(size_t n) {
+        if (borrowed) throw Error(ZK_E_INTERNAL, "a borrowed device buffer cannot grow");
         release();
         if (n == 0) n = 256;
         hipError_t e = hipMalloc(&p, n);
@@ -62,9 +75,10 @@ struct DevBuf {
         if (n > bytes) alloc(n);
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !borrowed) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
+        borrowed = false;
     }
     template <class T> T* as() const { return reinterpret_cast<T*>(p); }
 };

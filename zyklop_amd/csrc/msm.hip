@@ -1,6 +1,7 @@
 // Pippenger MSM kernels for G1/G2 on gfx950 (see msm.hpp for the design).
 #include <algorithm>
 #include <future>
+#include <mutex>
 #include <type_traits>
 
 #include "msm_kernels.hpp"
@@ -436,6 +437,72 @@ void msm_launch(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& 
     msm_launch_main(sort, g2, scr, out, st, d_points29);
 }
 
+// ---------------------------------------------------------------- fixed-scratch G1 MSM
+// The whole device workspace of msm_g1_noalloc: allocated with the code
+// object on every device the library runs on, held until the process ends.
+static __device__ __align__(256) uint8_t scratch[MSM_NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+// host partials of the fixed-scratch MSM: [W][nblk] R then T, XYZZ (128 B) each
+static uint64_t g_noalloc_host[16384];
+
+void msm_noalloc_setup(int device) {
+    ZK_REQUIRE(device >= 0 && device < NOALLOC_MAX_DEVICES, ZK_E_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return;
+    void* ptr = nullptr;
+    // the host-side name of a __device__ array is not its device address
+    ZK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_noalloc_ptr[device] = ptr;
+}
+
+zkh::Xyzz<zkh::Fp> msm_g1_noalloc(int device, const void* d_points, const void* d_scalars, size_t n, int window,
+                                  hipStream_t st) {
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        if (device >= 0 && device < NOALLOC_MAX_DEVICES) base = g_noalloc_ptr[device];
+    }
+    ZK_REQUIRE(base != nullptr, ZK_E_STATE, "fixed MSM scratch not set up on this device");
+    const SortShape z = sort_shape(n, window);
+    const MsmGeom& g = z.g;
+    const int split = msm_default_split(n, g.c, false);
+    // the same buffers the allocating path sizes, laid out in the array
+    MsmSort sort;
+    MsmScratch scr;
+    DevBuf p29;
+    size_t need = 0;
+    auto count = [&](const DevBuf&, size_t b) { need = ((need + 255) & ~(size_t)255) + b; };
+    sort_buffers(sort, z, count);
+    scratch_buffers(scr, g, z.max_chunks, false, split, count);
+    count(p29, std::max<size_t>(n, 1) * PT29_BYTES);
+    ZK_REQUIRE(need <= MSM_NOALLOC_SCRATCH_BYTES, ZK_E_ARG,
+               "MSM of " + std::to_string(n) + " points at window " + std::to_string(g.c) + " needs " +
+                   std::to_string(need) + " bytes of workspace; the fixed scratch holds " +
+                   std::to_string(MSM_NOALLOC_SCRATCH_BYTES) + " (65536 points at the default window)");
+    const size_t host_need = 2 * (size_t)g.W * g.nblk * 128;
+    ZK_REQUIRE(host_need <= sizeof g_noalloc_host, ZK_E_ARG, "MSM window too large for the fixed host partials");
+    size_t used = 0;
+    auto take = [&](DevBuf& b, size_t bytes) {
+        used = (used + 255) & ~(size_t)255;
+        b.borrow((char*)base + used, bytes);
+        used += bytes;
+    };
+    sort_buffers(sort, z, take);
+    scratch_buffers(scr, g, z.max_chunks, false, split, take);
+    take(p29, std::max<size_t>(n, 1) * PT29_BYTES);
+    MsmPartials part;
+    part.ext = g_noalloc_host;
+    part.ext_bytes = sizeof g_noalloc_host;
+    if (n) msm_points_to29(d_points, n, p29.p, false, st);
+    sort.run(d_scalars, n, window, st);
+    msm_launch(sort, d_points, false, scr, part, st, p29.p, split);
+    ZK_HIP(hipStreamSynchronize(st));
+    return msm_finish_g1(part);
+}
+
 // ---------------------------------------------------------------- host finish
 template <class HT>
 static HT load_h(const uint64_t* p);
@@ -450,7 +517,7 @@ template <class HT>
 static zkh::Xyzz<HT> window_value(const MsmPartials& p, int w) {
     const MsmGeom& g = p.g;
     const int lw = sizeof(HT) / 8;  // u64 words per coordinate
-    const uint64_t* R = p.host.as<uint64_t>();
+    const uint64_t* R = p.data();
     const uint64_t* Tt = R + (size_t)g.W * g.nblk * 4 * lw;
     const int logU = ilog2(g.l1) + ilog2(L2) + ilog2(p.g2 ? RedBlk<Fq2>::V : RedBlk<Fp>::V);
     auto ld = [&](const uint64_t* base, size_t i) {

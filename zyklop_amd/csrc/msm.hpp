@@ -74,6 +74,10 @@ struct MsmPartials {
     MsmGeom g{};
     bool g2 = false;
     HostBuf host;       // [W][nblk] R then [W][nblk] T, XYZZ each
+    // caller-owned host memory used instead of `host` when set (ext_bytes long)
+    void* ext = nullptr;
+    size_t ext_bytes = 0;
+    const uint64_t* data() const { return static_cast<const uint64_t*>(ext ? ext : host.p); }
 };
 
 // Device scratch for the bucket phase of one MSM.
@@ -120,6 +124,19 @@ void msm_points_to29(const void* d_points, size_t n, void* d_out, bool g2, hipSt
 
 // size out's host buffer for an MSM of geometry g (no launch)
 void msm_reserve_partials(MsmPartials& out, const MsmGeom& g, bool g2);
+
+// G1 MSM whose whole device workspace (29-bit base copies, digit sort,
+// buckets, reductions) lives in a fixed per-device array allocated at load
+// time, and whose host partials go to a fixed host array: no allocation of
+// any kind. msm_noalloc_setup(device) resolves the array's device address
+// once per device (with that device current); msm_g1_noalloc then runs on
+// stream st and waits for it. Sizes whose workspace exceeds the array are
+// rejected (ZK_E_ARG). Calls on one device share the array: overlapping calls
+// race.
+constexpr size_t MSM_NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+void msm_noalloc_setup(int device);
+zkh::Xyzz<zkh::Fp> msm_g1_noalloc(int device, const void* d_points, const void* d_scalars, size_t n, int window,
+                                  hipStream_t st);
 
 // Host finish: combine the partials into the MSM value.
 zkh::Xyzz<zkh::Fp> msm_finish_g1(const MsmPartials& p);

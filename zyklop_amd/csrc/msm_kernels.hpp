@@ -290,9 +290,15 @@ static void launch_main_t(const MsmSort& sort, const void* d_points29, MsmScratc
     size_t half = (size_t)g.W * g.nblk * xb;
     out.g = g;
     out.g2 = g2;
-    out.host.ensure(2 * half);
-    ZK_HIP(hipMemcpyAsync(out.host.p, scr.r2.p, half, hipMemcpyDeviceToHost, st));
-    ZK_HIP(hipMemcpyAsync((char*)out.host.p + half, scr.t2.p, half, hipMemcpyDeviceToHost, st));
+    void* dst = out.ext;
+    if (dst) {
+        ZK_REQUIRE(out.ext_bytes >= 2 * half, ZK_E_INTERNAL, "MSM partials exceed the caller's host buffer");
+    } else {
+        out.host.ensure(2 * half);
+        dst = out.host.p;
+    }
+    ZK_HIP(hipMemcpyAsync(dst, scr.r2.p, half, hipMemcpyDeviceToHost, st));
+    ZK_HIP(hipMemcpyAsync((char*)dst + half, scr.t2.p, half, hipMemcpyDeviceToHost, st));
     if (scr.ev_end) ZK_HIP(hipEventRecord(scr.ev_end, st));
 }
 

@@ -25,6 +25,7 @@ Context::Context(int dev) : device(dev) {
     ZK_REQUIRE(dev >= 0 && dev < count, ZK_E_ARG, "no such HIP device: " + std::to_string(dev));
     ZK_HIP(hipSetDevice(dev));
     ZK_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    msm_noalloc_setup(dev);   // device address of the fixed MSM scratch, once per device
 }
 
 Context::~Context() {
