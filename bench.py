@@ -25,7 +25,6 @@ proofs of the same inputs):
 Prints one JSON line.
 """
 import argparse
-import concurrent.futures as cf
 import ctypes
 import json
 import math
@@ -84,27 +83,66 @@ def r3(x):
     return None if x is None else round(float(x), 3)
 
 
-def cpu_baseline(m_cpu, n_pub_cpu, m, n_pub):
-    """Oracle (GMP, single thread) prove of a bounded sample, scaled to the metric."""
+def cpu_info():
+    model = None
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model,
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
+def cpu_baseline(m_cpu, n_pub_cpu, m, n_pub, threads):
+    """Oracle (GMP) proves on the host: the small circuit on one thread and on
+    all threads, then the metric's own circuit on all threads (measured, not
+    scaled). CRS setup (parallel) is untimed."""
     import oracle as O
 
+    info = cpu_info()
+    all_thr = O.set_threads(threads if threads > 0 else (info["affinity"] or info["nproc"] or 1))
     t0 = time.time()
     c = O.Circuit(99, m_cpu, n_pub_cpu, with_crs=True)
-    setup = time.time() - t0
+    setup_small = time.time() - t0
+    r, s = c.blinding()
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    c.prove(r, s)
+    t_small_1 = time.perf_counter() - t0
+    O.set_threads(all_thr)
+    t0 = time.perf_counter()
+    c.prove(r, s)
+    t_small_all = time.perf_counter() - t0
+    del c
+    log(f"[bench] cpu: m=2^{int(math.log2(m_cpu))}: 1 thread {t_small_1:.2f} s, {all_thr} threads {t_small_all:.2f} s")
+    t0 = time.time()
+    c = O.Circuit(98, m, n_pub, with_crs=True)
+    setup_big = time.time() - t0
     r, s = c.blinding()
     t0 = time.perf_counter()
     c.prove(r, s)
-    dt = time.perf_counter() - t0
+    t_big_all = time.perf_counter() - t0
+    del c
+    log(f"[bench] cpu: m=2^{int(math.log2(m))}: {all_thr} threads {t_big_all:.2f} s (setup {setup_big:.1f} s)")
     ratio = proof_work(m_cpu, n_pub_cpu) / proof_work(m, n_pub)
     return {
-        "value": round((1.0 / dt) * ratio, 6),
+        "value": round(1.0 / t_big_all, 6),
         "unit": "proofs/s",
-        "cores": 1,
+        "cores": all_thr,
         "kind": "port",
-        "sample": (f"one oracle prove (GMP, 1 thread) of the m=2^{int(math.log2(m_cpu))}, {n_pub_cpu}-input "
-                   f"synthetic circuit took {dt:.2f} s (CRS setup {setup:.1f} s untimed); scaled to m=2^"
-                   f"{int(math.log2(m))} by the MSM work ratio {ratio:.4f}"),
-        "measured_proofs_per_s": round(1.0 / dt, 6),
+        "sample": (f"one oracle prove (GMP, OpenMP over MSM windows, NTT butterflies, SpMV rows) of the "
+                   f"m=2^{int(math.log2(m))}, {n_pub}-input synthetic circuit on {all_thr} threads: "
+                   f"{t_big_all:.2f} s (CRS setup {setup_big:.1f} s untimed)"),
+        "host": info,
+        "single_thread": {"m": m_cpu, "n_public": n_pub_cpu, "seconds": round(t_small_1, 3),
+                          "proofs_per_s": round(1.0 / t_small_1, 6),
+                          "scaled_to_metric_proofs_per_s": round(ratio / t_small_1, 6),
+                          "scale_note": f"MSM work ratio {ratio:.5f} (F_G1/F_G2 formulas)"},
+        "all_threads_small": {"m": m_cpu, "seconds": round(t_small_all, 3), "threads": all_thr},
+        "setup_s": {"small": round(setup_small, 1), "metric": round(setup_big, 1)},
     }
 
 
@@ -432,7 +470,12 @@ def section_mixed(args, zk, ctx):
 
 
 # ---------------------------------------------------------------- served
-def section_served(args, zk, ctx, pool_futures):
+def section_served(args, zk, ctx):
+    """Poisson arrivals against the resident circuits through zk_server_*. Every
+    request carries a fresh witness (public inputs from a fresh seed, the aux
+    wires computed on the device, zk_circuit_witness_device) and fresh r, s.
+    Slots are sized for every circuit up front (zk_server_prepare), so no
+    request allocates device memory while others run."""
     n_pub = 1024
     sizes = args.serve_log_m
     circs = {}
@@ -445,32 +488,45 @@ def section_served(args, zk, ctx, pool_futures):
             t0 = time.time()
             tw = ctx.autotune(circs[lg])
             log(f"[bench] served: 2^{lg} windows {tw} tuned in {time.time() - t0:.1f} s")
-    pools = {lg: upload_all(ctx, pool_futures[lg].result()) for lg in sizes}
+    # per circuit, a ring of device witness buffers refilled per request
+    ring = {lg: [ctx.dev_alloc(circs[lg].n_vars * 32) for _ in range(args.serve_ring)] for lg in sizes}
+    gen_ms = {}
+    for lg in sizes:
+        t0 = time.perf_counter()
+        circs[lg].witness_device(71000 + lg, ring[lg][0])
+        gen_ms[lg] = (time.perf_counter() - t0) * 1e3
+    log(f"[bench] served: device witness generation ms {({k: round(v, 1) for k, v in gen_ms.items()})}")
     # plain capacity: one proof at a time per circuit
     svc = {}
     for lg in sizes:
         r, s = zk.synth_blinding(80000 + lg)
-        ctx.prove_device(circs[lg], pools[lg][0], r, s)
+        ctx.prove_device(circs[lg], ring[lg][0], r, s)
         ts = []
         for i in range(args.serve_svc_reps):
             t0 = time.perf_counter()
-            ctx.prove_device(circs[lg], pools[lg][i % len(pools[lg])], r, s)
+            ctx.prove_device(circs[lg], ring[lg][0], r, s)
             ts.append(time.perf_counter() - t0)
         svc[lg] = float(np.mean(ts))
     capacity = len(sizes) / sum(svc.values())          # requests/s, uniform circuit choice, one at a time
     log(f"[bench] served: plain service times (ms) {({k: round(v * 1e3, 1) for k, v in svc.items()})}, "
         f"capacity {capacity:.2f} req/s")
     srv = zk.Server(ctx, max_slots=args.serve_slots, reserve_bytes=args.serve_reserve_gib << 30)
+    t0 = time.time()
+    n_slots = srv.prepare([circs[lg] for lg in sizes])
+    log(f"[bench] served: {n_slots} slots prepared for all circuits in {time.time() - t0:.1f} s")
     rng = np.random.default_rng(args.seed + 5)
     rates = []
-    last = None
+    check = {}
     for frac in args.serve_loads:
         rate = frac * capacity
         N = args.serve_requests
         arrivals = np.cumsum(rng.exponential(1.0 / rate, N))
         which = rng.integers(0, len(sizes), N)
         picks = [(sizes[int(q)], int(rng.integers(0, 1 << 30))) for q in which]
-        tickets = []
+        results = {}
+        owner = {lg: [None] * args.serve_ring for lg in sizes}   # ticket that last used each buffer
+        nxt = {lg: 0 for lg in sizes}
+        tickets, late = [], 0
         t_base = time.perf_counter()
         for k in range(N):
             while True:
@@ -478,17 +534,29 @@ def section_served(args, zk, ctx, pool_futures):
                 if dt <= 0:
                     break
                 time.sleep(min(dt, 0.002))
+            if t_base + arrivals[k] - time.perf_counter() < -0.005:
+                late += 1
             lg, seed = picks[k]
+            j = nxt[lg]
+            nxt[lg] = (j + 1) % args.serve_ring
+            prev = owner[lg][j]
+            if prev is not None and prev not in results:
+                results[prev] = srv.wait(prev)                  # buffer still read by an earlier request
+            d = ring[lg][j]
+            circs[lg].witness_device(seed, d)
             r, s = zk.synth_blinding(seed)
-            d = pools[lg][seed % len(pools[lg])]
-            tickets.append((srv.submit_device(circs[lg], d, r, s), lg, d, r, s))
-        lat, queued, by, results = [], [], {lg: [] for lg in sizes}, []
-        for t, lg, d, r, s in tickets:
-            pr, tm = srv.wait(t)
+            t = srv.submit_device(circs[lg], d, r, s)
+            owner[lg][j] = t
+            tickets.append((t, lg, seed))
+        lat, queued, by = [], [], {lg: [] for lg in sizes}
+        for t, lg, seed in tickets:
+            if t not in results:
+                results[t] = srv.wait(t)
+            pr, tm = results[t]
             lat.append(tm[2])
             queued.append(tm[0])
             by[lg].append(tm[2])
-            results.append((pr, lg, d, r, s))
+            check.setdefault(lg, (pr, seed))
         wall = time.perf_counter() - t_base
         st = srv.stats()
         rates.append({
@@ -498,31 +566,35 @@ def section_served(args, zk, ctx, pool_futures):
             "p50_ms_by_log_m": {str(lg): r3(pct(v, 50)) for lg, v in by.items() if v},
             "p99_ms_by_log_m": {str(lg): r3(pct(v, 99)) for lg, v in by.items() if v},
             "peak_in_flight": int(st["peak_in_flight"]), "slots": int(st["slots"]),
+            "late_arrivals": late,
         })
         log(f"[bench] served load {frac:.2f} ({rate:.2f} req/s): {N / wall:.2f} proofs/s, p50 {pct(lat, 50):.1f} ms, "
-            f"p99 {pct(lat, 99):.1f} ms, peak in flight {int(st['peak_in_flight'])}")
-        last = results
+            f"p99 {pct(lat, 99):.1f} ms, peak in flight {int(st['peak_in_flight'])}, {late} late arrivals")
     failed = int(srv.stats()["failed"])
     srv.close()
-    # served results against plain proofs of the same inputs (one per circuit size)
-    same, done = True, set()
-    for pr, lg, d, r, s in last:
-        if lg in done:
-            continue
-        done.add(lg)
+    # one served result per circuit against the plain proof of the same inputs,
+    # and against the pairing check
+    same = verified = True
+    for lg, (pr, seed) in check.items():
+        d = ring[lg][0]
+        circs[lg].witness_device(seed, d)
+        r, s = zk.synth_blinding(seed)
         same &= bool(np.array_equal(pr.words(), ctx.prove_device(circs[lg], d, r, s).words()))
+        w_pub = ctx.download(d, circs[lg].n_vars * 32).reshape(-1, 4)[1:n_pub + 1]
+        verified &= zk.verify(circs[lg], w_pub, pr)
     for lg in sizes:
-        for p in pools[lg]:
+        for p in ring[lg]:
             ctx.dev_free(p)
         circs[lg].free()
     return {
         "circuits_log_m": sizes, "n_public": n_pub,
-        "witness_pool_per_circuit": {str(lg): len(pool_futures[lg].result()) for lg in sizes},
+        "witnesses": "fresh per request: public inputs from the request's seed, aux wires on the device",
+        "witness_generation_ms": {str(lg): r3(v) for lg, v in gen_ms.items()},
         "plain_service_ms": {str(lg): r3(v * 1e3) for lg, v in svc.items()},
         "plain_capacity_req_per_s": round(capacity, 3),
-        "rates": rates, "identical_to_plain": same, "failed": failed, "max_slots": args.serve_slots,
-        "reserve_gib": args.serve_reserve_gib,
-        "arrivals": "Poisson, circuit uniform over the four sizes, fresh r, s per request",
+        "rates": rates, "identical_to_plain": same, "pairing_verified": verified, "failed": failed,
+        "max_slots": args.serve_slots, "slots_prepared": n_slots, "reserve_gib": args.serve_reserve_gib,
+        "arrivals": "Poisson, circuit uniform over the sizes, fresh witness and r, s per request",
     }
 
 
@@ -538,6 +610,7 @@ def main():
     ap.add_argument("--witnesses", type=int, default=4)
     ap.add_argument("--cpu-log-m", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: every core this process may use)")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--probe-reps", type=int, default=5)
     ap.add_argument("--sections", default="headline,mixed,served",
@@ -546,7 +619,8 @@ def main():
     ap.add_argument("--mixed-slots", type=int, default=4)
     ap.add_argument("--serve-log-m", type=lambda s: [int(x) for x in s.split(",")], default=[18, 20, 22, 24])
     ap.add_argument("--serve-loads", type=lambda s: [float(x) for x in s.split(",")], default=[0.4, 0.7, 1.0])
-    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-ring", type=int, default=8, help="device witness buffers per served circuit")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-reserve-gib", type=int, default=8)
     ap.add_argument("--serve-svc-reps", type=int, default=2)
@@ -560,14 +634,6 @@ def main():
     import zyklop_amd as zk
 
     sections = set(args.sections.split(","))
-    # witnesses of the served circuits are generated on host threads while the GPU sections run
-    pool_futures, ex = {}, None
-    if "served" in sections:
-        ex = cf.ThreadPoolExecutor(max_workers=4)
-        for lg in args.serve_log_m:
-            k = 2 if lg <= 20 else 1
-            pool_futures[lg] = ex.submit(
-                lambda lg=lg, k=k: [zk.synth_witness(70000 + lg, 1 << lg, 1024, 71000 + lg * 10 + j) for j in range(k)])
 
     ctx = zk.Context(0)
     ctx.set_batch_slots(args.slots)
@@ -605,11 +671,10 @@ def main():
     if "mixed" in sections:
         result["mixed_batch"] = section_mixed(args, zk, ctx)
     if "served" in sections:
-        result["served"] = section_served(args, zk, ctx, pool_futures)
-        ex.shutdown()
+        result["served"] = section_served(args, zk, ctx)
     if not args.no_cpu:
-        log(f"[bench] cpu baseline: oracle prove at m=2^{args.cpu_log_m} ...")
-        result["cpu_baseline"] = cpu_baseline(1 << args.cpu_log_m, 1024, m, n_pub)
+        log(f"[bench] cpu baseline: oracle proves at m=2^{args.cpu_log_m} and 2^{args.log_m} ...")
+        result["cpu_baseline"] = cpu_baseline(1 << args.cpu_log_m, 1024, m, n_pub, args.cpu_threads)
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)

@@ -261,6 +261,10 @@ int zk_synth_witness(uint64_t seed, uint32_t m, uint32_t n_pub, uint64_t pub_see
 /* satisfying witness of a circuit built by zk_synth_circuit, public inputs
  * drawn from pub_seed's stream (n_vars x 4 words); reuses the resident R1CS */
 int zk_circuit_witness(const zk_circuit* c, uint64_t pub_seed, uint64_t* w_out);
+/* the same witness computed on the device into d_w (n_vars x 32 bytes,
+ * canonical), the constraint rows level by level in dependency order; returns
+ * when d_w is written. For a fresh witness per request at any circuit size. */
+int zk_circuit_witness_device(zk_ctx* ctx, const zk_circuit* c, uint64_t pub_seed, void* d_w);
 int zk_synth_blinding(uint64_t seed, uint64_t r[4], uint64_t s[4]);
 int zk_synth_uniform(uint64_t seed, size_t count, uint64_t* out);
 /* n_vars / domain of the synthetic circuit (m, n_pub) */

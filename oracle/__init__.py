@@ -65,6 +65,7 @@ def lib():
         L.or_g1_progression.argtypes = [_u64p, _u64p, ctypes.c_size_t, _u64p]
         L.or_fr_poly_eval.argtypes = [_u64p, ctypes.c_size_t, _u64p, ctypes.c_size_t, _u64p]
         L.or_fr_dot.argtypes = [_u64p, _u64p, ctypes.c_size_t, _u64p]
+        L.or_set_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -162,6 +163,11 @@ def g2_gen():
     g = np.zeros(16, dtype=np.uint64)
     lib().or_g2_generator(ptr(g))
     return g
+
+
+def set_threads(n=0):
+    """Threads of the oracle's parallel loops (n <= 0 keeps the setting); returns the count in effect."""
+    return lib().or_set_threads(n)
 
 
 def g1_progression(k0, d, n):

@@ -213,16 +213,17 @@ static void fe2_inv(fe2* r, const fe2* a) {
             }                                                                                 \
         return acc;                                                                           \
     }                                                                                         \
-    /* Pippenger with unsigned c-bit windows over canonical scalars < 2^256 */                \
+    /* Pippenger with unsigned c-bit windows over canonical scalars < 2^256;  \
+     * the window sums are independent and run in parallel */                               \
     static PFX##_jac PFX##_msm(const PFX##_aff* pts, const fe* sc, size_t n) {                \
         int c = 1;                                                                            \
         while (c < 16 && ((size_t)1 << (c + 2)) < n) c++;                                     \
         size_t nb = ((size_t)1 << c) - 1;                                                     \
-        PFX##_jac* bk = (PFX##_jac*)malloc(sizeof(PFX##_jac) * nb);                           \
-        PFX##_jac total = PFX##_jac_inf();                                                    \
         int nw = (256 + c - 1) / c;                                                           \
-        for (int w = nw - 1; w >= 0; w--) {                                                   \
-            for (int d = 0; d < c; d++) total = PFX##_dbl(&total);                            \
+        PFX##_jac* S = (PFX##_jac*)malloc(sizeof(PFX##_jac) * nw);                            \
+        _Pragma("omp parallel for schedule(dynamic, 1)")                                      \
+        for (int w = 0; w < nw; w++) {                                                        \
+            PFX##_jac* bk = (PFX##_jac*)malloc(sizeof(PFX##_jac) * nb);                       \
             for (size_t b = 0; b < nb; b++) bk[b] = PFX##_jac_inf();                          \
             for (size_t i = 0; i < n; i++) {                                                  \
                 if (pts[i].inf) continue;                                                     \
@@ -241,9 +242,15 @@ static void fe2_inv(fe2* r, const fe2* a) {
                 run = PFX##_add(&run, &bk[b]);                                                \
                 sum = PFX##_add(&sum, &run);                                                  \
             }                                                                                 \
-            total = PFX##_add(&total, &sum);                                                  \
+            S[w] = sum;                                                                       \
+            free(bk);                                                                         \
         }                                                                                     \
-        free(bk);                                                                             \
+        PFX##_jac total = PFX##_jac_inf();                                                    \
+        for (int w = nw - 1; w >= 0; w--) {                                                   \
+            for (int d = 0; d < c; d++) total = PFX##_dbl(&total);                            \
+            total = PFX##_add(&total, &S[w]);                                                 \
+        }                                                                                     \
+        free(S);                                                                              \
         return total;                                                                         \
     }
 
@@ -481,20 +488,28 @@ static void ntt_core(fe* a, int log_n, const fe* w) {
         j ^= bit;
         if (i < j) { fe t = a[i]; a[i] = a[j]; a[j] = t; }
     }
+    enum { TW_CHUNK = 4096 };
     for (int s = 1; s <= log_n; s++) {
         size_t len = (size_t)1 << s, half = len >> 1;
         fe wl = *w;                                 /* w_len = w^(n/len) */
         for (int k = s; k < log_n; k++) RMUL(&wl, &wl, &wl);
         fe* tw = (fe*)malloc(sizeof(fe) * half);
-        tw[0] = fe_u64(1);
-        for (size_t k = 1; k < half; k++) RMUL(&tw[k], &tw[k - 1], &wl);
-        for (size_t i = 0; i < n; i += len)
-            for (size_t k = 0; k < half; k++) {
-                fe u = a[i + k], v;
-                RMUL(&v, &a[i + k + half], &tw[k]);
-                RADD(&a[i + k], &u, &v);
-                RSUB(&a[i + k + half], &u, &v);
-            }
+        /* tw[k] = wl^k, in chunks that start from a power */
+#pragma omp parallel for schedule(static)
+        for (size_t c0 = 0; c0 < half; c0 += TW_CHUNK) {
+            fe e = fe_u64(c0);
+            f_pow(&tw[c0], &wl, &e, &R_MOD);
+            size_t c1 = c0 + TW_CHUNK < half ? c0 + TW_CHUNK : half;
+            for (size_t k = c0 + 1; k < c1; k++) RMUL(&tw[k], &tw[k - 1], &wl);
+        }
+#pragma omp parallel for schedule(static)
+        for (size_t t = 0; t < n / 2; t++) {
+            size_t k = t & (half - 1), i = (t >> (s - 1)) << s;
+            fe u = a[i + k], v;
+            RMUL(&v, &a[i + k + half], &tw[k]);
+            RADD(&a[i + k], &u, &v);
+            RSUB(&a[i + k + half], &u, &v);
+        }
         free(tw);
     }
 }
@@ -509,6 +524,7 @@ static int ntt_canon(fe* a, int log_n, int kind) {
         f_inv(&wi, &w, &R_MOD);
         f_inv(&ninv, &nn, &R_MOD);
         ntt_core(a, log_n, &wi);
+#pragma omp parallel for schedule(static)
         for (size_t i = 0; i < n; i++) RMUL(&a[i], &a[i], &ninv);
         if (kind == 3) {
             fe g, gi = fe_u64(1);
@@ -585,9 +601,11 @@ int or_spmv(const uint32_t* row_ptr, const uint32_t* cols, const uint64_t* vals,
 static void hpoly_canon(fe* a, fe* b, int log_n, fe* h) {
     size_t n = (size_t)1 << log_n;
     fe* c = (fe*)malloc(sizeof(fe) * n);
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < n; i++) RMUL(&c[i], &a[i], &b[i]);
     ntt_canon(a, log_n, 1); ntt_canon(b, log_n, 1); ntt_canon(c, log_n, 1);
     ntt_canon(a, log_n, 2); ntt_canon(b, log_n, 2); ntt_canon(c, log_n, 2);
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < n; i++) {
         fe t; RMUL(&t, &a[i], &b[i]); RSUB(&h[i], &t, &c[i]);
     }
@@ -812,6 +830,7 @@ or_circuit* or_circuit_create(uint64_t seed, uint32_t m, uint32_t n_pub, int wit
         c->IC = (g1_aff*)malloc(sizeof(g1_aff) * (n_pub + 1));
         c->C = (g1_aff*)malloc(sizeof(g1_aff) * (c->n_vars - n_pub - 1 + 1));
         c->H = (g1_aff*)malloc(sizeof(g1_aff) * n);
+#pragma omp parallel for schedule(dynamic, 256)
         for (uint32_t i = 0; i < c->n_vars; i++) {
             c->A[i] = g1_comb_mul(t1, &c->u[i]);
             c->B1[i] = g1_comb_mul(t1, &c->v[i]);
@@ -824,6 +843,7 @@ or_circuit* or_circuit_create(uint64_t seed, uint32_t m, uint32_t n_pub, int wit
         /* H_j = L^(2n)_{2j+1}(tau) / delta */
         fe* lag2 = (fe*)malloc(sizeof(fe) * 2 * n);
         lagrange_all(lag2, &c->tau, (int)ln + 1);
+#pragma omp parallel for schedule(dynamic, 256)
         for (uint32_t j = 0; j < n; j++) { fe t; RMUL(&t, &lag2[2 * j + 1], &di); c->H[j] = g1_comb_mul(t1, &t); }
         free(lag2);
         free(t1); free(t2);
@@ -920,6 +940,7 @@ int or_prove(const or_circuit* c, const uint64_t rr[4], const uint64_t ss[4], ui
     /* SpMV through the zkey convention: raw = c*R^2, frMul(raw, w) */
     fe* aw = (fe*)calloc(n, sizeof(fe));
     fe* bw = (fe*)calloc(n, sizeof(fe));
+#pragma omp parallel for schedule(static)
     for (uint32_t i = 0; i < c->m; i++)
         for (int q = 0; q < 3; q++) {
             fe raw, t;
@@ -934,6 +955,7 @@ int or_prove(const or_circuit* c, const uint64_t rr[4], const uint64_t ss[4], ui
         RADD(&aw[c->m + k], &aw[c->m + k], &t);
     }
     /* Montgomery -> canonical, H pipeline */
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < n; i++) { f_from_mont(&aw[i], &aw[i], &R_MOD); f_from_mont(&bw[i], &bw[i], &R_MOD); }
     fe* h = (fe*)malloc(sizeof(fe) * n);
     hpoly_canon(aw, bw, (int)c->log_n, h);
@@ -1160,4 +1182,11 @@ int or_fr_dot(const uint64_t* a, const uint64_t* b, size_t n, uint64_t out[4]) {
     memcpy(out, tot.v, 32);
     free(part);
     return 0;
+}
+
+/* worker threads of the parallel loops (OpenMP); 0 leaves the current setting */
+#include <omp.h>
+int or_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
 }

@@ -109,6 +109,10 @@ int or_fr_poly_eval(const uint64_t* coeffs, size_t n, const uint64_t* zs, size_t
 /* sum_i a_i * b_i mod r, canonical */
 int or_fr_dot(const uint64_t* a, const uint64_t* b, size_t n, uint64_t out[4]);
 
+/* threads of the parallel loops (MSM windows, NTT butterflies, SpMV rows, CRS
+ * points); n <= 0 keeps the setting. Returns the count now in effect. */
+int or_set_threads(int n);
+
 /* decimal string of a canonical 256-bit value */
 void or_to_dec(const uint64_t a[4], char* buf, size_t buf_size);
 

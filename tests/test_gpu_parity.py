@@ -351,3 +351,20 @@ def test_prove_2pow16_closed_form(ctx):
     got = ctx.prove(circ, w, r, s).words()
     assert np.array_equal(got, ref.prove_trapdoor(r, s))
     assert ref.verify(got, r, s)
+
+
+@pytest.mark.parametrize("seed,m,n_pub", [(5, 37, 3), (6, 2000, 8), (7, 1 << 16, 1024)])
+def test_device_witness_equals_host(ctx, seed, m, n_pub):
+    """zk_circuit_witness_device (rows level by level on the GPU) == the host
+    generator, for the seed's public inputs and a fresh stream of them."""
+    circ = ctx.synth_circuit(seed, m, n_pub)
+    d = ctx.dev_alloc(circ.n_vars * 32)
+    try:
+        for pub_seed in (seed, 4242 + seed):
+            circ.witness_device(pub_seed, d)
+            got = ctx.download(d, circ.n_vars * 32).reshape(-1, 4)
+            assert np.array_equal(got, circ.witness(pub_seed)), pub_seed
+        assert np.array_equal(got[:1 + n_pub], zk.synth_witness(seed, m, n_pub, 4242 + seed)[:1 + n_pub])
+    finally:
+        ctx.dev_free(d)
+    circ.free()

@@ -119,6 +119,7 @@ ABI = {
     "zk_synth_circuit": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_vp)]),
     "zk_synth_witness": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, _u64p]),
     "zk_circuit_witness": (ctypes.c_int, [_vp, ctypes.c_uint64, _u64p]),
+    "zk_circuit_witness_device": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, _vp]),
     "zk_synth_blinding": (ctypes.c_int, [ctypes.c_uint64, _u64p, _u64p]),
     "zk_synth_uniform": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_size_t, _u64p]),
     "zk_glv_split": (ctypes.c_int, [_u64p, ctypes.c_size_t, _u64p]),
@@ -521,6 +522,10 @@ class Circuit:
         w = np.zeros((self.n_vars, 4), np.uint64)
         _check(lib().zk_circuit_witness(self.h, pub_seed, u64p(w)))
         return w
+
+    def witness_device(self, pub_seed, d_w):
+        """The same witness computed on the device into d_w (n_vars x 32 bytes)."""
+        _check(lib().zk_circuit_witness_device(self.ctx.h, self.h, pub_seed, d_w))
 
 
 # ---- reference-API mirror (include/groth16.cuh) ----

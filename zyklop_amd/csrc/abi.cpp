@@ -857,6 +857,15 @@ int zk_circuit_witness(const zk_circuit* c, uint64_t pub_seed, uint64_t* w_out) 
     });
 }
 
+int zk_circuit_witness_device(zk_ctx* ctx, const zk_circuit* c, uint64_t pub_seed, void* d_w) {
+    return guard([&] {
+        need(ctx && c && d_w, "NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        zk::circuit_witness_device(*c->impl, pub_seed, d_w, ctx->impl.stream);
+    });
+}
+
 int zk_synth_blinding(uint64_t seed, uint64_t r[4], uint64_t s[4]) {
     return guard([&] {
         need(r && s, "NULL argument");

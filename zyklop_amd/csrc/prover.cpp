@@ -93,7 +93,7 @@ void Context::ensure_comb() {
 // ------------------------------------------------------------------ circuits
 size_t Circuit::device_bytes() const {
     return rowA.bytes + colA.bytes + valA.bytes + rowB.bytes + colB.bytes + valB.bytes + A.bytes + B1.bytes + B2.bytes +
-           C.bytes + H.bytes + A29.bytes + B1_29.bytes + B2_29.bytes + C29.bytes + H29.bytes;
+           C.bytes + H.bytes + A29.bytes + B1_29.bytes + B2_29.bytes + C29.bytes + H29.bytes + wit_rows.bytes;
 }
 
 static void upload(DevBuf& d, const void* src, size_t bytes) {
@@ -218,8 +218,47 @@ std::unique_ptr<Circuit> circuit_synthetic(Context& ctx, uint64_t seed, uint32_t
                  (const uint64_t*)(h1.data() + ex.off_ic * 64) + 8ull * (n_pub + 1));
     finish_circuit_bases(*c, h1.data() + ex.off_a * 64, h1.data() + ex.off_b1 * 64, h2.data() + ex.off_b2 * 128,
                          h1.data() + ex.off_c * 64, h1.data() + ex.off_h * 64);
+    // dependency levels of the constraint rows (device witness generator)
+    {
+        const uint32_t first_aux = 1 + n_pub;
+        std::vector<uint32_t> lev(m);
+        uint32_t top = 0;
+        for (uint32_t i = 0; i < m; i++) {
+            uint32_t mx = 0;
+            for (int q = 0; q < 3; q++) {
+                const uint32_t wa = r1.a_wire[3ull * i + q], wb = r1.b_wire[3ull * i + q];
+                if (wa >= first_aux) mx = std::max(mx, lev[wa - first_aux]);
+                if (wb >= first_aux) mx = std::max(mx, lev[wb - first_aux]);
+            }
+            lev[i] = mx + 1;
+            top = std::max(top, lev[i]);
+        }
+        c->wit_levels.assign(top + 2, 0);
+        for (uint32_t i = 0; i < m; i++) c->wit_levels[lev[i] + 1]++;
+        for (uint32_t L = 1; L <= top + 1; L++) c->wit_levels[L] += c->wit_levels[L - 1];
+        std::vector<uint32_t> pos(c->wit_levels.begin(), c->wit_levels.end() - 1), rows(m);
+        for (uint32_t i = 0; i < m; i++) rows[pos[lev[i]]++] = i;
+        upload(c->wit_rows, rows.data(), (size_t)m * 4);
+    }
     c->synth = std::make_unique<SynthR1cs>(std::move(r1));
     return c;
+}
+
+void circuit_witness_device(const Circuit& c, uint64_t pub_seed, void* d_w, hipStream_t st) {
+    ZK_REQUIRE(c.synth != nullptr, ZK_E_STATE, "circuit was not built by zk_synth_circuit");
+    const SynthR1cs& r1 = *c.synth;
+    // wire 0 and the public inputs from the host, the aux wires level by level
+    std::vector<uint64_t> head(4ull * (1 + r1.l), 0);
+    head[0] = 1;
+    const std::vector<Fr> pub = synth_public(pub_seed, r1.l);
+    for (uint32_t k = 0; k < r1.l; k++) pub[k].canonical(head.data() + 4ull * (1 + k));
+    ZK_HIP(hipMemcpyAsync(d_w, head.data(), head.size() * 8, hipMemcpyHostToDevice, st));
+    const uint32_t* rows = c.wit_rows.as<uint32_t>();
+    for (size_t L = 1; L + 1 < c.wit_levels.size(); L++)
+        witness_rows(c.rowA.as<uint32_t>(), c.colA.as<uint32_t>(), c.valA.p, c.rowB.as<uint32_t>(),
+                     c.colB.as<uint32_t>(), c.valB.p, rows + c.wit_levels[L], c.wit_levels[L + 1] - c.wit_levels[L],
+                     1 + r1.l, d_w, st);
+    ZK_HIP(hipStreamSynchronize(st));   // the host copy of wire 0 and the inputs is freed on return
 }
 
 void circuit_write_zkey(const Circuit& c, const std::string& path) {

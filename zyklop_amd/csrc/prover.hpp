@@ -47,6 +47,11 @@ struct Circuit {
     std::vector<uint8_t> coefs;   // §4 bytes, kept to write the circuit back out
     std::vector<uint64_t> ic;     // (l+1) x 8
     std::unique_ptr<SynthR1cs> synth;  // host R1CS of synthetic circuits (witness generation)
+    // synthetic circuits: constraint rows grouped by dependency level (a row's
+    // level is one more than the highest level among the aux wires it reads),
+    // levels[L]..levels[L+1] indexing wit_rows, for the device witness generator
+    DevBuf wit_rows;
+    std::vector<uint32_t> wit_levels;
     size_t device_bytes() const;
 };
 
@@ -180,6 +185,10 @@ struct BatchGraph {
 };
 
 std::unique_ptr<Circuit> circuit_from_zkey(Context& ctx, const std::string& path);
+// satisfying witness of a synthetic circuit with public inputs from pub_seed's
+// stream (the same values as synth_witness_from_public), computed on the
+// device into d_w (n_vars x 32 B canonical) level by level on stream st
+void circuit_witness_device(const Circuit& c, uint64_t pub_seed, void* d_w, hipStream_t st);
 std::unique_ptr<Circuit> circuit_synthetic(Context& ctx, uint64_t seed, uint32_t m, uint32_t n_pub);
 void circuit_write_zkey(const Circuit& c, const std::string& path);
 

@@ -32,6 +32,34 @@ void spmv_abc(const uint32_t* rowA, const uint32_t* colA, const void* valA, uint
     ZK_HIP(hipGetLastError());
 }
 
+// One dependency level of the synthetic witness: aux_i = (A_i.w)(B_i.w) for
+// the rows rows[0..count) (every wire they read is already final). The CSR
+// values are raw c*R^2, so the row sums land in Montgomery form; their product
+// times R^-1 is the Montgomery form of aux_i, stored canonical.
+__global__ void k_witness_rows(const uint32_t* __restrict__ rowA, const uint32_t* __restrict__ colA,
+                               const void* __restrict__ valA, const uint32_t* __restrict__ rowB,
+                               const uint32_t* __restrict__ colB, const void* __restrict__ valB,
+                               const uint32_t* __restrict__ rows, uint32_t count, uint32_t first_aux,
+                               void* __restrict__ w) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const uint32_t i = rows[t];
+    Fr a = Fr::zero(), b = Fr::zero();
+    for (uint32_t k = rowA[i], e = rowA[i + 1]; k < e; k++)
+        a = add<FrCfg>(a, mul<FrCfg>(load<FrCfg>(valA, k), load<FrCfg>(w, colA[k])));
+    for (uint32_t k = rowB[i], e = rowB[i + 1]; k < e; k++)
+        b = add<FrCfg>(b, mul<FrCfg>(load<FrCfg>(valB, k), load<FrCfg>(w, colB[k])));
+    store<FrCfg>(w, (size_t)first_aux + i, from_mont<FrCfg>(mul<FrCfg>(a, b)));
+}
+
+void witness_rows(const uint32_t* rowA, const uint32_t* colA, const void* valA, const uint32_t* rowB,
+                  const uint32_t* colB, const void* valB, const uint32_t* rows, uint32_t count, uint32_t first_aux,
+                  void* w, hipStream_t st) {
+    if (!count) return;
+    k_witness_rows<<<cdiv(count, 256), 256, 0, st>>>(rowA, colA, valA, rowB, colB, valB, rows, count, first_aux, w);
+    ZK_HIP(hipGetLastError());
+}
+
 __global__ void k_pointwise_mul(const void* __restrict__ a, const void* __restrict__ b, void* __restrict__ c, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;

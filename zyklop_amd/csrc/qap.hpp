@@ -30,6 +30,13 @@ void write_scalars4(void* dst, const uint64_t v[16], hipStream_t st);
 // Output affine Montgomery (G1 64 B / G2 128 B), infinity as zeros.
 void fixed_base_mul(const void* table, const void* exps, size_t n, void* out, bool g2, hipStream_t st);
 
+// Synthetic witness, one dependency level: w[first_aux + i] = canonical
+// (A_i.w)(B_i.w) for each row i in rows[0..count), CSR values raw c*R^2, w
+// canonical (the wires the rows read must already hold their final values).
+void witness_rows(const uint32_t* rowA, const uint32_t* colA, const void* valA, const uint32_t* rowB,
+                  const uint32_t* colB, const void* valB, const uint32_t* rows, uint32_t count, uint32_t first_aux,
+                  void* w, hipStream_t st);
+
 // fill a u32 buffer (capture-safe alternative to hipMemsetAsync)
 void fill_u32(void* p, uint32_t v, size_t count, hipStream_t st);
 
