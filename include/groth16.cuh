@@ -101,8 +101,9 @@ inline void groth16ReadZkeyHeader(const char* zkey_path, Groth16ZkeyHeader& hdr)
     memcpy(hdr.prime, h.prime, sizeof hdr.prime);
 }
 
-// coefs: entries of the selected matrix as Groth16CoefEntry (value = c*R^2,
-// the zkey's raw encoding); witness canonical; out canonical, zero-padded.
+// coefs: entries as Groth16CoefEntry with canonical values (the reference
+// header's contract, :93-98), entries of other matrices skipped; witness
+// canonical; out canonical, zero-padded to domain_size.
 inline void groth16SparseMvm(const Groth16CoefEntry* coefs, size_t n_coefs, uint8_t matrix_sel, const uint64_t* witness,
                              uint32_t n_vars, uint64_t* out, uint32_t domain_size) {
     std::vector<uint8_t> raw(44 * n_coefs);
@@ -113,6 +114,6 @@ inline void groth16SparseMvm(const Groth16CoefEntry* coefs, size_t n_coefs, uint
         memcpy(&raw[44 * i + 8], &coefs[i].signal, 4);
         memcpy(&raw[44 * i + 12], coefs[i].value, 32);
     }
-    if (zk_sparse_mvm(raw.data(), n_coefs, matrix_sel, witness, n_vars, out, domain_size) != ZK_OK)
+    if (zk_sparse_mvm_canonical(raw.data(), n_coefs, matrix_sel, witness, n_vars, out, domain_size) != ZK_OK)
         groth16_detail::die("groth16SparseMvm");
 }

@@ -195,11 +195,18 @@ int zk_read_zkey_header(const char* zkey_path, zk_zkey_header* out);
 int zk_read_wtns(const char* wtns_path, uint64_t* w, uint32_t cap, uint32_t* n_out);
 int zk_write_wtns(const char* wtns_path, const uint64_t* w, uint32_t n_vars);
 
-/* groth16SparseMvm (include/groth16.cuh:197-205; declared but never defined
- * by the reference): host CSR-free form over 44-byte §4 entries, canonical
- * witness in, canonical result out (domain_size x 4 words). */
+/* groth16SparseMvm (include/groth16.cuh:183-205; declared but never defined
+ * by the reference): out = M * w on the host for matrix_sel (0 = A, 1 = B,
+ * 2 = C) over 44-byte entries {u32 matrix, u32 row, u32 wire, 32-byte value},
+ * entries of other matrices skipped; witness canonical in, result canonical
+ * out (domain_size x 4 words, zero-padded). zk_sparse_mvm_canonical takes
+ * canonical values, the reference header's contract (:93-98); zk_sparse_mvm
+ * takes the zkey §4 encoding c*R^2 mod r, so §4 bytes can be passed as they
+ * are. Values and witness must be below r (ZK_E_ARG). */
 int zk_sparse_mvm(const uint8_t* coefs44, size_t n_coefs, uint8_t matrix_sel, const uint64_t* witness,
                   uint32_t n_vars, uint64_t* out, uint32_t domain_size);
+int zk_sparse_mvm_canonical(const uint8_t* coefs44, size_t n_coefs, uint8_t matrix_sel, const uint64_t* witness,
+                            uint32_t n_vars, uint64_t* out, uint32_t domain_size);
 
 /* ---- kernel-level entry points (host buffers), used by parity tests ----
  * NTT (include/ntt_bn254.cuh:418 ntt): data n = 2^log_n Montgomery Fr, in

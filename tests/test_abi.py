@@ -220,3 +220,81 @@ def test_glv_split_identity_and_bounds():
     with pytest.raises(zk.ZkError) as e:
         zk.glv_split(O.arr([R]))
     assert e.value.code == 1
+
+
+def _sec4(zkey):
+    data = open(zkey, "rb").read()
+    pos, n_sec = 12, int.from_bytes(data[8:12], "little")
+    for _ in range(n_sec):
+        t = int.from_bytes(data[pos:pos + 4], "little")
+        sz = int.from_bytes(data[pos + 4:pos + 12], "little")
+        if t == 4:
+            sec4 = data[pos + 12:pos + 12 + sz]
+            n = int.from_bytes(sec4[:4], "little")
+            return sec4[4:4 + 44 * n], n
+        pos += 12 + sz
+    raise AssertionError("no section 4")
+
+
+def test_sparse_mvm_canonical_contract(small_circuit, tmp_path):
+    """groth16SparseMvm's documented contract (canonical coefficient values,
+    include/groth16.cuh of the reference, :93-98 and :183-196), through the C
+    ABI and through this repository's C++ header compiled by g++."""
+    import subprocess
+    c, zkey, wtns = small_circuit
+    coefs, ncoef = _sec4(zkey)
+    rinv2 = pow(pyref.RM * pyref.RM, -1, pyref.R)
+    entries = []
+    for k in range(ncoef):
+        e = coefs[44 * k:44 * k + 44]
+        mat, row, wire = (int.from_bytes(e[4 * i:4 * i + 4], "little") for i in range(3))
+        entries.append((mat, row, wire, int.from_bytes(e[12:44], "little") * rinv2 % pyref.R))
+    canon = b"".join(m.to_bytes(4, "little") + r.to_bytes(4, "little") + w.to_bytes(4, "little") + v.to_bytes(32, "little")
+                     for m, r, w, v in entries)
+    w = c.witness()
+    wi = [O.from_limbs(x) for x in w]
+    src = tmp_path / "mvm.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include <vector>
+#include "groth16.cuh"
+int main() {
+    size_t n; unsigned nv, dom, sel;
+    if (scanf("%zu %u %u %u", &n, &nv, &dom, &sel) != 4) return 2;
+    std::vector<Groth16CoefEntry> e(n);
+    for (auto& x : e) {
+        unsigned m;
+        if (scanf("%u %u %u %lu %lu %lu %lu", &m, &x.constraint, &x.signal, &x.value[0], &x.value[1], &x.value[2],
+                  &x.value[3]) != 7) return 2;
+        x.matrix = (uint8_t)m;
+    }
+    std::vector<uint64_t> w(4ull * nv), out(4ull * dom);
+    for (auto& x : w) if (scanf("%lu", &x) != 1) return 2;
+    groth16SparseMvm(e.data(), n, (uint8_t)sel, w.data(), nv, out.data(), dom);
+    for (auto x : out) printf("%lu\n", x);
+}
+''')
+    exe = tmp_path / "mvm"
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-I", os.path.join(root, "include"), str(src), "-o", str(exe),
+                           "-L", os.path.join(root, "zyklop_amd"), "-lzkp",
+                           "-Wl,-rpath," + os.path.join(root, "zyklop_amd")])
+    for sel in (0, 1):
+        acc = [0] * c.n
+        for m, r, wire, v in entries:
+            if m == sel:
+                acc[r] = (acc[r] + v * wi[wire]) % pyref.R
+        out = zk.sparse_mvm(canon, sel, w, c.n_vars, c.n, canonical=True)
+        assert [O.from_limbs(x) for x in out] == acc
+        lines = [f"{len(entries)} {c.n_vars} {c.n} {sel}"]
+        lines += [f"{m} {r} {wire} " + " ".join(str(x) for x in O.to_limbs(v)) for m, r, wire, v in entries]
+        lines += [" ".join(str(int(x)) for x in row) for row in w]
+        res = subprocess.run([str(exe)], input="\n".join(lines), capture_output=True, text=True, timeout=60)
+        assert res.returncode == 0, res.stderr
+        words = [int(x) for x in res.stdout.split()]
+        assert [O.from_limbs(words[4 * i:4 * i + 4]) for i in range(c.n)] == acc
+    # a value not below r is an argument error, as is the raw encoding's check
+    bad = canon[:12] + (pyref.R + 1).to_bytes(32, "little") + canon[44:]
+    with pytest.raises(zk.ZkError) as e:
+        zk.sparse_mvm(bad, 0, w, c.n_vars, c.n, canonical=True)
+    assert e.value.code == 1

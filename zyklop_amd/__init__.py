@@ -96,6 +96,8 @@ ABI = {
     "zk_write_wtns": (ctypes.c_int, [ctypes.c_char_p, _u64p, ctypes.c_uint32]),
     "zk_sparse_mvm": (ctypes.c_int, [_u8p, ctypes.c_size_t, ctypes.c_uint8, _u64p, ctypes.c_uint32, _u64p,
                                      ctypes.c_uint32]),
+    "zk_sparse_mvm_canonical": (ctypes.c_int, [_u8p, ctypes.c_size_t, ctypes.c_uint8, _u64p, ctypes.c_uint32, _u64p,
+                                               ctypes.c_uint32]),
     "zk_ntt": (ctypes.c_int, [_vp, _u64p, ctypes.c_int, ctypes.c_int]),
     "zk_msm_g1": (ctypes.c_int, [_vp, _u64p, _u64p, ctypes.c_size_t, ctypes.c_int, _u64p, ctypes.POINTER(ctypes.c_int)]),
     "zk_msm_g2": (ctypes.c_int, [_vp, _u64p, _u64p, ctypes.c_size_t, ctypes.c_int, _u64p, ctypes.POINTER(ctypes.c_int)]),
@@ -584,13 +586,15 @@ def write_wtns(path, w):
     _check(lib().zk_write_wtns(path.encode(), u64p(a), a.size // 4))
 
 
-def sparse_mvm(coefs44, matrix_sel, witness, n_vars, domain_size):
-    """groth16SparseMvm (include/groth16.cuh:197-205): canonical in, canonical out."""
+def sparse_mvm(coefs44, matrix_sel, witness, n_vars, domain_size, canonical=False):
+    """groth16SparseMvm (include/groth16.cuh:183-205): 44-byte entries whose values
+    are the zkey's c*R^2 encoding, or canonical c (the header's contract) with
+    canonical=True; canonical witness in, canonical result out."""
     c = np.frombuffer(coefs44, dtype=np.uint8)
     w = np.ascontiguousarray(witness, dtype=np.uint64)
     out = np.zeros((domain_size, 4), np.uint64)
-    _check(lib().zk_sparse_mvm(c.ctypes.data_as(_u8p), c.size // 44, matrix_sel, u64p(w), n_vars, u64p(out),
-                               domain_size))
+    fn = lib().zk_sparse_mvm_canonical if canonical else lib().zk_sparse_mvm
+    _check(fn(c.ctypes.data_as(_u8p), c.size // 44, matrix_sel, u64p(w), n_vars, u64p(out), domain_size))
     return out
 
 

@@ -601,23 +601,41 @@ int zk_write_wtns(const char* wtns_path, const uint64_t* w, uint32_t n_vars) {
     });
 }
 
+namespace {
+// out = M_sel * w over 44-byte entries {u32 matrix, u32 row, u32 wire, 32 B
+// value}; value_raw: the zkey's c*R^2 encoding, else canonical c
+void sparse_mvm(const uint8_t* coefs44, size_t n_coefs, uint8_t matrix_sel, const uint64_t* witness, uint32_t n_vars,
+                uint64_t* out, uint32_t domain_size, bool value_raw) {
+    need(coefs44 && witness && out, "NULL argument");
+    need(matrix_sel <= 2, "matrix_sel must be 0, 1 or 2");
+    for (uint32_t i = 0; i < n_vars; i++) need(zkh::Fr::is_canonical(witness + 4ull * i), "witness value not canonical");
+    const zkh::Fr r2 = zkh::Fr::from_raw(zkh::FrP::R2);
+    std::vector<zkh::Fr> acc(domain_size, zkh::Fr::zero());
+    for (size_t i = 0; i < n_coefs; i++) {
+        const uint8_t* e = coefs44 + 44 * i;
+        uint32_t mat, row, wire;
+        memcpy(&mat, e, 4); memcpy(&row, e + 4, 4); memcpy(&wire, e + 8, 4);
+        if (mat != matrix_sel) continue;
+        ZK_REQUIRE(row < domain_size && wire < n_vars, ZK_E_FORMAT, "coefficient index out of range");
+        uint64_t v[4];
+        memcpy(v, e + 12, 32);
+        ZK_REQUIRE(zkh::Fr::is_canonical(v), ZK_E_ARG, "coefficient value not canonical (>= r)");
+        // raw c*R^2: frMul(c R^2, w) = (c w) R; canonical c: (c R) from c * R^2 * R^-1 first
+        const zkh::Fr c = value_raw ? zkh::Fr::from_raw(v) : zkh::Fr::from_raw(v) * r2 * r2;
+        acc[row] = acc[row] + c * zkh::Fr::from_raw(witness + 4ull * wire);
+    }
+    for (uint32_t i = 0; i < domain_size; i++) acc[i].canonical(out + 4ull * i);
+}
+}  // namespace
+
 int zk_sparse_mvm(const uint8_t* coefs44, size_t n_coefs, uint8_t matrix_sel, const uint64_t* witness, uint32_t n_vars,
                   uint64_t* out, uint32_t domain_size) {
-    return guard([&] {
-        need(coefs44 && witness && out, "NULL argument");
-        need(matrix_sel <= 2, "matrix_sel must be 0, 1 or 2");
-        std::vector<zkh::Fr> acc(domain_size, zkh::Fr::zero());
-        for (size_t i = 0; i < n_coefs; i++) {
-            const uint8_t* e = coefs44 + 44 * i;
-            uint32_t mat, row, wire;
-            memcpy(&mat, e, 4); memcpy(&row, e + 4, 4); memcpy(&wire, e + 8, 4);
-            if (mat != matrix_sel) continue;
-            ZK_REQUIRE(row < domain_size && wire < n_vars, ZK_E_FORMAT, "coefficient index out of range");
-            // frMul(c R^2, w) = (c w) R
-            acc[row] = acc[row] + zkh::Fr::from_raw((const uint64_t*)(e + 12)) * zkh::Fr::from_raw(witness + 4ull * wire);
-        }
-        for (uint32_t i = 0; i < domain_size; i++) acc[i].canonical(out + 4ull * i);
-    });
+    return guard([&] { sparse_mvm(coefs44, n_coefs, matrix_sel, witness, n_vars, out, domain_size, true); });
+}
+
+int zk_sparse_mvm_canonical(const uint8_t* coefs44, size_t n_coefs, uint8_t matrix_sel, const uint64_t* witness,
+                            uint32_t n_vars, uint64_t* out, uint32_t domain_size) {
+    return guard([&] { sparse_mvm(coefs44, n_coefs, matrix_sel, witness, n_vars, out, domain_size, false); });
 }
 
 int zk_ntt(zk_ctx* ctx, uint64_t* data, int log_n, int kind) {
@@ -860,9 +878,9 @@ int zk_circuit_witness(const zk_circuit* c, uint64_t pub_seed, uint64_t* w_out) 
 int zk_circuit_witness_device(zk_ctx* ctx, const zk_circuit* c, uint64_t pub_seed, void* d_w) {
     return guard([&] {
         need(ctx && c && d_w, "NULL argument");
-        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        std::lock_guard<std::mutex> lk(ctx->impl.aux_mu);
         ZK_HIP(hipSetDevice(ctx->impl.device));
-        zk::circuit_witness_device(*c->impl, pub_seed, d_w, ctx->impl.stream);
+        zk::circuit_witness_device(*c->impl, pub_seed, d_w, ctx->impl.aux_stream);
     });
 }
 

@@ -116,6 +116,43 @@ ZK_DEV F29 f29_mul(const F29& a, const F29& b) {
     return t;
 }
 
+// Montgomery square a*a/R' mod p for normalized a: 45 partial products
+// (the off-diagonal a_i*a_j, i < j, taken once against 2*a_j, which stays
+// below 2^30) instead of 81; a column then sums at most 5 products below
+// 2^59, 9 reduction products below 2^58 and the carry, under 2^64.
+ZK_DEV F29 f29_sqr(const F29& a) {
+    using namespace f29c;
+    uint32_t m[9], d[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) d[i] = a.l[i] << 1;
+    F29 t;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < 17; k++) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            const int j = k - i;
+            if (j > i && j < 9) acc += (uint64_t)a.l[i] * d[j];
+            if (j == i) acc += (uint64_t)a.l[i] * a.l[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            const int j = k - i;
+            if (i < k && j >= 0 && j < 9) acc += (uint64_t)m[i] * P[j];
+        }
+        if (k < 9) {
+            m[k] = f29_qword((uint32_t)acc);
+            acc += (uint64_t)m[k] * P[0];
+        } else {
+            t.l[k - 9] = (uint32_t)acc & MASK;
+        }
+        acc >>= 29;
+        asm volatile("" : "+v"(acc));
+    }
+    t.l[8] = (uint32_t)acc;
+    return t;
+}
+
 // sum of N Montgomery products, sum_k a_k*b_k/R' mod p, with one reduction
 // (product scanning as in f29_mul). Operands normalized (limbs < 2^29); a
 // column then sums at most (2N + 1)*9 products below 2^58 plus the carry,
@@ -299,10 +336,10 @@ ZK_DEV void f29_madd(Xyzz29& A, const F29& x2, const F29& y2) {
         }
         return;
     }
-    const F29 PP = f29_mul(P_, P_);
+    const F29 PP = f29_sqr(P_);
     const F29 PPP = f29_mul(P_, PP);
     const F29 Q = f29_mul(A.X, PP);
-    const F29 RR = f29_mul(R_, R_);
+    const F29 RR = f29_sqr(R_);
     // X3 = RR - PPP - 2Q + 12p: limbs stay in [0, 2^32)
     F29 x3;
 #pragma unroll
@@ -318,13 +355,12 @@ ZK_DEV void f29_madd(Xyzz29& A, const F29& x2, const F29& y2) {
     A.ZZZ = f29_mul(A.ZZZ, PPP);
 }
 
-// One 128-byte slot per G1 base P = (x, y), so that the random gathers of the
-// bucket loop touch one cache line per point: the coordinates of P in this
-// form, each packed into 8 words (canonical, < p), and the x of the
-// endomorphism image lambda*P = (beta*x, y) of the GLV split:
-//   words 0..7 y, 8..15 x, 16..23 beta*x, 24..31 unused.
-// Base index i < n reads (x, y) of slot i; n <= i < 2n reads (beta*x, y) of
-// slot i - n.
+// One 128-byte slot per G1 base P = (x, y): the two bases of the GLV split,
+// P and its endomorphism image lambda*P = (beta*x, y), each as a 64-byte half
+// holding its coordinates in this form packed into 8 words (canonical, < p):
+//   words 0..7 y, 8..15 x | 16..23 y, 24..31 beta*x.
+// Base index i < n reads the first half of slot i; n <= i < 2n the second
+// half of slot i - n. A gather thus touches one aligned 64-byte half.
 constexpr int PT29_SLOT_UINT4 = 8;
 
 ZK_DEV F29 f29_load_packed(const uint4* q) {
@@ -335,9 +371,10 @@ ZK_DEV F29 f29_load_packed(const uint4* q) {
 
 ZK_DEV void f29_load_point(const void* base, uint32_t idx, uint32_t n, F29& x, F29& y) {
     const bool phi = idx >= n;
-    const uint4* p = reinterpret_cast<const uint4*>(base) + (size_t)PT29_SLOT_UINT4 * (phi ? idx - n : idx);
+    const uint4* p = reinterpret_cast<const uint4*>(base) + (size_t)PT29_SLOT_UINT4 * (phi ? idx - n : idx) +
+                     (phi ? 4 : 0);
     y = f29_load_packed(p);
-    x = f29_load_packed(p + (phi ? 4 : 2));
+    x = f29_load_packed(p + 2);
 }
 
 // ---------------------------------------------------------------- Fq2 (G2)
@@ -381,19 +418,19 @@ ZK_DEV F29q f29q_sub_nr(const F29q& a, const F29q& b) {
 ZK_DEV F29q f29q_from_std(const Fq2& x) { return {f29_from_std(x.c0), f29_from_std(x.c1)}; }
 ZK_DEV Fq2 f29q_to_std(const F29q& v) { return {f29_to_std(v.c0), f29_to_std(v.c1)}; }
 
-// One 256-byte slot per G2 base (two cache lines per gather), coordinates
-// packed as for G1: words 0..15 y (c0, c1), 16..31 x, 32..47 beta*x,
-// 48..63 unused; base index i >= n reads (beta*x, y) of slot i - n.
+// One 256-byte slot per G2 base, halves as for G1 (one 128-byte line per
+// gather): words 0..15 y (c0, c1), 16..31 x | 32..47 y, 48..63 beta*x; base
+// index i >= n reads the second half of slot i - n.
 constexpr int PT29Q_SLOT_UINT4 = 16;
 
 ZK_DEV void f29q_load_point(const void* base, uint32_t idx, uint32_t n, F29q& x, F29q& y) {
     const bool phi = idx >= n;
-    const uint4* p = reinterpret_cast<const uint4*>(base) + (size_t)PT29Q_SLOT_UINT4 * (phi ? idx - n : idx);
+    const uint4* p = reinterpret_cast<const uint4*>(base) + (size_t)PT29Q_SLOT_UINT4 * (phi ? idx - n : idx) +
+                     (phi ? 8 : 0);
     y.c0 = f29_load_packed(p);
     y.c1 = f29_load_packed(p + 2);
-    const uint4* px = p + (phi ? 8 : 4);
-    x.c0 = f29_load_packed(px);
-    x.c1 = f29_load_packed(px + 2);
+    x.c0 = f29_load_packed(p + 4);
+    x.c1 = f29_load_packed(p + 6);
 }
 
 // A G2 XYZZ accumulator: X, Y and ZZZ in LDS, 3 coordinates x 18 limbs laid

@@ -36,16 +36,16 @@ __global__ void k_points_to29(const void* __restrict__ pts, uint32_t n, void* __
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Aff<Fp> a = PointIO<Fp>::load_aff(pts, i);
-    uint32_t w[24] = {};
+    uint32_t w[32] = {};
     if (!aff_is_inf(a)) {
         pack29(a.y, w);
         pack29(a.x, w + 8);
-        pack29(mul<FpCfg>(a.x, beta_mont(glv::BETA_G1_MONT)), w + 16);
+        pack29(a.y, w + 16);
+        pack29(mul<FpCfg>(a.x, beta_mont(glv::BETA_G1_MONT)), w + 24);
     }
     uint4* o = reinterpret_cast<uint4*>(out) + PT29_SLOT_UINT4 * i;
 #pragma unroll
-    for (int k = 0; k < PT29_SLOT_UINT4; k++)
-        o[k] = k < 6 ? make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]) : make_uint4(0u, 0u, 0u, 0u);
+    for (int k = 0; k < PT29_SLOT_UINT4; k++) o[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
 }
 
 template <>

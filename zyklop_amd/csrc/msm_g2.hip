@@ -45,20 +45,21 @@ __global__ void k_points_to29q(const void* __restrict__ pts, uint32_t n, void* _
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Aff<Fq2> a = PointIO<Fq2>::load_aff(pts, i);
-    uint32_t w[48] = {};
+    uint32_t w[64] = {};
     if (!aff_is_inf(a)) {
         const Fp b = beta_mont(glv::BETA_G2_MONT);
         pack29(a.y.c0, w);
         pack29(a.y.c1, w + 8);
         pack29(a.x.c0, w + 16);
         pack29(a.x.c1, w + 24);
-        pack29(mul<FpCfg>(a.x.c0, b), w + 32);
-        pack29(mul<FpCfg>(a.x.c1, b), w + 40);
+        pack29(a.y.c0, w + 32);
+        pack29(a.y.c1, w + 40);
+        pack29(mul<FpCfg>(a.x.c0, b), w + 48);
+        pack29(mul<FpCfg>(a.x.c1, b), w + 56);
     }
     uint4* o = reinterpret_cast<uint4*>(out) + PT29Q_SLOT_UINT4 * i;
 #pragma unroll
-    for (int k = 0; k < PT29Q_SLOT_UINT4; k++)
-        o[k] = k < 12 ? make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]) : make_uint4(0u, 0u, 0u, 0u);
+    for (int k = 0; k < PT29Q_SLOT_UINT4; k++) o[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
 }
 
 template <>

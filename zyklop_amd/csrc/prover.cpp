@@ -25,6 +25,7 @@ Context::Context(int dev) : device(dev) {
     ZK_REQUIRE(dev >= 0 && dev < count, ZK_E_ARG, "no such HIP device: " + std::to_string(dev));
     ZK_HIP(hipSetDevice(dev));
     ZK_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    ZK_HIP(hipStreamCreateWithFlags(&aux_stream, hipStreamNonBlocking));
     msm_noalloc_setup(dev);   // device address of the fixed MSM scratch, once per device
 }
 
@@ -33,6 +34,7 @@ Context::~Context() {
     slot.reset();
     ntt_plans.clear();
     if (stream) (void)hipStreamDestroy(stream);
+    if (aux_stream) (void)hipStreamDestroy(aux_stream);
 }
 
 NttPlan& Context::ntt_plan(int log_n) {

@@ -133,6 +133,10 @@ struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
+    // side work that needs no other context state (synthetic witness
+    // generation): its own stream and lock, so it does not wait for provers
+    hipStream_t aux_stream = nullptr;
+    std::mutex aux_mu;
     std::map<int, std::unique_ptr<NttPlan>> ntt_plans;
     std::unique_ptr<ProveSlot> slot;   // slot used by the one-shot prove entry point
     DevBuf comb_g1, comb_g2;           // fixed-base tables for synthetic setup

@@ -261,7 +261,8 @@ void Server::loop() {
             std::string err;
             try {
                 if (slots_[i]->device_done()) {
-                    std::lock_guard<std::mutex> g(ctx_.mu);
+                    // the host tail reads only the slot's own buffers: no context lock,
+                    // so other callers of the context are not held up by it
                     slots_[i]->finish(&req->proof);
                     done = true;
                 }
