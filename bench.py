@@ -343,7 +343,42 @@ def section_probes(args, zk, ctx, circ):
         "GBps": {k: round(v, 1) for k, v in gbs.items()},
     }
     msm22 = probe_msm_2p22(args, zk, ctx, circ)
-    return roofline, roofline_ntt, msm22
+    roofline_spmv = probe_spmv(args, zk, ctx, circ)
+    # the VALU issue ceiling the MSM roofline is priced against, measured here
+    mad = ctx.probe_valu(0) / 1e12
+    add = ctx.probe_valu(1) / 1e12
+    log(f"[bench] VALU probe: v_mad_u64_u32 {mad:.2f} T lane-ops/s, v_add_u32 {add:.2f} T lane-ops/s "
+        f"(nominal {PEAK_MAC_TS:.2f})")
+    roofline["peak_measured"] = {"v_mad_u64_u32": round(mad, 2), "v_add_u32": round(add, 2), "unit": "T lane-ops/s",
+                                 "probe": "8 independent chains per thread, 2048 x 16 x 8 ops, 8 waves per SIMD"}
+    return roofline, roofline_ntt, msm22, roofline_spmv
+
+
+def probe_spmv(args, zk, ctx, circ):
+    """The fused A*w, B*w, (A*w)o(B*w) kernel of the headline circuit alone on the
+    device, against HBM. Algorithmic bytes per matrix M: 36 nnz_M (value + column)
+    + 4 (rows + 1) (row pointers) + 32 rows (output) + 32 n_vars (witness), plus
+    32 rows for the product output."""
+    w = circ.witness(args.seed * 1000 + 1)
+    d_w = ctx.dev_alloc(w.nbytes)
+    ctx.upload(d_w, w)
+    outs = [ctx.dev_alloc(circ.n * 32) for _ in range(3)]
+    ts = []
+    for i in range(2 + args.probe_reps):
+        t = ctx.spmv_device(circ, d_w, *outs)
+        if i >= 2:
+            ts.append(t)
+    for p in outs + [d_w]:
+        ctx.dev_free(p)
+    ms = float(np.mean(ts))
+    rows = circ.n
+    algo = sum(36 * nnz + 4 * (rows + 1) + 32 * rows + 32 * circ.n_vars for nnz in (circ.nnz_a, circ.nnz_b)) + 32 * rows
+    gbs = algo / (ms / 1e3) / 1e9
+    log(f"[bench] SpMV (A, B, AoB) alone: {ms:.3f} ms, {algo / 1e6:.1f} MB algorithmic -> {gbs:.0f} GB/s")
+    return {"bound": "hbm", "kernel": "k_spmv_abc (A*w, B*w and their product, fused)", "achieved": round(gbs, 1),
+            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
+            "avg_launch_ms": r3(ms), "algorithmic_bytes": int(algo),
+            "unit_def": "36 B per nonzero, 4 B per row pointer, 32 B per output row and per witness value"}
 
 
 def probe_msm_2p22(args, zk, ctx, circ):
@@ -652,7 +687,7 @@ def main():
     ctx = zk.Context(0)
     ctx.set_batch_slots(args.slots)
     circ, tuned = setup_headline(args, zk, ctx)
-    roofline, roofline_ntt, msm22 = section_probes(args, zk, ctx, circ)
+    roofline, roofline_ntt, msm22, roofline_spmv = section_probes(args, zk, ctx, circ)
     m, n_pub, K = 1 << args.log_m, args.n_pub, args.steps
     result = {
         "metric": "Groth16 proofs/s + p50/p99 latency at 2^20 constraints, batched vs plain",
@@ -680,6 +715,7 @@ def main():
         result.update(head)
     result["roofline"] = roofline
     result["roofline_ntt"] = roofline_ntt
+    result["roofline_spmv"] = roofline_spmv
     result["msm_g1_2p22"] = msm22
     circ.free()
     if "mixed" in sections:

@@ -297,6 +297,34 @@ int zk_verify_json(const char* zkey_path, const char* public_json, const char* p
  * as (a0, a1) Montgomery (48 words). For tests of the pairing itself. */
 int zk_pairing(const uint64_t p[8], const uint64_t q[16], uint64_t out[48]);
 
+/* ---- device field arithmetic, element-wise (tests of the field code) ----
+ * No counterpart entry point in the reference, whose unit tests call fpMul,
+ * frMul etc. from test kernels (test/unit/). zk_field_op_device runs the
+ * standard form (8 x 32-bit, Montgomery R = 2^256) of field (0 = Fp, 1 = Fr)
+ * on n elements of 4 words: op 0 add, 1 sub, 2 neg, 3 Montgomery product (the
+ * product-scanning asm), 4 square, 5 inverse (Montgomery in and out), 6 to
+ * Montgomery, 7 from Montgomery, 8 Montgomery product by the C++ CIOS form.
+ * zk_field29_op_device runs the bucket loops' 9 x 29-bit form of Fp
+ * (Montgomery R' = 2^261) on raw limbs, 9 words per element, limbs as given
+ * (no range check: inputs outside the form's documented bounds give
+ * unspecified values): op 0 product, 1 square, 2 a*b + b*a with one
+ * reduction, 3 a*a + a*b + b*a + b*b with one reduction, 4 negation of a
+ * value < p, 5 a + 4p - b normalized, 6 reduction of a value < 16p below 2p,
+ * 7 conversion to the standard form and back. */
+int zk_field_op_device(zk_ctx* ctx, int field, int op, const uint64_t* a, const uint64_t* b, uint64_t* out, size_t n);
+int zk_field29_op_device(zk_ctx* ctx, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, size_t n);
+
+/* ---- measurement probes (no reference counterpart) ----
+ * zk_probe_valu: lane operations per second of v_mad_u64_u32 (kind 0) or
+ * v_add_u32 (kind 1) over the whole GPU, 8 independent chains per thread:
+ * the issue-rate ceiling behind the MSM roofline.
+ * zk_circuit_spmv_device: the circuit's fused A*w, B*w, A*w o B*w kernel
+ * (src/groth16.cu:144-167) alone on the device from d_witness (n_vars x 32
+ * bytes canonical) into three domain-size arrays; zk_last_timing()[0] gives
+ * its device time. */
+int zk_probe_valu(zk_ctx* ctx, int kind, double* lane_ops_per_s);
+int zk_circuit_spmv_device(zk_ctx* ctx, const zk_circuit* c, const void* d_witness, void* d_aw, void* d_bw, void* d_cw);
+
 /* ---- host-side check of the MSM's scalar split (no GPU) ---- */
 /* No counterpart in the reference, whose MSMs run over full 254-bit scalars
  * (include/msm_g1.cuh:271-329). The GLV split the MSM kernels apply to every

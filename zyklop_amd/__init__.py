@@ -131,6 +131,10 @@ ABI = {
                                       ctypes.POINTER(ctypes.c_int)]),
     "zk_verify_json": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
     "zk_pairing": (ctypes.c_int, [_u64p, _u64p, _u64p]),
+    "zk_field_op_device": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _u64p, _u64p, _u64p, ctypes.c_size_t]),
+    "zk_field29_op_device": (ctypes.c_int, [_vp, ctypes.c_int, _u32p, _u32p, _u32p, ctypes.c_size_t]),
+    "zk_probe_valu": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "zk_circuit_spmv_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 
@@ -284,6 +288,35 @@ class Context:
         inf = ctypes.c_int(0)
         _check(lib().zk_msm_g1_device_noalloc(self.h, d_points, d_scalars, n, window, u64p(out), ctypes.byref(inf)))
         return out, bool(inf.value)
+
+    def field_op(self, field, op, a, b=None):
+        """Element-wise device Fp (field 0) / Fr (1) op on (n, 4) words (zk_field_op_device)."""
+        a = np.ascontiguousarray(a, dtype=np.uint64).reshape(-1, 4)
+        bb = None if b is None else np.ascontiguousarray(b, dtype=np.uint64).reshape(-1, 4)
+        out = np.zeros_like(a)
+        _check(lib().zk_field_op_device(self.h, field, op, u64p(a), u64p(bb) if bb is not None else None, u64p(out),
+                                        a.shape[0]))
+        return out
+
+    def field29_op(self, op, a, b=None):
+        """Element-wise device op of the 29-bit Fp form on (n, 9) raw limbs (zk_field29_op_device)."""
+        a = np.ascontiguousarray(a, dtype=np.uint32).reshape(-1, 9)
+        bb = None if b is None else np.ascontiguousarray(b, dtype=np.uint32).reshape(-1, 9)
+        out = np.zeros_like(a)
+        _check(lib().zk_field29_op_device(self.h, op, u32p(a), u32p(bb) if bb is not None else None, u32p(out),
+                                          a.shape[0]))
+        return out
+
+    def probe_valu(self, kind=0):
+        """Lane-ops/s of v_mad_u64_u32 (kind 0) or v_add_u32 (kind 1) over the GPU."""
+        r = ctypes.c_double(0)
+        _check(lib().zk_probe_valu(self.h, kind, ctypes.byref(r)))
+        return r.value
+
+    def spmv_device(self, circuit, d_witness, d_aw, d_bw, d_cw):
+        """The circuit's fused SpMV alone on the device; returns its device ms."""
+        _check(lib().zk_circuit_spmv_device(self.h, circuit.h, d_witness, d_aw, d_bw, d_cw))
+        return self.last_timing()[0]
 
     def msm_g2_device(self, d_points, d_scalars, n, window=0):
         out = np.zeros(16, np.uint64)

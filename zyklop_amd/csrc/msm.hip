@@ -158,10 +158,15 @@ __global__ void __launch_bounds__(1024) k_sort_tile_scatter(const uint32_t* __re
     }
 }
 
+// Entries of a group go to their final positions through an LDS stage of
+// `stage_cap` words when the group fits (then the block writes its whole
+// range with coalesced stores), else straight to global memory: scattered
+// 4-byte stores into a range that other blocks do not touch.
+constexpr uint32_t SORT_GROUP_LDS = 65536 - 256;   // dynamic LDS bytes of k_sort_group
 __global__ void __launch_bounds__(256) k_sort_group(const uint2* __restrict__ pairs, const uint32_t* __restrict__ mscan,
-                                                    uint32_t nt, int s, uint32_t nb, uint32_t* __restrict__ offsets,
-                                                    uint32_t* __restrict__ entries) {
-    extern __shared__ uint32_t cnt[];   // 2^s counters
+                                                    uint32_t nt, int s, uint32_t nb, uint32_t stage_cap,
+                                                    uint32_t* __restrict__ offsets, uint32_t* __restrict__ entries) {
+    extern __shared__ uint32_t cnt[];   // 2^s counters, then the stage
     __shared__ uint32_t wsum[16];
     const uint32_t g = blockIdx.x, F = 1u << s, mask = F - 1;
     const uint32_t start = mscan[(size_t)g * nt], end = mscan[(size_t)(g + 1) * nt];
@@ -188,9 +193,19 @@ __global__ void __launch_bounds__(256) k_sort_group(const uint2* __restrict__ pa
     }
     if (g == gridDim.x - 1 && threadIdx.x == 0) offsets[nb] = end;
     __syncthreads();
-    for (uint32_t e = start + threadIdx.x; e < end; e += blockDim.x) {
-        const uint2 p = pairs[e];
-        entries[atomicAdd(&cnt[p.y & mask], 1u)] = p.x;
+    if (end - start <= stage_cap) {
+        uint32_t* stage = cnt + F;
+        for (uint32_t e = start + threadIdx.x; e < end; e += blockDim.x) {
+            const uint2 p = pairs[e];
+            stage[atomicAdd(&cnt[p.y & mask], 1u) - start] = p.x;
+        }
+        __syncthreads();
+        for (uint32_t e = threadIdx.x; e < end - start; e += blockDim.x) entries[start + e] = stage[e];
+    } else {
+        for (uint32_t e = start + threadIdx.x; e < end; e += blockDim.x) {
+            const uint2 p = pairs[e];
+            entries[atomicAdd(&cnt[p.y & mask], 1u)] = p.x;
+        }
     }
 }
 
@@ -353,8 +368,13 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
                                                           pairs.as<uint2>());
         ZK_HIP(hipGetLastError());
     }
-    k_sort_group<<<G, 256, (4u << sh), st>>>(pairs.as<uint2>(), msc, nt, sh, (uint32_t)nb, offsets.as<uint32_t>(),
-                                             entries.as<uint32_t>());
+    // stage for about 1.25x the mean group (group sizes are sums of many
+    // digits, so far larger ones are rare and take the global path)
+    ZK_REQUIRE((4u << sh) < SORT_GROUP_LDS / 2, ZK_E_INTERNAL, "sort group counters exceed the LDS budget");
+    const uint32_t stage_cap = (uint32_t)std::min<size_t>((SORT_GROUP_LDS - (4u << sh)) / 4,
+                                                          z.ne / G + z.ne / G / 4 + 256);
+    k_sort_group<<<G, 256, 4u * ((1u << sh) + stage_cap), st>>>(pairs.as<uint2>(), msc, nt, sh, (uint32_t)nb, stage_cap,
+                                                                offsets.as<uint32_t>(), entries.as<uint32_t>());
     ZK_HIP(hipGetLastError());
     const uint32_t* off = offsets.as<uint32_t>();
     uint32_t* hist = lkeys.as<uint32_t>();
