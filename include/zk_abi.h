@@ -245,16 +245,18 @@ int zk_msm_g2_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, s
  * serving loops, harnesses that forbid hipMalloc). Its whole device workspace
  * is a fixed 64 MiB array that the library holds on every device it runs on
  * (enough for 65536 points at the default window); larger workspaces are
- * refused with ZK_E_ARG. The host partials go to a fixed host array. Calls
+ * refused with ZK_E_ARG. The window values go to a fixed host array. Calls
  * on one device share both arrays: calls that overlap on one device race.
  * Results are identical to zk_msm_g1_device's. */
 int zk_msm_g1_device_noalloc(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t n, int window,
                              uint64_t out[8], int* is_inf);
 /* device time of the last kernel-level call on ctx (HIP events on its stream):
  * MSM: [0] total ms [1] digit sort ms [2] bucket-accumulation kernel ms
- *      [3] number of (point, bucket) entries accumulated;
+ *      [3] number of (point, bucket) entries accumulated
+ *      [4] bucket reduction ms (end of the accumulation to the window values
+ *          on the host);
  * NTT: [0] total ms, others 0 */
-int zk_last_timing(zk_ctx* ctx, float ms[4]);
+int zk_last_timing(zk_ctx* ctx, float ms[5]);
 /* device pointer and point count of a resident base array: which = 0 A',
  * 1 B1', 2 B2' (G2), 3 C', 4 H; the primed arrays carry the 4 extra slots
  * described in the prover (alpha/beta/delta terms) after the n_vars bases */
@@ -315,9 +317,14 @@ int zk_field_op_device(zk_ctx* ctx, int field, int op, const uint64_t* a, const 
 int zk_field29_op_device(zk_ctx* ctx, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, size_t n);
 
 /* ---- measurement probes (no reference counterpart) ----
- * zk_probe_valu: lane operations per second of v_mad_u64_u32 (kind 0) or
- * v_add_u32 (kind 1) over the whole GPU, 8 independent chains per thread:
- * the issue-rate ceiling behind the MSM roofline.
+ * zk_probe_valu: lane operations per second over the whole GPU of
+ * kind & 15: 0 v_mad_u64_u32 (8 independent chains per thread), 1 v_add_u32,
+ * 2 v_fma_f64 (8 chains), 3 v_mad_u64_u32 in one dependent chain, 4 / 5
+ * dependent chains of the bucket accumulation's mixed XYZZ additions (29-bit
+ * form) / the bucket reduction's general ones (standard form), in point
+ * additions per second; with
+ * kind >> 4 waves per SIMD (0: 8). Kind 0 is the issue-rate ceiling behind
+ * the MSM roofline; the others show what a lone wave can issue.
  * zk_circuit_spmv_device: the circuit's fused A*w, B*w, A*w o B*w kernel
  * (src/groth16.cu:144-167) alone on the device from d_witness (n_vars x 32
  * bytes canonical) into three domain-size arrays; zk_last_timing()[0] gives

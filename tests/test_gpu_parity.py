@@ -124,6 +124,31 @@ def test_msm_g1_equal_points_in_bucket(ctx, window):
         assert not inf and np.array_equal(out, exp), (ks, ss, window)
 
 
+@pytest.mark.parametrize("window", [0, 8, 13])
+def test_msm_equal_sums_in_reduction(ctx, window):
+    """Consecutive buckets holding the same point (or its negative): the
+    reduction's running sums and tree nodes then meet equal and opposite
+    operands (doubling and infinity branches of the general addition).
+    Scalars 1..m put one point in each of the lowest buckets of window 0."""
+    a = 192837465
+    m = 300
+    sc = O.arr(list(range(1, m + 1)))
+    for ks in ([a] * m, [a if i % 2 else pyref.R - a for i in range(m)],
+               [a if (i // 3) % 2 else pyref.R - a for i in range(m)]):
+        tot = sum((i + 1) * k for i, k in enumerate(ks)) % pyref.R
+        out, inf = ctx.msm_g1(mont_pts_g1(ks), sc, window)
+        e, _ = O.g1_mul(O.g1_gen(), tot)
+        exp = O.field_op(O.FP, O.TO_MONT, e.reshape(2, 4)).reshape(8)
+        assert not inf and np.array_equal(out, exp), (window, ks[:4])
+    # G2: buckets 0..29 get P and -P (infinity), buckets 30..59 P alone
+    pts = mont_pts_g2([a, pyref.R - a])
+    pts = np.concatenate([np.repeat(pts[:1], 60, 0), np.repeat(pts[1:], 30, 0)])
+    sc2 = O.arr(list(range(1, 61)) + list(range(1, 31)))
+    out, inf = ctx.msm_g2(pts, sc2, window)
+    exp, einf = oracle_msm_g2(pts, sc2)
+    assert not einf and not inf and np.array_equal(out, exp), window
+
+
 def test_msm_g1_skewed_large_buckets(ctx):
     """Most scalars equal 1 or 2: a few buckets get thousands of entries and take
     the chunked (oversized-bucket) path; the rest stay small."""

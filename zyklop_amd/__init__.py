@@ -241,10 +241,10 @@ class Context:
         return h
 
     def last_timing(self):
-        """(total_ms, sort_ms, accumulation_kernel_ms, entries) of the last kernel-level call."""
-        ms = (ctypes.c_float * 4)()
+        """(total_ms, sort_ms, accumulation_kernel_ms, entries, reduction_ms) of the last kernel-level call."""
+        ms = (ctypes.c_float * 5)()
         _check(lib().zk_last_timing(self.h, ms))
-        return float(ms[0]), float(ms[1]), float(ms[2]), float(ms[3])
+        return tuple(float(v) for v in ms)
 
     def set_windows(self, w_witness=0, w_h=0):
         _check(lib().zk_set_windows(self.h, w_witness, w_h))
@@ -308,7 +308,8 @@ class Context:
         return out
 
     def probe_valu(self, kind=0):
-        """Lane-ops/s of v_mad_u64_u32 (kind 0) or v_add_u32 (kind 1) over the GPU."""
+        """Lane-ops/s over the GPU of kind & 15: 0 v_mad_u64_u32 (8 chains), 1 v_add_u32,
+        2 v_fma_f64 (8 chains), 3 one dependent v_mad_u64_u32 chain; kind >> 4 waves per SIMD (0: 8)."""
         r = ctypes.c_double(0)
         _check(lib().zk_probe_valu(self.h, kind, ctypes.byref(r)))
         return r.value

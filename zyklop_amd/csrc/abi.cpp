@@ -28,7 +28,7 @@ struct zk_ctx {
     std::mutex servers_mu;
     std::set<zk_server*> servers;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
-    float last_ms[4] = {0, 0, 0, 0};
+    float last_ms[5] = {0, 0, 0, 0, 0};
     bool profiling = false;
     int batch_slots_n = 3;
     explicit zk_ctx(int dev) : impl(dev) {
@@ -167,6 +167,7 @@ void msm_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t
     ZK_HIP(hipEventElapsedTime(&ctx->last_ms[0], ctx->ev0, ctx->ev2));
     ZK_HIP(hipEventElapsedTime(&ctx->last_ms[1], ctx->ev0, ctx->ev1));
     ZK_HIP(hipEventElapsedTime(&ctx->last_ms[2], scr.ev_acc0, scr.ev_acc1));
+    ZK_HIP(hipEventElapsedTime(&ctx->last_ms[4], scr.ev_acc1, scr.ev_end));
     uint32_t entries = 0;
     ZK_HIP(hipMemcpy(&entries, sort.offsets.as<uint32_t>() + (size_t)sort.g.W * sort.g.K, 4, hipMemcpyDeviceToHost));
     ctx->last_ms[3] = (float)entries;
@@ -185,7 +186,7 @@ void ntt_device(zk_ctx* ctx, void* const* in, void* const* out, int batch, int l
     ZK_HIP(hipEventRecord(ctx->ev2, C.stream));
     ZK_HIP(hipStreamSynchronize(C.stream));
     ZK_HIP(hipEventElapsedTime(&ctx->last_ms[0], ctx->ev0, ctx->ev2));
-    ctx->last_ms[1] = ctx->last_ms[2] = ctx->last_ms[3] = 0;
+    ctx->last_ms[1] = ctx->last_ms[2] = ctx->last_ms[3] = ctx->last_ms[4] = 0;
 }
 }  // namespace
 
@@ -818,10 +819,10 @@ int zk_dev_download(zk_ctx* ctx, void* dst, const void* src, size_t bytes) {
     });
 }
 
-int zk_last_timing(zk_ctx* ctx, float ms[4]) {
+int zk_last_timing(zk_ctx* ctx, float ms[5]) {
     return guard([&] {
         need(ctx && ms, "NULL argument");
-        for (int i = 0; i < 4; i++) ms[i] = ctx->last_ms[i];
+        for (int i = 0; i < 5; i++) ms[i] = ctx->last_ms[i];
     });
 }
 
@@ -1019,7 +1020,7 @@ int zk_field29_op_device(zk_ctx* ctx, int op, const uint32_t* a, const uint32_t*
 int zk_probe_valu(zk_ctx* ctx, int kind, double* lane_ops_per_s) {
     return guard([&] {
         need(ctx && lane_ops_per_s, "NULL argument");
-        need(kind == 0 || kind == 1, "kind must be 0 (v_mad_u64_u32) or 1 (v_add_u32)");
+        need(kind >= 0 && (kind & 15) <= 5 && (kind >> 4) <= 16, "kind: op 0..5 + 16 * waves per SIMD (0..16)");
         std::lock_guard<std::mutex> lk(ctx->impl.mu);
         ZK_HIP(hipSetDevice(ctx->impl.device));
         *lane_ops_per_s = zk::valu_probe(kind, ctx->impl.stream);
@@ -1039,6 +1040,6 @@ int zk_circuit_spmv_device(zk_ctx* ctx, const zk_circuit* c, const void* d_witne
         ZK_HIP(hipEventRecord(ctx->ev2, st));
         ZK_HIP(hipEventSynchronize(ctx->ev2));
         ZK_HIP(hipEventElapsedTime(&ctx->last_ms[0], ctx->ev0, ctx->ev2));
-        ctx->last_ms[1] = ctx->last_ms[2] = ctx->last_ms[3] = 0;
+        ctx->last_ms[1] = ctx->last_ms[2] = ctx->last_ms[3] = ctx->last_ms[4] = 0;
     });
 }

@@ -89,7 +89,7 @@ __global__ void __launch_bounds__(256) k_valu_probe(int kind, uint32_t iters, ui
 #pragma unroll
                 for (int j = 0; j < 8; j++) acc[j] = (uint64_t)(uint32_t)acc[j] * a[j] + acc[j];
         }
-    } else {
+    } else if (kind == 1) {
         uint32_t x[8], y[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
@@ -107,6 +107,34 @@ __global__ void __launch_bounds__(256) k_valu_probe(int kind, uint32_t iters, ui
         }
 #pragma unroll
         for (int j = 0; j < 8; j++) acc[j] = x[j] ^ ((uint64_t)y[j] << 32);
+    } else if (kind == 2) {
+        // v_fma_f64, 8 independent chains
+        double x[8], y[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            x[j] = (double)(acc[j] & 0xffff) * 0x1p-20;
+            y[j] = 1.0 - (double)(a[j] & 0xff) * 0x1p-30;
+        }
+        for (uint32_t it = 0; it < iters; it++) {
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+#pragma unroll
+                for (int j = 0; j < 8; j++) x[j] = __builtin_fma(x[j], y[j], 0x1p-40);
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc[j] = (uint64_t)__double_as_longlong(x[j]);
+    } else {
+        // one dependent chain of v_mad_u64_u32 (8 steps per inner round)
+        uint64_t c = acc[0];
+        for (uint32_t it = 0; it < iters; it++) {
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+#pragma unroll
+                for (int j = 0; j < 8; j++) c = (uint64_t)(uint32_t)c * a[j] + c;
+        }
+        acc[0] = c;
+#pragma unroll
+        for (int j = 1; j < 8; j++) acc[j] = 0;
     }
     uint64_t s = 0;
 #pragma unroll
@@ -114,22 +142,83 @@ __global__ void __launch_bounds__(256) k_valu_probe(int kind, uint32_t iters, ui
     sink[t] = s;
 }
 
+// dependent chains of point additions, 16 per iteration: op 4 mixed XYZZ
+// additions in the 29-bit form (the bucket accumulation's), op 5 general XYZZ
+// additions in the standard form (the bucket reduction's). The operands are
+// limb patterns below p, not curve points: only the instruction stream counts.
+__global__ void __launch_bounds__(64) k_point_probe(int op, uint32_t iters, uint32_t seed, uint64_t* __restrict__ sink) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t s = 0;
+    if (op == 4) {
+        Xyzz29 p;
+        F29 qx, qy;
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            const uint32_t h = seed * (t * 9 + i + 1), m = i < 8 ? f29c::MASK : 0xffffu;
+            p.X.l[i] = (h * 2654435761u) & m;
+            p.Y.l[i] = (h * 40503u + 7) & m;
+            p.ZZ.l[i] = (h * 69069u + 1) & m;
+            p.ZZZ.l[i] = (h * 1103515245u + 12345u) & m;
+            qx.l[i] = p.Y.l[i] ^ 0x5555u;
+            qy.l[i] = p.ZZ.l[i] ^ 0x3333u;
+        }
+        p.inf = false;
+        for (uint32_t it = 0; it < iters; it++) {
+#pragma unroll 1
+            for (int r = 0; r < 16; r++) f29_madd(p, qx, qy);
+        }
+#pragma unroll
+        for (int i = 0; i < 9; i++) s ^= p.X.l[i] ^ p.Y.l[i] ^ p.ZZ.l[i] ^ p.ZZZ.l[i];
+    } else {
+        Xyzz<Fp> p, q;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint32_t h = seed * (t * 8 + i + 1), m = i < 7 ? 0xffffffffu : 0xffffu;
+            p.X.v[i] = (h * 2654435761u) & m;
+            p.Y.v[i] = (h * 40503u + 7) & m;
+            p.ZZ.v[i] = (h * 69069u + 1) & m;
+            p.ZZZ.v[i] = (h * 1103515245u + 12345u) & m;
+            q.X.v[i] = p.Y.v[i] ^ 0x5555u;
+            q.Y.v[i] = p.ZZ.v[i] ^ 0x3333u;
+            q.ZZ.v[i] = p.X.v[i] ^ 0x0f0fu;
+            q.ZZZ.v[i] = p.ZZZ.v[i] ^ 0x00ffu;
+        }
+        for (uint32_t it = 0; it < iters; it++) {
+#pragma unroll 1
+            for (int r = 0; r < 16; r++) p = xyzz_add_i(p, q);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) s ^= p.X.v[i] ^ p.Y.v[i] ^ p.ZZ.v[i] ^ p.ZZZ.v[i];
+    }
+    sink[t] = s;
+}
+
+// kind = op + 16 * waves: op 0 v_mad_u64_u32 (8 independent chains), 1
+// v_add_u32 (2 per step), 2 v_fma_f64 (8 chains), 3 one dependent chain of
+// v_mad_u64_u32, 4 / 5 point additions (k_point_probe); waves = waves per
+// SIMD on a 1024-SIMD chip (0: 8)
 double valu_probe(int kind, hipStream_t st) {
-    const uint32_t blocks = 256 * 4 * 8, threads = 256, iters = 2048;
+    const int op = kind & 15, waves = kind >> 4 ? kind >> 4 : 8;
+    const bool pts = op >= 4;
+    const uint32_t threads = 64, blocks = 1024 * (uint32_t)waves, iters = pts ? 16 : 2048;
     DevBuf sink((size_t)blocks * threads * 8);
     hipEvent_t e0, e1;
     ZK_HIP(hipEventCreate(&e0));
     ZK_HIP(hipEventCreate(&e1));
-    k_valu_probe<<<blocks, threads, 0, st>>>(kind, 16, 12345u, sink.as<uint64_t>());  // warm-up
+    auto launch = [&](uint32_t it) {
+        if (pts) k_point_probe<<<blocks, threads, 0, st>>>(op, it, 12345u, sink.as<uint64_t>());
+        else k_valu_probe<<<blocks, threads, 0, st>>>(op, it, 12345u, sink.as<uint64_t>());
+    };
+    launch(1);  // warm-up
     ZK_HIP(hipEventRecord(e0, st));
-    k_valu_probe<<<blocks, threads, 0, st>>>(kind, iters, 12345u, sink.as<uint64_t>());
+    launch(iters);
     ZK_HIP(hipEventRecord(e1, st));
     ZK_HIP(hipEventSynchronize(e1));
     float ms = 0;
     ZK_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    const double ops_per_step = kind == 0 ? 1.0 : 2.0;
+    const double ops_per_step = op == 1 ? 2.0 : (pts ? 1.0 / 8 : 1.0);
     return (double)blocks * threads * iters * 16 * 8 * ops_per_step / (ms * 1e-3);   // lane-ops per second
 }
 
