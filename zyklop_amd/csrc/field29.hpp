@@ -1,5 +1,5 @@
-// BN254 Fp in 9 limbs of 29 bits, Montgomery with R' = 2^261, for the G1
-// bucket-accumulation loop.
+// BN254 Fp in 9 limbs of 29 bits, Montgomery with R' = 2^261, for the MSM
+// bucket phase (accumulation and reduction, G1 and G2).
 //
 // Why: with 32-bit limbs a column of the Montgomery product sums up to 16
 // products of 64 bits, so every v_mad_u64_u32 needs a second instruction to
@@ -17,8 +17,9 @@
 // returns N (t = (ab + mp) / R' < ab/R' + p < 2p).
 //
 // Points enter this form once per circuit (from the standard 8 x 32-bit,
-// R = 2^256 Montgomery form) and buckets leave it once per MSM, so every
-// other kernel keeps the standard form.
+// R = 2^256 Montgomery form); the buckets stay in it through the reduction
+// and only the level-2 partials leave it, so every other kernel keeps the
+// standard form.
 #pragma once
 #include <stdint.h>
 
@@ -258,6 +259,14 @@ ZK_DEV F29 f29_neg(const F29& y) {
     return f29_norm(r);
 }
 
+// limb-wise sum (no carry propagation)
+ZK_DEV F29 f29_add(const F29& a, const F29& b) {
+    F29 r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.l[i] = a.l[i] + b.l[i];
+    return r;
+}
+
 // 8 x 32-bit words (value < 2^256) <-> 9 x 29-bit limbs
 ZK_DEV F29 f29_unpack(const uint32_t w[8]) {
     F29 r;
@@ -355,6 +364,105 @@ ZK_DEV void f29_madd(Xyzz29& A, const F29& x2, const F29& y2) {
     A.ZZZ = f29_mul(A.ZZZ, PPP);
 }
 
+ZK_DEV Xyzz29 xyzz29_inf() {
+    Xyzz29 r;
+    r.X = r.Y = r.ZZ = r.ZZZ = F29{};
+    r.inf = true;
+    return r;
+}
+
+// General XYZZ operations of this form, for the bucket reduction.
+// dbl-2008-s-1 (a = 0) on an N point: U = 2Y, V = U^2, W = U*V, S = X*V,
+// M = 3X^2, X3 = M^2 - 2S, Y3 = M*(S - X3) - W*Y, ZZ3 = V*ZZ, ZZZ3 = W*ZZZ.
+// Bounds: U normalized < 4p, M normalized < 6p (both fine for f29_sqr /
+// f29_mul); M^2 + 8p - 2S lies in (4p, 10p) with every limb of 8p - 2S
+// positive; Y3 sums M*(S - X3) < 36p^2 and W*(-Y) <= 4p^2 (f29_mulsum<2>
+// bound). BN254 G1 has no point of order 2, so Y = 0 only at infinity.
+ZK_DEV Xyzz29 f29_xyzz_dbl(const Xyzz29& p) {
+    using namespace f29c;
+    if (p.inf) return p;
+    const F29 U = f29_norm(f29_add(p.Y, p.Y));
+    const F29 V = f29_sqr(U);
+    const F29 Wv = f29_mul(U, V);
+    const F29 S = f29_mul(p.X, V);
+    const F29 XX = f29_sqr(p.X);
+    const F29 M = f29_norm(f29_add(f29_add(XX, XX), XX));
+    const F29 MM = f29_sqr(M);
+    F29 x3;
+#pragma unroll
+    for (int i = 0; i < 9; i++) x3.l[i] = (P8[i] - S.l[i] - S.l[i]) + MM.l[i];
+    const F29 X3 = f29_reduce(f29_norm(x3));
+    const F29 T = f29_norm(f29_sub4(S, X3));
+    const F29 ya[2] = {M, Wv}, yb[2] = {T, f29_neg(p.Y)};
+    Xyzz29 r;
+    r.X = X3;
+    r.Y = f29_mulsum<2>(ya, yb);
+    r.ZZ = f29_mul(V, p.ZZ);
+    r.ZZZ = f29_mul(Wv, p.ZZZ);
+    r.inf = false;
+    return r;
+}
+
+// add-2008-s: p + q for N points, either of which may be infinity; equal
+// points go through the doubling, opposite points give infinity. Same bounds
+// as f29_madd; the result is N.
+ZK_DEV Xyzz29 f29_xyzz_add(const Xyzz29& p, const Xyzz29& q) {
+    using namespace f29c;
+    if (p.inf) return q;
+    if (q.inf) return p;
+    const F29 U1 = f29_mul(p.X, q.ZZ), U2 = f29_mul(q.X, p.ZZ);
+    const F29 S1 = f29_mul(p.Y, q.ZZZ), S2 = f29_mul(q.Y, p.ZZZ);
+    const F29 P_ = f29_norm(f29_sub4(U2, U1));
+    const F29 R_ = f29_norm(f29_sub4(S2, S1));
+    if (f29_sub4_is_zero(P_)) {
+        if (f29_sub4_is_zero(R_)) return f29_xyzz_dbl(p);
+        return xyzz29_inf();
+    }
+    const F29 PP = f29_sqr(P_);
+    const F29 PPP = f29_mul(P_, PP);
+    const F29 Q = f29_mul(U1, PP);
+    const F29 RR = f29_sqr(R_);
+    F29 x3;
+#pragma unroll
+    for (int i = 0; i < 9; i++) x3.l[i] = (P12[i] - PPP.l[i] - 2 * Q.l[i]) + RR.l[i];
+    const F29 X3 = f29_reduce(f29_norm(x3));
+    const F29 T = f29_norm(f29_sub4(Q, X3));
+    const F29 ya[2] = {R_, S1}, yb[2] = {T, f29_neg(PPP)};
+    Xyzz29 r;
+    r.X = X3;
+    r.Y = f29_mulsum<2>(ya, yb);
+    r.ZZ = f29_mul(f29_mul(p.ZZ, q.ZZ), PP);
+    r.ZZZ = f29_mul(f29_mul(p.ZZZ, q.ZZZ), PPP);
+    r.inf = false;
+    return r;
+}
+
+// An XYZZ point of this form in 32 words: X, Y, ZZ, ZZZ packed (values < 2p
+// fit 256 bits). Infinity is stored with ZZ = 0, which no finite point has
+// (its ZZ is a nonzero square mod p, so neither 0 nor p).
+ZK_DEV void xyzz29_pack(const Xyzz29& p, uint32_t w[32]) {
+    f29_pack(p.X, w);
+    f29_pack(p.Y, w + 8);
+    f29_pack(p.ZZ, w + 16);
+    f29_pack(p.ZZZ, w + 24);
+    if (p.inf) {
+#pragma unroll
+        for (int k = 16; k < 24; k++) w[k] = 0;
+    }
+}
+ZK_DEV Xyzz29 xyzz29_unpack(const uint32_t w[32]) {
+    Xyzz29 r;
+    r.X = f29_unpack(w);
+    r.Y = f29_unpack(w + 8);
+    r.ZZ = f29_unpack(w + 16);
+    r.ZZZ = f29_unpack(w + 24);
+    uint32_t z = 0;
+#pragma unroll
+    for (int k = 16; k < 24; k++) z |= w[k];
+    r.inf = z == 0;
+    return r;
+}
+
 // One 128-byte slot per G1 base P = (x, y): the two bases of the GLV split,
 // P and its endomorphism image lambda*P = (beta*x, y), each as a 64-byte half
 // holding its coordinates in this form packed into 8 words (canonical, < p):
@@ -383,13 +491,6 @@ struct F29q {
     F29 c0, c1;
 };
 
-ZK_DEV F29 f29_add(const F29& a, const F29& b) {
-    F29 r;
-#pragma unroll
-    for (int i = 0; i < 9; i++) r.l[i] = a.l[i] + b.l[i];
-    return r;
-}
-
 // a: components normalized with value < 6p; b: components N. Each component
 // is a sum of two products with one reduction (f29_mulsum): c0 = a0*b0 +
 // a1*(-b1), c1 = a0*b1 + a1*b0; result N. Four products and two reductions
@@ -417,6 +518,119 @@ ZK_DEV F29q f29q_sub_nr(const F29q& a, const F29q& b) {
 
 ZK_DEV F29q f29q_from_std(const Fq2& x) { return {f29_from_std(x.c0), f29_from_std(x.c1)}; }
 ZK_DEV Fq2 f29q_to_std(const F29q& v) { return {f29_to_std(v.c0), f29_to_std(v.c1)}; }
+
+// ---- general XYZZ operations over Fp2 in this form (the G2 bucket
+// reduction); bounds as in f29q_madd
+ZK_DEV F29q f29q_reduce(const F29q& a) { return {f29_reduce(a.c0), f29_reduce(a.c1)}; }
+ZK_DEV bool f29q_sub_is_zero(const F29q& x) { return f29_sub4_is_zero(x.c0) && f29_sub4_is_zero(x.c1); }
+
+// T*A - B*C, each component one sum of four products with one reduction:
+// T normalized < 6p, A, B, C N (below 32p^2 in all: the f29_mulsum bound)
+ZK_DEV F29q f29q_mul_sub_mul(const F29q& T, const F29q& A, const F29q& B, const F29q& C) {
+    const F29 nb0 = f29_neg(B.c0), nb1 = f29_neg(B.c1), na1 = f29_neg(A.c1);
+    const F29 a0[4] = {T.c0, T.c1, nb0, B.c1}, b0[4] = {A.c0, na1, C.c0, C.c1};
+    const F29 a1[4] = {T.c0, T.c1, nb0, nb1}, b1[4] = {A.c1, A.c0, C.c1, C.c0};
+    return {f29_mulsum<4>(a0, b0), f29_mulsum<4>(a1, b1)};
+}
+
+struct Xyzz29q {
+    F29q X, Y, ZZ, ZZZ;
+    bool inf;
+};
+
+ZK_DEV Xyzz29q xyzz29q_inf() {
+    Xyzz29q r;
+    r.X = r.Y = r.ZZ = r.ZZZ = F29q{};
+    r.inf = true;
+    return r;
+}
+
+// dbl-2008-s-1 over Fp2 (G2 has no point of order 2 either)
+ZK_DEV Xyzz29q f29q_xyzz_dbl(const Xyzz29q& p) {
+    using namespace f29c;
+    if (p.inf) return p;
+    const F29q U = {f29_reduce(f29_norm(f29_add(p.Y.c0, p.Y.c0))), f29_reduce(f29_norm(f29_add(p.Y.c1, p.Y.c1)))};
+    const F29q V = f29q_sqr(U);
+    const F29q Wv = f29q_mul(U, V);
+    const F29q S = f29q_mul(p.X, V);
+    const F29q XX = f29q_sqr(p.X);
+    const F29q M = {f29_reduce(f29_norm(f29_add(f29_add(XX.c0, XX.c0), XX.c0))),
+                    f29_reduce(f29_norm(f29_add(f29_add(XX.c1, XX.c1), XX.c1)))};
+    const F29q MM = f29q_sqr(M);
+    F29 x0, x1;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        x0.l[i] = (P8[i] - S.c0.l[i] - S.c0.l[i]) + MM.c0.l[i];
+        x1.l[i] = (P8[i] - S.c1.l[i] - S.c1.l[i]) + MM.c1.l[i];
+    }
+    Xyzz29q r;
+    r.X = {f29_reduce(f29_norm(x0)), f29_reduce(f29_norm(x1))};
+    r.Y = f29q_mul_sub_mul(f29q_sub_nr(S, r.X), M, Wv, p.Y);
+    r.ZZ = f29q_mul(V, p.ZZ);
+    r.ZZZ = f29q_mul(Wv, p.ZZZ);
+    r.inf = false;
+    return r;
+}
+
+// add-2008-s over Fp2
+ZK_DEV Xyzz29q f29q_xyzz_add(const Xyzz29q& p, const Xyzz29q& q) {
+    using namespace f29c;
+    if (p.inf) return q;
+    if (q.inf) return p;
+    const F29q U1 = f29q_mul(p.X, q.ZZ), U2 = f29q_mul(q.X, p.ZZ);
+    const F29q S1 = f29q_mul(p.Y, q.ZZZ), S2 = f29q_mul(q.Y, p.ZZZ);
+    const F29q Pn = f29q_sub_nr(U2, U1), Rn = f29q_sub_nr(S2, S1);
+    if (f29q_sub_is_zero(Pn)) {
+        if (f29q_sub_is_zero(Rn)) return f29q_xyzz_dbl(p);
+        return xyzz29q_inf();
+    }
+    const F29q P_ = f29q_reduce(Pn), R_ = f29q_reduce(Rn);
+    const F29q PP = f29q_sqr(P_);
+    const F29q PPP = f29q_mul(P_, PP);
+    const F29q Q = f29q_mul(U1, PP);
+    const F29q RR = f29q_sqr(R_);
+    F29 x0, x1;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        x0.l[i] = (P12[i] - PPP.c0.l[i] - 2 * Q.c0.l[i]) + RR.c0.l[i];
+        x1.l[i] = (P12[i] - PPP.c1.l[i] - 2 * Q.c1.l[i]) + RR.c1.l[i];
+    }
+    Xyzz29q r;
+    r.X = {f29_reduce(f29_norm(x0)), f29_reduce(f29_norm(x1))};
+    r.Y = f29q_mul_sub_mul(f29q_sub_nr(Q, r.X), R_, S1, PPP);
+    r.ZZ = f29q_mul(f29q_mul(p.ZZ, q.ZZ), PP);
+    r.ZZZ = f29q_mul(f29q_mul(p.ZZZ, q.ZZZ), PPP);
+    r.inf = false;
+    return r;
+}
+
+// 64 words: X, Y, ZZ, ZZZ, each c0 then c1 packed; infinity has ZZ = 0
+ZK_DEV void xyzz29q_pack(const Xyzz29q& p, uint32_t w[64]) {
+    const F29q* c[4] = {&p.X, &p.Y, &p.ZZ, &p.ZZZ};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        f29_pack(c[k]->c0, w + 16 * k);
+        f29_pack(c[k]->c1, w + 16 * k + 8);
+    }
+    if (p.inf) {
+#pragma unroll
+        for (int k = 32; k < 48; k++) w[k] = 0;
+    }
+}
+ZK_DEV Xyzz29q xyzz29q_unpack(const uint32_t w[64]) {
+    Xyzz29q r;
+    F29q* c[4] = {&r.X, &r.Y, &r.ZZ, &r.ZZZ};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        c[k]->c0 = f29_unpack(w + 16 * k);
+        c[k]->c1 = f29_unpack(w + 16 * k + 8);
+    }
+    uint32_t z = 0;
+#pragma unroll
+    for (int k = 32; k < 48; k++) z |= w[k];
+    r.inf = z == 0;
+    return r;
+}
 
 // One 256-byte slot per G2 base, halves as for G1 (one 128-byte line per
 // gather): words 0..15 y (c0, c1), 16..31 x | 32..47 y, 48..63 beta*x; base

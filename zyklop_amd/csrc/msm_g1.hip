@@ -24,9 +24,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         if (ref >> 31) y = f29_neg(y);
         f29_madd(acc, x, y);
     }
-    Xyzz<Fp> o = xyzz_inf<Fp>();
-    if (!acc.inf) o = {f29_to_std(acc.X), f29_to_std(acc.Y), f29_to_std(acc.ZZ), f29_to_std(acc.ZZZ)};
-    PointIO<Fp>::store_xyzz(buckets, slot, o);
+    red_store<Fp>(buckets, slot, acc);
 }
 
 // packed words of the 29-bit form of x (std Montgomery), at w[0..7]

@@ -1,5 +1,6 @@
 // G2 bucket phase of the Pippenger MSM (design in msm.hpp): accumulation in
-// the 29-bit-limb form over Fp2 with the accumulator partly in LDS, base-slot
+// the 29-bit-limb form over Fp2 with the accumulator partly in LDS (the
+// buckets stay in that form for the reduction), base-slot
 // conversion, and the Fp2 instances of the shared chunk / reduction kernels
 // (msm_kernels.hpp).
 #include "msm_kernels.hpp"
@@ -32,9 +33,7 @@ __global__ void __launch_bounds__(ACC29Q_THREADS) __attribute__((amdgpu_waves_pe
         f29q_madd(A, inf, x, y);
     }
     using L = AccLds<ACC29Q_THREADS>;
-    Xyzz<Fq2> o = xyzz_inf<Fq2>();
-    if (!inf) o = {f29q_to_std(A.ld(L::X)), f29q_to_std(A.ld(L::Y)), f29q_to_std(A.ld(L::ZZ)), f29q_to_std(A.ld(L::ZZZ))};
-    PointIO<Fq2>::store_xyzz(buckets, slot, o);
+    red_store<Fq2>(buckets, slot, Xyzz29q{A.ld(L::X), A.ld(L::Y), A.ld(L::ZZ), A.ld(L::ZZZ), inf});
 }
 
 // packed words of the 29-bit form of x (std Montgomery), at w[0..7]

@@ -29,13 +29,96 @@ struct LargeRec { uint32_t bucket, first, count, pad; };
 // oversized-bucket threshold floor, chunk size and level sizes are shared with
 // the sort (msm.hip), which builds the chunk lists
 
+// ---------------------------------------------------------------- point forms
+// Buckets, level-1 results and the level-2 working set are in the 29-bit form
+// (field29.hpp), in which the accumulation leaves its buckets; the level-2
+// partials go to the host in the standard form.
+template <class T> struct Red;
+template <>
+struct Red<Fp> {
+    using Pt = Xyzz29;
+    static constexpr int WORDS = 32;  // packed words per point (xyzz29_pack)
+    ZK_DEV static Pt inf() { return xyzz29_inf(); }
+    ZK_DEV static Pt add(const Pt& a, const Pt& b) { return f29_xyzz_add(a, b); }
+    ZK_DEV static Pt dbl(const Pt& a) { return f29_xyzz_dbl(a); }
+    ZK_DEV static void pack(const Pt& p, uint32_t* w) { xyzz29_pack(p, w); }
+    ZK_DEV static Pt unpack(const uint32_t* w) { return xyzz29_unpack(w); }
+    ZK_DEV static Pt from_std(const Xyzz<Fp>& s) {
+        if (xyzz_is_inf(s)) return xyzz29_inf();
+        Xyzz29 r;
+        r.X = f29_from_std(s.X);
+        r.Y = f29_from_std(s.Y);
+        r.ZZ = f29_from_std(s.ZZ);
+        r.ZZZ = f29_from_std(s.ZZZ);
+        r.inf = false;
+        return r;
+    }
+    ZK_DEV static Xyzz<Fp> to_std(const Pt& p) {
+        if (p.inf) return xyzz_inf<Fp>();
+        return {f29_to_std(p.X), f29_to_std(p.Y), f29_to_std(p.ZZ), f29_to_std(p.ZZZ)};
+    }
+};
+template <>
+struct Red<Fq2> {
+    using Pt = Xyzz29q;
+    static constexpr int WORDS = 64;  // packed words per point (xyzz29q_pack)
+    ZK_DEV static Pt inf() { return xyzz29q_inf(); }
+    ZK_DEV static Pt add(const Pt& a, const Pt& b) { return f29q_xyzz_add(a, b); }
+    ZK_DEV static Pt dbl(const Pt& a) { return f29q_xyzz_dbl(a); }
+    ZK_DEV static void pack(const Pt& p, uint32_t* w) { xyzz29q_pack(p, w); }
+    ZK_DEV static Pt unpack(const uint32_t* w) { return xyzz29q_unpack(w); }
+    ZK_DEV static Pt from_std(const Xyzz<Fq2>& s) {
+        if (xyzz_is_inf(s)) return xyzz29q_inf();
+        Xyzz29q r;
+        r.X = f29q_from_std(s.X);
+        r.Y = f29q_from_std(s.Y);
+        r.ZZ = f29q_from_std(s.ZZ);
+        r.ZZZ = f29q_from_std(s.ZZZ);
+        r.inf = false;
+        return r;
+    }
+    ZK_DEV static Xyzz<Fq2> to_std(const Pt& p) {
+        if (p.inf) return xyzz_inf<Fq2>();
+        return {f29q_to_std(p.X), f29q_to_std(p.Y), f29q_to_std(p.ZZ), f29q_to_std(p.ZZZ)};
+    }
+};
+
+// points of this form in memory: WORDS words each, 16-byte aligned
+template <class T>
+ZK_DEV typename Red<T>::Pt red_load(const void* base, size_t i) {
+    const uint4* q = reinterpret_cast<const uint4*>(base) + (size_t)(Red<T>::WORDS / 4) * i;
+    uint32_t w[Red<T>::WORDS];
+#pragma unroll
+    for (int k = 0; k < Red<T>::WORDS / 4; k++) {
+        const uint4 v = q[k];
+        w[4 * k] = v.x;
+        w[4 * k + 1] = v.y;
+        w[4 * k + 2] = v.z;
+        w[4 * k + 3] = v.w;
+    }
+    return Red<T>::unpack(w);
+}
+template <class T>
+ZK_DEV void red_store(void* base, size_t i, const typename Red<T>::Pt& p) {
+    uint4* o = reinterpret_cast<uint4*>(base) + (size_t)(Red<T>::WORDS / 4) * i;
+    uint32_t w[Red<T>::WORDS];
+    Red<T>::pack(p, w);
+#pragma unroll
+    for (int k = 0; k < Red<T>::WORDS / 4; k++) o[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+}
+// a bucket slot written from the standard form (oversized buckets, empty parts)
+template <class T>
+ZK_DEV void store_bucket(void* buckets, size_t slot, const Xyzz<T>& p) {
+    red_store<T>(buckets, slot, Red<T>::from_std(p));
+}
+
 // ---------------------------------------------------------------- buckets
 // Bucket accumulation, G1. One thread per bucket, in the sort's accumulation
 // order (longest first, so a wave's lanes carry near-equal loads and the
 // launch has no long tail); buckets above lmax entries go to the chunked path.
 // Arithmetic in the 29-bit-limb form (field29.hpp): points come from the
-// converted copy of the bases, and each bucket is converted back to the
-// standard form when it is stored.
+// converted copy of the bases; the buckets stay in that form for the
+// reduction.
 // Thread tid of an accumulation launch over nb buckets, each split into
 // `split` parts of consecutive entries (bucket b's parts go to slots
 // b*split + h, summed by the level-1 reduction). Returns false when the thread
@@ -55,7 +138,7 @@ ZK_DEV bool acc_range(uint32_t tid, uint32_t nb, uint32_t split, const uint32_t*
     const uint32_t cnt = e0 - s0;
     slot = b * split + h;
     if (cnt > lmax) {
-        if (h != 0) PointIO<T>::store_xyzz(buckets, slot, xyzz_inf<T>());
+        if (h != 0) store_bucket<T>(buckets, slot, xyzz_inf<T>());
         return false;
     }
     st = s0 + (uint32_t)(((uint64_t)cnt * h) / split);
@@ -144,13 +227,13 @@ __global__ void __launch_bounds__(256) k_msm_merge(const uint32_t* __restrict__ 
         for (uint32_t j = threadIdx.x; j < lr.count; j += blockDim.x)
             acc = xyzz_add(acc, PointIO<T>::load_xyzz(partials, lr.first + j));
         Xyzz<T> s = block_sum<T>(acc, sh);
-        if (threadIdx.x == 0) PointIO<T>::store_xyzz(buckets, (size_t)lr.bucket * split, s);
+        if (threadIdx.x == 0) store_bucket<T>(buckets, (size_t)lr.bucket * split, s);
     }
 }
 
 // ---------------------------------------------------------------- reduction
-// The point additions here are inlined, each loop body holding one or two
-// addition sites (level 2 selects its operands per step; the step index is
+// Arithmetic in the 29-bit form (Red<T>). The point additions are inlined,
+// each loop body holding one or two addition sites (level 2 selects its operands per step; the step index is
 // uniform across the wave): out-of-line calls would pass the 128/256-byte
 // operands through scratch memory on every step of these latency-bound chains.
 //
@@ -161,20 +244,21 @@ __global__ void __launch_bounds__(256) k_msm_merge(const uint32_t* __restrict__ 
 template <class T>
 __global__ void __launch_bounds__(128) k_msm_reduce1(const void* __restrict__ buckets, uint32_t K, int W, uint32_t L1,
                                                      uint32_t split, void* __restrict__ R, void* __restrict__ Tt) {
+    using P = Red<T>;
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t nseg = K / L1;
     if (j >= (uint32_t)W * nseg) return;
     uint32_t w = j / nseg, jj = j - w * nseg;
     size_t base = (size_t)w * K + (size_t)jj * L1;
-    Xyzz<T> run = xyzz_inf<T>(), acc = xyzz_inf<T>();
+    typename P::Pt run = P::inf(), acc = P::inf();
 #pragma unroll 1
     for (int b = (int)L1 - 1; b >= 0; b--) {
 #pragma unroll 1
-        for (uint32_t h = 0; h < split; h++) run = xyzz_add_i(run, PointIO<T>::load_xyzz(buckets, (base + b) * split + h));
-        acc = xyzz_add_i(acc, run);
+        for (uint32_t h = 0; h < split; h++) run = P::add(run, red_load<T>(buckets, (base + b) * split + h));
+        acc = P::add(acc, run);
     }
-    PointIO<T>::store_xyzz(R, j, run);
-    PointIO<T>::store_xyzz(Tt, j, acc);
+    red_store<T>(R, j, run);
+    red_store<T>(Tt, j, acc);
 }
 
 // Level 2: input per window is M pairs (R_j, T_j) worth sum_j (T_j + j*U*R_j),
@@ -189,20 +273,22 @@ template <class T, int BLK>
 __global__ void __launch_bounds__(BLK) k_msm_reduce2(const void* __restrict__ Rin, const void* __restrict__ Tin, uint32_t M,
                                                      int logU, uint32_t nblk, void* __restrict__ Rout,
                                                      void* __restrict__ Tout) {
-    __shared__ Xyzz<T> sh[BLK];
+    using P = Red<T>;
+    using Pt = typename P::Pt;
+    __shared__ Pt sh[BLK];
     const int t = threadIdx.x;
     const uint32_t w = blockIdx.x / nblk, blk = blockIdx.x - w * nblk;
     const uint32_t j0 = (blk * BLK + t) * L2;
-    Xyzz<T> run = xyzz_inf<T>(), acc = xyzz_inf<T>(), tsum = xyzz_inf<T>();
+    Pt run = P::inf(), acc = P::inf(), tsum = P::inf();
     // per item, highest j first: acc += run; run += R_j; tsum += T_j
 #pragma unroll 1
     for (int step = 0; step < 3 * L2; step++) {
         const int q = L2 - 1 - step / 3, k = step % 3;
         const uint32_t j = j0 + q;
         if (j >= M) continue;
-        Xyzz<T> x = k == 0 ? acc : (k == 1 ? run : tsum);
-        Xyzz<T> y = k == 0 ? run : PointIO<T>::load_xyzz(k == 1 ? Rin : Tin, (size_t)w * M + j);
-        Xyzz<T> r = xyzz_add_i(x, y);
+        Pt x = k == 0 ? acc : (k == 1 ? run : tsum);
+        Pt y = k == 0 ? run : red_load<T>(k == 1 ? Rin : Tin, (size_t)w * M + j);
+        Pt r = P::add(x, y);
         if (k == 0) acc = r;
         else if (k == 1) run = r;
         else tsum = r;
@@ -212,23 +298,23 @@ __global__ void __launch_bounds__(BLK) k_msm_reduce2(const void* __restrict__ Ri
 #pragma unroll 1
     for (int d = 1; d < BLK; d <<= 1) {
         __syncthreads();
-        Xyzz<T> mine = sh[t];
-        Xyzz<T> other = (t + d < BLK) ? sh[t + d] : xyzz_inf<T>();
+        Pt mine = sh[t];
+        Pt other = (t + d < BLK) ? sh[t + d] : P::inf();
         __syncthreads();
-        sh[t] = xyzz_add_i(mine, other);
+        sh[t] = P::add(mine, other);
     }
     __syncthreads();
-    const Xyzz<T> S = sh[t], total_r = sh[0];
+    const Pt S = sh[t], total_r = sh[0];
     __syncthreads();
     // V_t = tsum + U * (acc + L2 * [t >= 1] * S_t): doublings and additions
     // through one site each
-    Xyzz<T> v = t >= 1 ? S : xyzz_inf<T>();
+    Pt v = t >= 1 ? S : P::inf();
 #pragma unroll 1
     for (int step = 0; step < LOG_L2 + 1 + logU + 1; step++) {
         if (step < LOG_L2 || (step > LOG_L2 && step <= LOG_L2 + logU)) {
-            v = xyzz_dbl_i(v);
+            v = P::dbl(v);
         } else {
-            v = xyzz_add_i(v, step == LOG_L2 ? acc : tsum);
+            v = P::add(v, step == LOG_L2 ? acc : tsum);
         }
     }
     // tree sum of V over the block
@@ -236,12 +322,12 @@ __global__ void __launch_bounds__(BLK) k_msm_reduce2(const void* __restrict__ Ri
 #pragma unroll 1
     for (int h = BLK >> 1; h >= 1; h >>= 1) {
         __syncthreads();
-        if (t < h) sh[t] = xyzz_add_i(sh[t], sh[t + h]);
+        if (t < h) sh[t] = P::add(sh[t], sh[t + h]);
     }
     __syncthreads();
     if (t == 0) {
-        PointIO<T>::store_xyzz(Rout, blockIdx.x, total_r);
-        PointIO<T>::store_xyzz(Tout, blockIdx.x, sh[0]);
+        PointIO<T>::store_xyzz(Rout, blockIdx.x, P::to_std(total_r));
+        PointIO<T>::store_xyzz(Tout, blockIdx.x, P::to_std(sh[0]));
     }
 }
 
