@@ -39,7 +39,11 @@ MsmGeom msm_geom(size_t n, int c, bool g2) {
     g.c = c;
     g.nlo = g.W * c - GLV_BITS;
     g.K = 1u << (c - 1);
-    g.l1 = std::min<uint32_t>(L1_MAX, g.K);
+    // level-1 segments: 4 G1 buckets up to 2^15 buckets per window (their
+    // chains, split-fold longer, set the reduction's latency), else 8, which
+    // keeps the level-2 partials the host combines to a few dozen per window
+    const uint32_t l1 = (!g2 && g.K <= 32768u) ? 4u : L1_MAX;
+    g.l1 = std::min<uint32_t>(l1, g.K);
     g.nseg = g.K / g.l1;
     uint32_t per_blk = (uint32_t)(g2 ? RedBlk<Fq2>::V : RedBlk<Fp>::V) * L2;
     g.nblk = (g.nseg + per_blk - 1) / per_blk;
