@@ -43,12 +43,18 @@ MACS_PER_FPMUL = 136                    # 8x8 products + 8x8 reduction + 8 quoti
 MADD_FPMUL = 10                         # XYZZ mixed add (madd-2008-s): 8M + 2S
 MADD_FPMUL_G2 = 30                      # the same in Fp2: 10 Fp2 products x 3 Fp products (Karatsuba)
 CALIB_BYTES = 384 << 20                 # device copy before the probes (profiler counter calibration)
-# HBM bytes per entry of the G1 accumulation kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE
-# runs of the probe (8 windows of 16 bits over the GLV halves, 16,842,520 entries:
-# 3.029 GB per launch), scaled by the calibration copy;
-# profiles/msm_accum_m2p20_hbm_counters.md
-HBM_BYTES_PER_ENTRY_G1 = 3.029e9 / 16842520
+# HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE runs of the probes
+# (`bench.py --sections none --no-cpu --no-tune --probe-reps 1`), scaled by the
+# calibration copy; profiles/probe_kernels_hbm_counters.md
+# G1 / G2 accumulation, 8 windows of 16 bits over the GLV halves, 16,842,520 entries:
+HBM_BYTES_PER_ENTRY_G1 = (2.7947e9 + 0.1066e9) / 16842520
+HBM_BYTES_PER_ENTRY_G2 = (2.6599e9 + 0.0671e9) / 16842520
 ALGO_BYTES_PER_ENTRY_G1 = 68            # y and x (or beta*x) of the base, 32 B each packed, + the 4 B entry
+ALGO_BYTES_PER_ENTRY_G2 = 132           # the same over Fp2, + the 4 B entry
+# NTT of 2^22, per transform (three pass pairs + the last pass)
+HBM_BYTES_NTT_2P22 = {"forward": 1.2249e9, "inverse": 1.3620e9, "coset": 1.3617e9}
+# SpMV of the 2^20 circuit (k_spmv_abc)
+HBM_BYTES_SPMV_2P20 = 1.3450e9 + 0.2016e9
 
 
 def log(msg):
@@ -329,6 +335,8 @@ def section_probes(args, zk, ctx, circ):
         "msm_total_ms": r3(t1[0]), "sort_ms": r3(t1[1]), "reduction_ms": r3(t1[4]),
         "g2": {"kernel": "k_msm_accum29q (G2, 29-bit limbs, accumulator in LDS)", "achieved": round(g2_achieved, 3),
                "frac": round(g2_achieved / PEAK_MAC_TS, 4), "avg_launch_ms": r3(t2[2]),
+               "traffic": int(float(t2[3]) * HBM_BYTES_PER_ENTRY_G2),
+               "algorithmic_bytes": int(float(t2[3]) * ALGO_BYTES_PER_ENTRY_G2),
                "msm_total_ms": r3(t2[0]), "reduction_ms": r3(t2[4]), "unit_def": "entry = Fp2 XYZZ mixed add = 30 Fp-mul x 136 MACs"},
     }
     ntt = probe_ntt(args, zk, ctx, 22)
@@ -336,9 +344,13 @@ def section_probes(args, zk, ctx, circ):
     gbs = {k: 64 * n / (v / 1e3) / 1e9 for k, v in ntt.items()}
     log(f"[bench] NTT 2^22 ms {ntt} -> GB/s {({k: round(v) for k, v in gbs.items()})}")
     roofline_ntt = {
-        "bound": "hbm", "kernel": "NTT 2^22 (k_ntt_pass + k_ntt_final), forward",
+        "bound": "hbm", "kernel": "NTT 2^22 (three k_ntt_pass2 launches + k_ntt_final<4>), forward",
         "achieved": round(gbs["forward"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-        "frac": round(gbs["forward"] / PEAK_HBM_GBS, 4), "traffic": None,
+        "frac": round(gbs["forward"] / PEAK_HBM_GBS, 4), "traffic": int(HBM_BYTES_NTT_2P22["forward"]),
+        "traffic_by_kind": {k: int(v) for k, v in HBM_BYTES_NTT_2P22.items()},
+        "traffic_GBps": {k: round(HBM_BYTES_NTT_2P22[k] / (v / 1e3) / 1e9, 1) for k, v in ntt.items()},
+        "note": ("the passes are bound by Fr products on the VALU, not by HBM: the profiler's traffic runs at "
+                 "under a quarter of the HBM peak"),
         "avg_launch_ms": r3(ntt["forward"]), "algorithmic_bytes": 64 * n,
         "ms": {k: r3(v) for k, v in ntt.items()},
         "GBps": {k: round(v, 1) for k, v in gbs.items()},
@@ -352,6 +364,21 @@ def section_probes(args, zk, ctx, circ):
         f"(nominal {PEAK_MAC_TS:.2f})")
     roofline["peak_measured"] = {"v_mad_u64_u32": round(mad, 2), "v_add_u32": round(add, 2), "unit": "T lane-ops/s",
                                  "probe": "8 independent chains per thread, 2048 x 16 x 8 ops, 8 waves per SIMD"}
+    # what one to eight waves per SIMD can issue: instruction streams, then the
+    # accumulation's and the reduction's point additions as dependent chains
+    waves = (1, 2, 4, 8)
+    by_waves = {}
+    for op, name, scale in ((0, "v_mad_u64_u32 (8 chains)", 1e12), (3, "v_mad_u64_u32 (1 chain)", 1e12),
+                            (2, "v_fma_f64 (8 chains)", 1e12), (1, "v_add_u32", 1e12),
+                            (4, "29-bit mixed XYZZ add (accumulation)", 1e9),
+                            (5, "29-bit general XYZZ add (reduction)", 1e9)):
+        vals = [ctx.probe_valu(op + 16 * w) / scale for w in waves]
+        by_waves[name] = [round(v, 2) for v in vals]
+        unit = "T lane-ops/s" if scale == 1e12 else "G adds/s"
+        log(f"[bench] issue probe {name}, 1/2/4/8 waves per SIMD: " + " / ".join(f"{v:.2f}" for v in vals) + f" {unit}")
+    roofline["peak_measured"]["by_waves_per_simd"] = {
+        "waves": list(waves), "rates": by_waves,
+        "units": "T lane-ops/s for instructions, G point additions/s for the XYZZ chains (1024 SIMDs)"}
     return roofline, roofline_ntt, msm22, roofline_spmv
 
 
@@ -377,7 +404,8 @@ def probe_spmv(args, zk, ctx, circ):
     gbs = algo / (ms / 1e3) / 1e9
     log(f"[bench] SpMV (A, B, AoB) alone: {ms:.3f} ms, {algo / 1e6:.1f} MB algorithmic -> {gbs:.0f} GB/s")
     return {"bound": "hbm", "kernel": "k_spmv_abc (A*w, B*w and their product, fused)", "achieved": round(gbs, 1),
-            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
+            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4),
+            "traffic": int(HBM_BYTES_SPMV_2P20), "traffic_GBps": round(HBM_BYTES_SPMV_2P20 / (ms / 1e3) / 1e9, 1),
             "avg_launch_ms": r3(ms), "algorithmic_bytes": int(algo),
             "unit_def": "36 B per nonzero, 4 B per row pointer, 32 B per output row and per witness value"}
 

@@ -320,8 +320,8 @@ int zk_field29_op_device(zk_ctx* ctx, int op, const uint32_t* a, const uint32_t*
  * zk_probe_valu: lane operations per second over the whole GPU of
  * kind & 15: 0 v_mad_u64_u32 (8 independent chains per thread), 1 v_add_u32,
  * 2 v_fma_f64 (8 chains), 3 v_mad_u64_u32 in one dependent chain, 4 / 5
- * dependent chains of the bucket accumulation's mixed XYZZ additions (29-bit
- * form) / the bucket reduction's general ones (standard form), in point
+ * dependent chains of the bucket accumulation's mixed XYZZ additions / the
+ * bucket reduction's general ones (both in the 29-bit form), in point
  * additions per second; with
  * kind >> 4 waves per SIMD (0: 8). Kind 0 is the issue-rate ceiling behind
  * the MSM roofline; the others show what a lone wave can issue.

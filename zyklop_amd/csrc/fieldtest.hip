@@ -142,54 +142,36 @@ __global__ void __launch_bounds__(256) k_valu_probe(int kind, uint32_t iters, ui
     sink[t] = s;
 }
 
-// dependent chains of point additions, 16 per iteration: op 4 mixed XYZZ
-// additions in the 29-bit form (the bucket accumulation's), op 5 general XYZZ
-// additions in the standard form (the bucket reduction's). The operands are
-// limb patterns below p, not curve points: only the instruction stream counts.
+// dependent chains of point additions in the 29-bit form, 16 per iteration:
+// op 4 mixed XYZZ additions (the bucket accumulation's), op 5 general XYZZ
+// additions (the bucket reduction's). The operands are limb patterns below p,
+// not curve points: only the instruction stream counts.
 __global__ void __launch_bounds__(64) k_point_probe(int op, uint32_t iters, uint32_t seed, uint64_t* __restrict__ sink) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t s = 0;
-    if (op == 4) {
-        Xyzz29 p;
-        F29 qx, qy;
+    Xyzz29 p, q;
 #pragma unroll
-        for (int i = 0; i < 9; i++) {
-            const uint32_t h = seed * (t * 9 + i + 1), m = i < 8 ? f29c::MASK : 0xffffu;
-            p.X.l[i] = (h * 2654435761u) & m;
-            p.Y.l[i] = (h * 40503u + 7) & m;
-            p.ZZ.l[i] = (h * 69069u + 1) & m;
-            p.ZZZ.l[i] = (h * 1103515245u + 12345u) & m;
-            qx.l[i] = p.Y.l[i] ^ 0x5555u;
-            qy.l[i] = p.ZZ.l[i] ^ 0x3333u;
-        }
-        p.inf = false;
-        for (uint32_t it = 0; it < iters; it++) {
-#pragma unroll 1
-            for (int r = 0; r < 16; r++) f29_madd(p, qx, qy);
-        }
-#pragma unroll
-        for (int i = 0; i < 9; i++) s ^= p.X.l[i] ^ p.Y.l[i] ^ p.ZZ.l[i] ^ p.ZZZ.l[i];
-    } else {
-        Xyzz<Fp> p, q;
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const uint32_t h = seed * (t * 8 + i + 1), m = i < 7 ? 0xffffffffu : 0xffffu;
-            p.X.v[i] = (h * 2654435761u) & m;
-            p.Y.v[i] = (h * 40503u + 7) & m;
-            p.ZZ.v[i] = (h * 69069u + 1) & m;
-            p.ZZZ.v[i] = (h * 1103515245u + 12345u) & m;
-            q.X.v[i] = p.Y.v[i] ^ 0x5555u;
-            q.Y.v[i] = p.ZZ.v[i] ^ 0x3333u;
-            q.ZZ.v[i] = p.X.v[i] ^ 0x0f0fu;
-            q.ZZZ.v[i] = p.ZZZ.v[i] ^ 0x00ffu;
-        }
-        for (uint32_t it = 0; it < iters; it++) {
-#pragma unroll 1
-            for (int r = 0; r < 16; r++) p = xyzz_add_i(p, q);
-        }
-#pragma unroll
-        for (int i = 0; i < 8; i++) s ^= p.X.v[i] ^ p.Y.v[i] ^ p.ZZ.v[i] ^ p.ZZZ.v[i];
+    for (int i = 0; i < 9; i++) {
+        const uint32_t h = seed * (t * 9 + i + 1), m = i < 8 ? f29c::MASK : 0xffffu;
+        p.X.l[i] = (h * 2654435761u) & m;
+        p.Y.l[i] = (h * 40503u + 7) & m;
+        p.ZZ.l[i] = (h * 69069u + 1) & m;
+        p.ZZZ.l[i] = (h * 1103515245u + 12345u) & m;
+        q.X.l[i] = p.Y.l[i] ^ 0x5555u;
+        q.Y.l[i] = p.ZZ.l[i] ^ 0x3333u;
+        q.ZZ.l[i] = p.X.l[i] ^ 0x0f0fu;
+        q.ZZZ.l[i] = p.ZZZ.l[i] ^ 0x00ffu;
     }
+    p.inf = q.inf = false;
+    for (uint32_t it = 0; it < iters; it++) {
+#pragma unroll 1
+        for (int r = 0; r < 16; r++) {
+            if (op == 4) f29_madd(p, q.X, q.Y);
+            else p = f29_xyzz_add(p, q);
+        }
+    }
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) s ^= p.X.l[i] ^ p.Y.l[i] ^ p.ZZ.l[i] ^ p.ZZZ.l[i];
     sink[t] = s;
 }
 

@@ -166,7 +166,10 @@ __global__ void __launch_bounds__(1024) k_sort_tile_scatter(const uint32_t* __re
 // `stage_cap` words when the group fits (then the block writes its whole
 // range with coalesced stores), else straight to global memory: scattered
 // 4-byte stores into a range that other blocks do not touch.
-constexpr uint32_t SORT_GROUP_LDS = 65536 - 256;   // dynamic LDS bytes of k_sort_group
+constexpr uint32_t SORT_GROUP_LDS = 65536 - 256;   // dynamic LDS bytes of k_sort_group by default
+// gfx950 gives one workgroup up to 160 KiB of LDS once the kernel asks for
+// more than the default 64 KiB
+constexpr uint32_t SORT_GROUP_LDS_MAX = 160 * 1024 - 1024;
 __global__ void __launch_bounds__(256) k_sort_group(const uint2* __restrict__ pairs, const uint32_t* __restrict__ mscan,
                                                     uint32_t nt, int s, uint32_t nb, uint32_t stage_cap,
                                                     uint32_t* __restrict__ offsets, uint32_t* __restrict__ entries) {
@@ -373,10 +376,20 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
         ZK_HIP(hipGetLastError());
     }
     // stage for about 1.25x the mean group (group sizes are sums of many
-    // digits, so far larger ones are rare and take the global path)
+    // digits, so far larger ones are rare and take the global path); above
+    // 64 KiB of LDS (groups of 2^9 buckets at 2^22 points) the kernel is
+    // allowed up to 160 KiB, or keeps 64 KiB if the runtime refuses
     ZK_REQUIRE((4u << sh) < SORT_GROUP_LDS / 2, ZK_E_INTERNAL, "sort group counters exceed the LDS budget");
-    const uint32_t stage_cap = (uint32_t)std::min<size_t>((SORT_GROUP_LDS - (4u << sh)) / 4,
-                                                          z.ne / G + z.ne / G / 4 + 256);
+    const size_t want = z.ne / G + z.ne / G / 4 + 256;
+    uint32_t lds_budget = SORT_GROUP_LDS;
+    if (4 * ((1u << sh) + want) > SORT_GROUP_LDS) {
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sort_group), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)SORT_GROUP_LDS_MAX) == hipSuccess)
+            lds_budget = SORT_GROUP_LDS_MAX;
+        else
+            (void)hipGetLastError();
+    }
+    const uint32_t stage_cap = (uint32_t)std::min<size_t>((lds_budget - (4u << sh)) / 4, want);
     k_sort_group<<<G, 256, 4u * ((1u << sh) + stage_cap), st>>>(pairs.as<uint2>(), msc, nt, sh, (uint32_t)nb, stage_cap,
                                                                 offsets.as<uint32_t>(), entries.as<uint32_t>());
     ZK_HIP(hipGetLastError());
