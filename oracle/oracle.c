@@ -22,8 +22,17 @@
  *   SpMV           closed form with Python big ints (c*R^2 convention)
  *   prove          algorithmic prover == trapdoor closed form, and the
  *                  Groth16 equation holds in the exponent (or_verify_trapdoor)
- *   zkey/wtns I/O  parity unpinned against real snarkjs files (no snarkjs
- *                  fixture exists); pinned internally by prove == closed form
+ *   zkey/wtns I/O  tests/test_zkey_layout.py parses a written zkey/wtns at
+ *                  the reference reader's offsets and checks every field
+ *                  against values fixed independently (the R1CS regenerated
+ *                  by a Python restatement of the generator, CRS points from
+ *                  exponents computed with Python big ints); no snarkjs-made
+ *                  file exists in the reference to compare bytes with
+ *   g1_progression, fr_poly_eval, fr_dot (helpers of the full-size GPU
+ *                  checks): textbook double-and-add, the DFT definition and
+ *                  Python big ints (tests/test_oracle.py)
+ * Several loops run under OpenMP (MSM windows, NTT butterflies, SpMV rows,
+ * CRS generation); or_set_threads chooses the thread count.
  */
 #include "oracle.h"
 

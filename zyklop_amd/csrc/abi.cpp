@@ -258,7 +258,7 @@ int zk_prove(zk_ctx* ctx, const zk_circuit* c, const uint64_t* witness, uint32_t
         std::lock_guard<std::mutex> lk(ctx->impl.mu);
         ZK_HIP(hipSetDevice(ctx->impl.device));
         zk::ProveSlot& sl = ctx_slot(ctx);
-        sl.enqueue(ctx->impl, *c->impl, witness, false, r, s);
+        sl.enqueue(ctx->impl, *c->impl, witness, false, r, s, true);
         sl.finish(out);
     });
 }
@@ -270,7 +270,7 @@ int zk_prove_device(zk_ctx* ctx, const zk_circuit* c, const void* d_witness, con
         std::lock_guard<std::mutex> lk(ctx->impl.mu);
         ZK_HIP(hipSetDevice(ctx->impl.device));
         zk::ProveSlot& sl = ctx_slot(ctx);
-        sl.enqueue(ctx->impl, *c->impl, (const uint64_t*)d_witness, true, r, s);
+        sl.enqueue(ctx->impl, *c->impl, (const uint64_t*)d_witness, true, r, s, true);
         sl.finish(out);
     });
 }
@@ -386,7 +386,7 @@ int zk_prove_files(zk_ctx* ctx, const char* zkey_path, const char* wtns_path, co
                    "witness has " + std::to_string(nw) + " entries, need " + std::to_string(c->n_vars));
         check_scalars(w.data(), c->n_vars);
         zk::ProveSlot& sl = ctx_slot(ctx);
-        sl.enqueue(ctx->impl, *c, w.data(), false, r, s);
+        sl.enqueue(ctx->impl, *c, w.data(), false, r, s, true);
         sl.finish(out);
     });
 }

@@ -192,12 +192,17 @@ __device__ Xyzz<T> block_sum(Xyzz<T> v, Xyzz<T>* sh) {
     return r;
 }
 
+// one-wave workgroups with a small LDS tree (8 KiB for G1, 16 KiB for G2):
+// they must find CU slots while other streams' accumulations hold most of
+// the LDS, and with uniform scalars they have nothing to do
+constexpr int CHUNK_THREADS = 64;
+
 // one workgroup per chunk of an oversized bucket (grid-stride over the chunk list)
 template <class T>
-__global__ void __launch_bounds__(256) k_msm_chunks(const uint32_t* __restrict__ entries, const void* __restrict__ points,
+__global__ void __launch_bounds__(CHUNK_THREADS) k_msm_chunks(const uint32_t* __restrict__ entries, const void* __restrict__ points,
                                                     uint32_t nbase, const uint32_t* __restrict__ counters,
                                                     const ChunkRec* __restrict__ chunks, void* __restrict__ partials) {
-    __shared__ Xyzz<T> sh[256];
+    __shared__ Xyzz<T> sh[CHUNK_THREADS];
     const uint32_t nch = counters[0];
     for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
         ChunkRec cr = chunks[ci];
@@ -216,10 +221,10 @@ __global__ void __launch_bounds__(256) k_msm_chunks(const uint32_t* __restrict__
 
 // one workgroup per oversized bucket: sum its chunk partials
 template <class T>
-__global__ void __launch_bounds__(256) k_msm_merge(const uint32_t* __restrict__ counters, const LargeRec* __restrict__ large,
+__global__ void __launch_bounds__(CHUNK_THREADS) k_msm_merge(const uint32_t* __restrict__ counters, const LargeRec* __restrict__ large,
                                                    const void* __restrict__ partials, uint32_t split,
                                                    void* __restrict__ buckets) {
-    __shared__ Xyzz<T> sh[256];
+    __shared__ Xyzz<T> sh[CHUNK_THREADS];
     const uint32_t nl = counters[1];
     for (uint32_t li = blockIdx.x; li < nl; li += gridDim.x) {
         LargeRec lr = large[li];
@@ -334,17 +339,21 @@ __global__ void __launch_bounds__(BLK) k_msm_reduce2(const void* __restrict__ Ri
 // The chunked path of the oversized buckets (lists built by the sort): their
 // sums land in bucket slot b*split before the accumulation. Queued ahead of
 // the accumulation so that these short launches run while the GPU has room,
-// instead of waiting behind the large accumulation launches of other streams.
+// instead of waiting behind the large accumulation launches of other streams;
+// a few small workgroups each (grid-stride over the lists): with uniform
+// scalars the lists are empty, and a launch that needs many CU slots would
+// wait behind other streams' accumulations while its own stream's
+// accumulation waits behind it.
 template <class T>
 static void launch_pre_t(const MsmSort& sort, const void* d_points, MsmScratch& scr, hipStream_t st, int split) {
     const bool g2 = std::is_same<T, Fq2>::value;
     const MsmGeom g = msm_geom(sort.n, sort.g.c, g2);
     scr.ensure(g, sort.n_chunk_max, g2, split);
     scr.split = split;
-    k_msm_chunks<T><<<512, 256, 0, st>>>(sort.entries.as<uint32_t>(), d_points, (uint32_t)sort.n,
+    k_msm_chunks<T><<<256, CHUNK_THREADS, 0, st>>>(sort.entries.as<uint32_t>(), d_points, (uint32_t)sort.n,
                                           sort.counters.as<uint32_t>(), sort.chunks.as<ChunkRec>(), scr.partials.p);
     ZK_HIP(hipGetLastError());
-    k_msm_merge<T><<<256, 256, 0, st>>>(sort.counters.as<uint32_t>(), sort.large.as<LargeRec>(), scr.partials.p,
+    k_msm_merge<T><<<64, CHUNK_THREADS, 0, st>>>(sort.counters.as<uint32_t>(), sort.large.as<LargeRec>(), scr.partials.p,
                                          (uint32_t)split, scr.buckets.p);
     ZK_HIP(hipGetLastError());
 }

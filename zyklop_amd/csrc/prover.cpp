@@ -354,7 +354,7 @@ void ProveSlot::reserve(Context& ctx, const Circuit& c, bool host_witness) {
 }
 
 void ProveSlot::enqueue(Context& ctx, const Circuit& c, const uint64_t* witness, bool on_dev, const uint64_t r_[4],
-                        const uint64_t s_[4]) {
+                        const uint64_t s_[4], bool alone) {
     circ = &c;
     for (int i = 0; i < 4; i++) { r[i] = r_ ? r_[i] : 0; s[i] = s_ ? s_[i] : 0; }
     ZK_REQUIRE(Fr::is_canonical(r) && Fr::is_canonical(s), ZK_E_ARG, "blinding scalars must be canonical (< r)");
@@ -374,10 +374,10 @@ void ProveSlot::enqueue(Context& ctx, const Circuit& c, const uint64_t* witness,
     uint64_t ex[16];
     blinding_tail(r, s, ex);
     write_scalars4((char*)scal.p + nv * 32, ex, s0);
-    body(ctx, c, parts);
+    body(ctx, c, parts, alone);
 }
 
-void ProveSlot::body(Context& ctx, const Circuit& c, ProofParts& out) {
+void ProveSlot::body(Context& ctx, const Circuit& c, ProofParts& out, bool alone) {
     const size_t nS = (size_t)c.n_vars + 4, n = c.n;
     NttPlan& plan = ctx.ntt_plan(c.log_n);
     window_w = ctx.window_for(0, nS);
@@ -385,24 +385,36 @@ void ProveSlot::body(Context& ctx, const Circuit& c, ProofParts& out) {
     const int sp_w1 = ctx.split_for(0, nS, false), sp_w2 = ctx.split_for(0, nS, true), sp_h = ctx.split_for(1, n, false);
     ZK_HIP(hipEventRecord(ev_scal, s0));
     // witness MSMs first: the G2 one is the longest chain of the proof, so its
-    // sort and accumulation are queued before the H pipeline. One shared sort
-    // on s1, then the four MSMs read the same sorted entries on their own
-    // streams: one MSM's latency-bound reduction overlaps another's
-    // accumulation.
-    ZK_HIP(hipStreamWaitEvent(s1, ev_scal, 0));
-    if (timed) ZK_HIP(hipEventRecord(ev[4], s1));
-    sortW.run(scal.p, nS, window_w, s1);
-    ZK_HIP(hipEventRecord(ev_done1, s1));
-    if (timed) ZK_HIP(hipEventRecord(ev[5], s1));
-    for (hipStream_t p : {s2, s3, s4}) ZK_HIP(hipStreamWaitEvent(p, ev_done1, 0));
+    // sort and accumulation are queued before the H pipeline. One shared
+    // sort, then the four MSMs read the same sorted entries on their own
+    // streams (B2 s4, B1 s2, C s3). Two layouts for the sort and A:
+    //  - batches (several proofs in flight): sort and A on s1, so each MSM
+    //    has a stream and one MSM's latency-bound reduction overlaps
+    //    another's accumulation;
+    //  - a proof alone on the device: sort on s2, A after B1 on s2, and s1
+    //    left empty. With more streams than hardware queues s1 shares its
+    //    queue with s0 (creation order), and work on it holds the H
+    //    pipeline, the proof's longest chain, behind it (plain proofs
+    //    13.8 -> 13.4 ms; in batches the other slots fill that gap and the
+    //    extra MSM stream does better).
+    alone_layout = alone;
+    hipStream_t s_sort = alone ? s2 : s1, s_a = alone ? s2 : s1;
+    ZK_HIP(hipStreamWaitEvent(s_sort, ev_scal, 0));
+    if (timed) ZK_HIP(hipEventRecord(ev[4], s_sort));
+    sortW.run(scal.p, nS, window_w, s_sort);
+    ZK_HIP(hipEventRecord(ev_done1, s_sort));
+    if (timed) ZK_HIP(hipEventRecord(ev[5], s_sort));
+    for (hipStream_t p : {s1, s2, s3, s4})
+        if (p != s_sort) ZK_HIP(hipStreamWaitEvent(p, ev_done1, 0));
     msm_launch_pre(sortW, c.B2.p, true, scrB2, s4, sp_w2);
-    msm_launch_pre(sortW, c.A.p, false, scrA, s1, sp_w1);
     msm_launch_pre(sortW, c.B1.p, false, scrB1, s2, sp_w1);
     msm_launch_pre(sortW, c.C.p, false, scrC, s3, sp_w1);
+    msm_launch_pre(sortW, c.A.p, false, scrA, s_a, sp_w1);
     msm_launch_main(sortW, true, scrB2, out.b2, s4, c.B2_29.p);
-    msm_launch_main(sortW, false, scrA, out.a, s1, c.A29.p);
+    if (!alone) msm_launch_main(sortW, false, scrA, out.a, s_a, c.A29.p);
     msm_launch_main(sortW, false, scrB1, out.b1, s2, c.B1_29.p);
     msm_launch_main(sortW, false, scrC, out.c, s3, c.C29.p);
+    if (alone) msm_launch_main(sortW, false, scrA, out.a, s_a, c.A29.p);
     // H pipeline on s0
     spmv_abc(c.rowA.as<uint32_t>(), c.colA.as<uint32_t>(), c.valA.p, c.n_rowsA, c.rowB.as<uint32_t>(),
              c.colB.as<uint32_t>(), c.valB.p, c.n_rowsB, scal.p, n, aw.p, bw.p, cw.p, s0);
@@ -520,7 +532,9 @@ double assemble_proof(const ProofParts& p, const uint64_t r[4], const uint64_t s
 }
 
 void ProveSlot::finish(zk_proof* out) {
-    const hipEvent_t ready[5] = {ev_join[0], ev_join[1], ev_join[2], ev_join[3], ev_h};
+    // A on s1 (ev_join[0]), or after B1 on s2 (ev_join[1]) in the layout of a
+    // proof alone; B1 s2, C s3, B2 s4, H s0
+    const hipEvent_t ready[5] = {alone_layout ? ev_join[1] : ev_join[0], ev_join[1], ev_join[2], ev_join[3], ev_h};
     const double tail_ms = assemble_proof(parts, r, s, out, ready);
     ZK_HIP(hipStreamSynchronize(s0));   // s1..s4 are joined into s0
     if (timed) {

@@ -74,6 +74,7 @@ struct ProveSlot {
     // Each proof forks s1..s4 from s0 and joins them back into s0 at its end.
     hipStream_t s0 = nullptr, s1 = nullptr, s2 = nullptr, s3 = nullptr, s4 = nullptr;
     hipEvent_t ev_scal = nullptr, ev_done1 = nullptr, ev_join[4] = {}, ev_h = nullptr;
+    bool alone_layout = false;   // the layout body() used for the queued proof
     DevBuf scal, aw, bw, cw, t0, t1, t2, h, wtmp;
     HostBuf staging;                 // pinned copy of a host witness (async upload)
     MsmSort sortW, sortH;
@@ -92,9 +93,11 @@ struct ProveSlot {
     ~ProveSlot();
     ProveSlot(const ProveSlot&) = delete;
     ProveSlot& operator=(const ProveSlot&) = delete;
-    // queue all device work for one proof; witness is host (canonical) or device
+    // queue all device work for one proof; witness is host (canonical) or
+    // device; alone: no other proof shares the device (the stream layout for
+    // latency, see body)
     void enqueue(Context& ctx, const Circuit& c, const uint64_t* witness, bool witness_on_device, const uint64_t r[4],
-                 const uint64_t s[4]);
+                 const uint64_t s[4], bool alone = false);
     // size every device buffer a proof of c needs in this slot
     void prepare(Context& ctx, const Circuit& c);
     // size every buffer a proof of c needs in this slot, sorts, MSM scratch
@@ -105,7 +108,7 @@ struct ProveSlot {
     // the device work of one proof once scal holds [w | 1, r, s, -rs] (queued
     // on s0 before the call): everything up to the partials in out, ending
     // with s1..s4 joined into s0. Stream-capturable after prepare().
-    void body(Context& ctx, const Circuit& c, ProofParts& out);
+    void body(Context& ctx, const Circuit& c, ProofParts& out, bool alone = false);
     // wait for the device work and run the host tail
     void finish(zk_proof* out);
     // true when every stream of the slot has drained (non-blocking)
