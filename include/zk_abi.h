@@ -331,6 +331,13 @@ int zk_field29_op_device(zk_ctx* ctx, int op, const uint32_t* a, const uint32_t*
  * its device time. */
 int zk_probe_valu(zk_ctx* ctx, int kind, double* lane_ops_per_s);
 int zk_circuit_spmv_device(zk_ctx* ctx, const zk_circuit* c, const void* d_witness, void* d_aw, void* d_bw, void* d_cw);
+/* Check of the admission arithmetic (no reference counterpart): for a fresh
+ * proof slot on ctx, out[0] = the device bytes counted for a proof of c at the
+ * context's windows and splits (what serving admission uses), out[1] = the
+ * bytes the slot holds once reserved for c, out[2] = the bytes still counted
+ * after that (0 when the count covers every buffer). The context's NTT plan
+ * for c's domain is created first and is in neither figure. */
+int zk_slot_bytes_check(zk_ctx* ctx, const zk_circuit* c, uint64_t out[3]);
 
 /* ---- host-side check of the MSM's scalar split (no GPU) ---- */
 /* No counterpart in the reference, whose MSMs run over full 254-bit scalars

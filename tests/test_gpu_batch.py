@@ -152,3 +152,34 @@ def test_bucket_splits_leave_proofs_unchanged(tmp_path):
     assert ref.verify(base, r, s, w=w)
     c.free()
     ctx.close()
+
+
+def test_admission_count_matches_reserve(tmp_path):
+    """The device bytes counted for a proof (what serving admission compares
+    with free HBM) equal what a fresh slot holds once reserved, at the default,
+    tuned and forced windows and at every bucket split; nothing is left to
+    grow afterwards."""
+    ctx = zk.Context(0)
+    c = ctx.synth_circuit(86, 5000, 16)
+
+    def check(tag):
+        counted, held, left = ctx.slot_bytes_check(c)
+        assert counted == held and left == 0, (tag, counted, held, left)
+        return held
+
+    base = check("default")
+    ctx.autotune(c)
+    check("tuned")
+    path = tmp_path / "t.txt"
+    ctx.tune_save(path)
+    lines = [ln.split() for ln in path.read_text().splitlines() if not ln.startswith("#")]
+    for sg1, sg2 in [(1, 1), (3, 2), (4, 4)]:
+        path.write_text("".join(f"{a} {k} {cls} {cw} {sg1} {sg2}\n" for a, k, cls, cw, _, _ in lines))
+        assert ctx.tune_load(path) == 2
+        check(("splits", sg1, sg2))
+    for cw, ch in [(13, 12), (22, 22)]:
+        ctx.set_windows(cw, ch)
+        held = check(("forced", cw, ch))
+    assert held > base                        # 2^21 buckets per window at c = 22
+    c.free()
+    ctx.close()

@@ -135,6 +135,7 @@ ABI = {
     "zk_field29_op_device": (ctypes.c_int, [_vp, ctypes.c_int, _u32p, _u32p, _u32p, ctypes.c_size_t]),
     "zk_probe_valu": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "zk_circuit_spmv_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "zk_slot_bytes_check": (ctypes.c_int, [_vp, _vp, _u64p]),
 }
 
 
@@ -313,6 +314,13 @@ class Context:
         r = ctypes.c_double(0)
         _check(lib().zk_probe_valu(self.h, kind, ctypes.byref(r)))
         return r.value
+
+    def slot_bytes_check(self, circuit):
+        """(counted device bytes of a fresh slot for a proof of circuit, bytes the slot
+        holds once reserved, bytes still counted after that)."""
+        out = np.zeros(3, np.uint64)
+        _check(lib().zk_slot_bytes_check(self.h, circuit.h, u64p(out)))
+        return int(out[0]), int(out[1]), int(out[2])
 
     def spmv_device(self, circuit, d_witness, d_aw, d_bw, d_cw):
         """The circuit's fused SpMV alone on the device; returns its device ms."""

@@ -1027,6 +1027,21 @@ int zk_probe_valu(zk_ctx* ctx, int kind, double* lane_ops_per_s) {
     });
 }
 
+int zk_slot_bytes_check(zk_ctx* ctx, const zk_circuit* c, uint64_t out[3]) {
+    return guard([&] {
+        need(ctx && c && out, "NULL argument");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        const zk::Circuit& ci = *c->impl;
+        ctx->impl.ntt_plan(ci.log_n);
+        zk::ProveSlot sl;
+        out[0] = zk::ProveSlot::bytes_needed(ctx->impl, ci);
+        sl.reserve(ctx->impl, ci, false);
+        out[1] = sl.device_bytes();
+        out[2] = sl.grow_bytes(ctx->impl, ci);
+    });
+}
+
 int zk_circuit_spmv_device(zk_ctx* ctx, const zk_circuit* c, const void* d_witness, void* d_aw, void* d_bw, void* d_cw) {
     return guard([&] {
         need(ctx && c && d_witness && d_aw && d_bw && d_cw, "NULL argument");
