@@ -10,14 +10,17 @@ proofs of the same inputs):
       trapdoor CRS resident in HBM.
         plain   : one proof at a time (zk_prove_device, stream-synchronised)
         batched : the same proofs with several in flight, as streams
-                  (zk_prove_batch_device) and as one replayed HIP graph
-                  (zk_batch_graph_*); `value` is the faster of the two.
+                  (zk_prove_batch_device); the captured-graph variant
+                  (zk_batch_graph_*) is tried and reported as not run while
+                  the library keeps it switched off. `value` is the faster
+                  of the batched variants that ran.
       Kernel probes (alone on the device): the G1 and G2 witness MSMs over the
       resident bases (roofline of the bucket-accumulation kernel), NTTs of
       2^22 (forward, inverse, coset) and a G1 MSM of 2^22 points.
   mixed: 64 proofs, m log-uniform in [2^14, 2^20], 1024 public inputs; half
       over 4 shared circuits (shared CRS), half over 32 circuits of their own;
-      windows from the autotuner; plain vs one captured graph vs streams.
+      windows from the autotuner; plain vs streams (vs one captured graph,
+      when the library enables it).
   served: Poisson arrivals against 4 resident circuits (m = 2^18, 2^20, 2^22,
       2^24, 1024 public inputs), continuous batching (zk_server_*), at three
       arrival rates set from the measured plain capacity; proofs/s, p50/p99.

@@ -245,8 +245,9 @@ int zk_msm_g2_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, s
  * serving loops, harnesses that forbid hipMalloc). Its whole device workspace
  * is a fixed 64 MiB array that the library holds on every device it runs on
  * (enough for 65536 points at the default window); larger workspaces are
- * refused with ZK_E_ARG. The window values go to a fixed host array. Calls
- * on one device share both arrays: calls that overlap on one device race.
+ * refused with ZK_E_ARG. The per-block partial sums of every window go to a
+ * fixed host array. Calls on one device share both arrays: calls that
+ * overlap on one device race.
  * Results are identical to zk_msm_g1_device's. */
 int zk_msm_g1_device_noalloc(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t n, int window,
                              uint64_t out[8], int* is_inf);
