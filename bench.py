@@ -49,15 +49,16 @@ CALIB_BYTES = 384 << 20                 # device copy before the probes (profile
 # HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE runs of the probes
 # (`bench.py --sections none --no-cpu --no-tune --probe-reps 1`), scaled by the
 # calibration copy; profiles/probe_kernels_hbm_counters.md
-# G1 / G2 accumulation, 8 windows of 16 bits over the GLV halves, 16,842,520 entries:
-HBM_BYTES_PER_ENTRY_G1 = (2.7947e9 + 0.1066e9) / 16842520
-HBM_BYTES_PER_ENTRY_G2 = (2.6599e9 + 0.0671e9) / 16842520
+# G1 / G2 accumulation over the window tables of the 2^20 circuit (7 windows
+# of 18-19 bits over the GLV halves sharing one bucket set), 14,737,409 entries:
+HBM_BYTES_PER_ENTRY_G1 = (2.7032e9 + 0.1070e9) / 14737409
+HBM_BYTES_PER_ENTRY_G2 = (2.3905e9 + 0.0671e9) / 14737409
 ALGO_BYTES_PER_ENTRY_G1 = 68            # y and x (or beta*x) of the base, 32 B each packed, + the 4 B entry
 ALGO_BYTES_PER_ENTRY_G2 = 132           # the same over Fp2, + the 4 B entry
 # NTT of 2^22, per transform (three pass pairs + the last pass)
-HBM_BYTES_NTT_2P22 = {"forward": 1.2249e9, "inverse": 1.3620e9, "coset": 1.3617e9}
+HBM_BYTES_NTT_2P22 = {"forward": 1.2245e9, "inverse": 1.3620e9, "coset": 1.3616e9}
 # SpMV of the 2^20 circuit (k_spmv_abc)
-HBM_BYTES_SPMV_2P20 = 1.3450e9 + 0.2016e9
+HBM_BYTES_SPMV_2P20 = 1.3491e9 + 0.2017e9
 
 
 def log(msg):
@@ -172,7 +173,8 @@ def tuned_table(ctx):
         path = os.path.join(d, "windows.txt")
         ctx.tune_save(path)
         rows = [ln.split() for ln in open(path) if ln.strip() and not ln.startswith("#")]
-    return [{"kind": "witness" if r[1] == "0" else "h", "log2_n": int(r[2]), "c": int(r[3]),
+    kinds = {"0": "witness", "1": "h", "2": "witness (window tables)", "3": "h (window tables)"}
+    return [{"kind": kinds.get(r[1], r[1]), "log2_n": int(r[2]), "c": int(r[3]),
              "split_g1": int(r[4]), "split_g2": int(r[5])} for r in rows]
 
 
@@ -268,7 +270,9 @@ def section_headline(args, zk, ctx, circ, tuned):
 
 # ---------------------------------------------------------------- kernel probes
 def probe_msm(args, zk, ctx, circ, which, c_w, g2):
-    """Witness MSM over resident bases, alone on the device: (total, sort, accumulation ms, entries)."""
+    """Witness MSM over resident bases through the arrays a proof reads (the
+    window tables when the circuit has them), alone on the device:
+    (total, sort, accumulation ms, entries, reduction ms)."""
     d_bases, n_ext = circ.bases(which)
     w0 = circ.witness(args.seed * 1000 + 1)
     ext = np.ascontiguousarray(np.concatenate([w0, zk.synth_uniform(args.seed + 7, 4)]), dtype=np.uint64)
@@ -276,9 +280,8 @@ def probe_msm(args, zk, ctx, circ, which, c_w, g2):
     d_sc = ctx.dev_alloc(ext.nbytes)
     ctx.upload(d_sc, ext)
     rows = []
-    fn = ctx.msm_g2_device if g2 else ctx.msm_g1_device
     for i in range(2 + args.probe_reps):
-        fn(d_bases, d_sc, n_ext, c_w)
+        ctx.circuit_msm_device(circ, which, d_sc, c_w)
         if i >= 2:
             rows.append(ctx.last_timing())
     ctx.dev_free(d_sc)
@@ -306,7 +309,8 @@ def probe_ntt(args, zk, ctx, log_n):
 def section_probes(args, zk, ctx, circ):
     """Kernels alone on the device, before any proof. A device copy of a known
     size first gives the HBM-counter calibration point for profiler runs."""
-    c_w = ctx.tuned_window(0, circ.n_vars + 4)
+    tabs = ctx.circuit_window_tables(circ)
+    c_w = tabs[0] or ctx.tuned_window(0, circ.n_vars + 4)
     nb = CALIB_BYTES
     a, b = ctx.dev_alloc(nb), ctx.dev_alloc(nb)
     ctx.dev_copy(b, a, nb)
@@ -333,7 +337,8 @@ def section_probes(args, zk, ctx, circ):
         "avg_launch_ms": round(t_launch * 1e3, 4), "units_per_launch": int(units),
         "unit_def": ("one signed-digit (point, bucket) entry = one XYZZ mixed add = 10 Fp products x 136 32x32 "
                      "MACs (the algorithmic count of a 256-bit Montgomery product, whatever the limb size)"),
-        "probe": (f"G1 MSM over the resident A' bases, n={n_ext}, c={c_w}, alone on the device, "
+        "probe": (f"G1 MSM over the resident A' bases, n={n_ext}, c={c_w}"
+                  f"{' over window tables (one bucket set)' if tabs[0] else ''}, alone on the device, "
                   f"{args.probe_reps} launches, HIP events on the MSM stream"),
         "msm_total_ms": r3(t1[0]), "sort_ms": r3(t1[1]), "reduction_ms": r3(t1[4]),
         "g2": {"kernel": "k_msm_accum29q (G2, 29-bit limbs, accumulator in LDS)", "achieved": round(g2_achieved, 3),
@@ -691,6 +696,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: every core this process may use)")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tables", choices=("auto", "on", "off"), default="auto",
+                    help="window tables of the resident circuits (default: on when they fit)")
     ap.add_argument("--probe-reps", type=int, default=5)
     ap.add_argument("--sections", default="headline,mixed,served",
                     help="comma list of: headline, mixed, served (the kernel probes always run)")
@@ -715,6 +722,7 @@ def main():
 
     ctx = zk.Context(0)
     ctx.set_batch_slots(args.slots)
+    ctx.set_window_tables({"auto": -1, "on": 1, "off": 0}[args.tables])
     circ, tuned = setup_headline(args, zk, ctx)
     roofline, roofline_ntt, msm22, roofline_spmv = section_probes(args, zk, ctx, circ)
     m, n_pub, K = 1 << args.log_m, args.n_pub, args.steps
@@ -733,7 +741,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"groth16_bn254_prove_m2^{args.log_m}_pub{n_pub}", "constraints": m,
                    "n_public": n_pub, "n_vars": circ.n_vars, "domain": circ.n, "batched_slots": args.slots,
-                   "blinding": "fresh r,s"},
+                   "blinding": "fresh r,s", "window_tables": list(ctx.circuit_window_tables(circ))},
     }
     if "headline" in sections:
         head = section_headline(args, zk, ctx, circ, tuned)

@@ -88,6 +88,24 @@ int zk_prove_device(zk_ctx* ctx, const zk_circuit* c, const void* d_witness, con
  * (include/msm_g1.cuh:11, MSM_G1_C_BITS). */
 int zk_set_windows(zk_ctx* ctx, int window_witness, int window_h);
 
+/* Window tables (no reference counterpart: the reference runs every MSM over
+ * its bases as given, include/msm_g1.cuh:714-913). A circuit's resident
+ * bases can also be held as W tables, table w holding 2^(offset of window w)
+ * times every base, so that the MSMs of a proof put the digits of all windows
+ * into one set of buckets: fewer, wider windows for the same reduction work.
+ * Costs (W - 1) more copies of the 29-bit base arrays in device memory
+ * (W = 6..10, by size). Proofs are the same either way.
+ * zk_set_window_tables: mode for circuits loaded on ctx from now on: 1 on,
+ * 0 off, -1 (default) on when the tables take at most a third of the device
+ * memory still free.
+ * zk_circuit_window_tables: builds (1) or frees (0) c's tables, or decides
+ * as above (-1), or only reports (-2); windows (may be NULL) receives the
+ * tables' (witness, h) windows, 0 when c has none. Waits for the device: no
+ * proof of c may be in flight. A window forced with zk_set_windows whose
+ * window count differs from the tables' proves without them. */
+int zk_set_window_tables(zk_ctx* ctx, int mode);
+int zk_circuit_window_tables(zk_ctx* ctx, zk_circuit* c, int mode, int windows[2]);
+
 /* MSM window autotuner (no reference counterpart: the reference's window is a
  * compile-time constant, include/msm_g1.cuh:11). For circuit c it measures on
  * the device, over c's resident bases and uniform scalars, the digit sort plus
@@ -98,7 +116,8 @@ int zk_set_windows(zk_ctx* ctx, int window_witness, int window_h);
  * window, with its G1 and G2 splits, is recorded per (kind, ceil log2 n) in
  * the context's table, which every later proof on ctx uses unless
  * zk_set_windows forces one. force = 0 keeps size classes already in the
- * table.
+ * table. For a circuit with window tables the window is the tables' and only
+ * the splits are tuned.
  * windows (may be NULL) receives the (witness, h) windows now in effect. */
 int zk_autotune(zk_ctx* ctx, const zk_circuit* c, int force, int windows[2]);
 /* window a proof on ctx would use for an MSM of kind (0 witness, 1 h) and size n */
@@ -262,6 +281,16 @@ int zk_last_timing(zk_ctx* ctx, float ms[5]);
  * 1 B1', 2 B2' (G2), 3 C', 4 H; the primed arrays carry the 4 extra slots
  * described in the prover (alpha/beta/delta terms) after the n_vars bases */
 int zk_circuit_bases(const zk_circuit* c, int which, void** d_ptr, uint64_t* count);
+/* MSM of d_scalars (count of zk_circuit_bases, canonical, device) over the
+ * resident base array `which` (as in zk_circuit_bases; 2 is G2, out[16]),
+ * through the arrays the prover itself reads: the window tables when c has
+ * them (window 0, or one with the tables' window count), else the 29-bit
+ * copy at `window` (0 = tuned or default). Same result as zk_msm_g1_device /
+ * zk_msm_g2_device over zk_circuit_bases' points; timing in zk_last_timing.
+ * No counterpart in the reference (its MSMs run inside groth16Prove,
+ * src/groth16.cu:1143-1397). */
+int zk_circuit_msm_device(zk_ctx* ctx, const zk_circuit* c, int which, const void* d_scalars, int window,
+                          uint64_t out[16], int* is_inf);
 
 /* ---- synthetic workload (seeded; see the generator's recipe) ---- */
 /* builds the R1CS + trapdoor setup for (seed, m, n_pub) and makes it resident */

@@ -93,6 +93,7 @@ def test_batch_graph_rejects_bad_blinding(setup):
 
 def test_autotune_table(tmp_path):
     ctx = zk.Context(0)                       # fresh context: empty table
+    ctx.set_window_tables(0)                  # the window search (tables fix the window)
     c = ctx.synth_circuit(84, 2000, 8)
     w = c.witness(5)
     r, s = zk.synth_blinding(3)
@@ -120,6 +121,19 @@ def test_autotune_table(tmp_path):
     with pytest.raises(zk.ZkError) as e:
         ctx2.tune_load(tmp_path / "missing.txt")
     assert e.value.code == 3
+    # over window tables the window is the tables' and only the splits are
+    # tuned, under kinds 2 and 3
+    ctx.set_windows(0, 0)
+    ctx.set_window_tables(-1)
+    ct = ctx.synth_circuit(87, 2000, 8)
+    tabs = ctx.circuit_window_tables(ct)
+    assert tabs[0] and tabs[1] and ctx.autotune(ct) == tabs
+    ctx.tune_save(path)
+    kinds = sorted(int(ln.split()[1]) for ln in path.read_text().splitlines() if not ln.startswith("#"))
+    assert kinds == [0, 1, 2, 3]
+    assert np.array_equal(ctx.prove(ct, ct.witness(5), r, s).words(),
+                          ctx.prove(ct, ct.witness(5), r, s).words())
+    ct.free()
     c.free()
     ctx2.close()
     ctx.close()

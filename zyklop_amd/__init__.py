@@ -62,6 +62,8 @@ ABI = {
     "zk_prove": (ctypes.c_int, [_vp, _vp, _u64p, ctypes.c_uint32, _u64p, _u64p, ctypes.POINTER(Proof)]),
     "zk_prove_device": (ctypes.c_int, [_vp, _vp, _vp, _u64p, _u64p, ctypes.POINTER(Proof)]),
     "zk_set_windows": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "zk_set_window_tables": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "zk_circuit_window_tables": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "zk_autotune": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "zk_tune_get": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
     "zk_tune_load": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
@@ -118,6 +120,8 @@ ABI = {
                                                 ctypes.POINTER(ctypes.c_int)]),
     "zk_last_timing": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
     "zk_circuit_bases": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_uint64)]),
+    "zk_circuit_msm_device": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _u64p,
+                                             ctypes.POINTER(ctypes.c_int)]),
     "zk_synth_circuit": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_vp)]),
     "zk_synth_witness": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, _u64p]),
     "zk_circuit_witness": (ctypes.c_int, [_vp, ctypes.c_uint64, _u64p]),
@@ -250,6 +254,17 @@ class Context:
     def set_windows(self, w_witness=0, w_h=0):
         _check(lib().zk_set_windows(self.h, w_witness, w_h))
 
+    def set_window_tables(self, mode):
+        """Window tables for circuits loaded from now on: 1 on, 0 off, -1 auto (default)."""
+        _check(lib().zk_set_window_tables(self.h, mode))
+
+    def circuit_window_tables(self, circuit, mode=-2):
+        """Build (1), free (0), decide (-1) or only report (-2) circuit's window tables;
+        returns their (witness, h) windows, 0 for none."""
+        w = (ctypes.c_int * 2)()
+        _check(lib().zk_circuit_window_tables(self.h, circuit.h, mode, w))
+        return int(w[0]), int(w[1])
+
     # ---- device buffers ----
     def dev_alloc(self, nbytes):
         p = _vp()
@@ -282,6 +297,15 @@ class Context:
         inf = ctypes.c_int(0)
         _check(lib().zk_msm_g1_device(self.h, d_points, d_scalars, n, window, u64p(out), ctypes.byref(inf)))
         return out, bool(inf.value)
+
+    def circuit_msm_device(self, circuit, which, d_scalars, window=0):
+        """MSM over a circuit's resident base array (0 A', 1 B1', 2 B2' (G2), 3 C', 4 H) through
+        the arrays the prover reads (window tables when present); (words, is_inf)."""
+        out = np.zeros(16, np.uint64)
+        inf = ctypes.c_int(0)
+        _check(lib().zk_circuit_msm_device(self.h, circuit.h, which, d_scalars, window, u64p(out),
+                                           ctypes.byref(inf)))
+        return (out if which == 2 else out[:8].copy()), bool(inf.value)
 
     def msm_g1_device_noalloc(self, d_points, d_scalars, n, window=0):
         """G1 MSM on device buffers with the library's fixed scratch (no allocation)."""

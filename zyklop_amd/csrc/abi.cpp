@@ -144,8 +144,10 @@ void put_result(const zkh::Xyzz<HT>& p, uint64_t* out, int* is_inf) {
     memcpy(out + lw, &a.y, 8 * lw);
 }
 
+// MSM over device points; slots29, when given, are the points' 29-bit slots
+// already made (a circuit's resident arrays, W window tables when merged)
 void msm_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t n, int window, bool g2, uint64_t* out,
-                int* is_inf) {
+                int* is_inf, const void* slots29 = nullptr, bool merged = false) {
     zk::Context& C = ctx->impl;
     hipStream_t st = C.stream;
     zk::MsmSort sort;
@@ -154,14 +156,15 @@ void msm_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t
     scr.enable_timing();
     // buckets are accumulated in the 29-bit-limb form: convert the points first
     zk::DevBuf p29;
-    if (n) {
+    if (n && !slots29) {
         p29.ensure(n * (g2 ? zk::PT29Q_BYTES : zk::PT29_BYTES));
         zk::msm_points_to29(d_points, n, p29.p, g2, st);
     }
     ZK_HIP(hipEventRecord(ctx->ev0, st));
-    sort.run(d_scalars, n, window, st);
+    sort.run(d_scalars, n, window, st, merged);
     ZK_HIP(hipEventRecord(ctx->ev1, st));
-    zk::msm_launch(sort, d_points, g2, scr, part, st, p29.p, zk::msm_default_split(n, sort.g.c, g2));
+    zk::msm_launch(sort, g2, scr, part, st, slots29 ? slots29 : p29.p,
+                   zk::msm_default_split(n, sort.g.c, g2, merged));
     ZK_HIP(hipEventRecord(ctx->ev2, st));
     ZK_HIP(hipStreamSynchronize(st));
     ZK_HIP(hipEventElapsedTime(&ctx->last_ms[0], ctx->ev0, ctx->ev2));
@@ -169,7 +172,7 @@ void msm_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t
     ZK_HIP(hipEventElapsedTime(&ctx->last_ms[2], scr.ev_acc0, scr.ev_acc1));
     ZK_HIP(hipEventElapsedTime(&ctx->last_ms[4], scr.ev_acc1, scr.ev_end));
     uint32_t entries = 0;
-    ZK_HIP(hipMemcpy(&entries, sort.offsets.as<uint32_t>() + (size_t)sort.g.W * sort.g.K, 4, hipMemcpyDeviceToHost));
+    ZK_HIP(hipMemcpy(&entries, sort.offsets.as<uint32_t>() + (size_t)sort.g.Wr * sort.g.K, 4, hipMemcpyDeviceToHost));
     ctx->last_ms[3] = (float)entries;
     if (g2) put_result(zk::msm_finish_g2(part), out, is_inf);
     else put_result(zk::msm_finish_g1(part), out, is_inf);
@@ -283,6 +286,27 @@ int zk_set_windows(zk_ctx* ctx, int window_witness, int window_h) {
         std::lock_guard<std::mutex> lk(ctx->impl.mu);
         ctx->impl.force_w = window_witness;
         ctx->impl.force_h = window_h;
+    });
+}
+
+int zk_set_window_tables(zk_ctx* ctx, int mode) {
+    return guard([&] {
+        need(ctx != nullptr, "NULL ctx");
+        need(mode >= -1 && mode <= 1, "mode must be -1 (auto), 0 (off) or 1 (on)");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ctx->impl.tables_mode = mode;
+    });
+}
+
+int zk_circuit_window_tables(zk_ctx* ctx, zk_circuit* c, int mode, int windows[2]) {
+    return guard([&] {
+        need(ctx && c, "NULL argument");
+        need(mode >= -2 && mode <= 1, "mode must be -2 (query), -1 (auto), 0 (off) or 1 (on)");
+        need(c->impl->ctx == &ctx->impl, "circuit belongs to another context");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        if (mode != -2) zk::circuit_set_tables(*c->impl, mode);
+        if (windows) { windows[0] = c->impl->tab_c[0]; windows[1] = c->impl->tab_c[1]; }
     });
 }
 
@@ -720,6 +744,28 @@ int zk_msm_g2_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, s
         std::lock_guard<std::mutex> lk(ctx->impl.mu);
         ZK_HIP(hipSetDevice(ctx->impl.device));
         msm_device(ctx, d_points, d_scalars, n, window, true, out, is_inf);
+    });
+}
+
+int zk_circuit_msm_device(zk_ctx* ctx, const zk_circuit* c, int which, const void* d_scalars, int window,
+                          uint64_t out[16], int* is_inf) {
+    return guard([&] {
+        need(ctx && c && out && d_scalars, "NULL argument");
+        need(which >= 0 && which <= 4, "which must be 0..4 (A, B1, B2, C, H)");
+        need(window == 0 || (window >= 4 && window <= 22), "window must be 0 or 4..22");
+        need(c->impl->ctx == &ctx->impl, "circuit belongs to another context");
+        std::lock_guard<std::mutex> lk(ctx->impl.mu);
+        ZK_HIP(hipSetDevice(ctx->impl.device));
+        const zk::Circuit& k = *c->impl;
+        const zk::DevBuf* std_pts[5] = {&k.A, &k.B1, &k.B2, &k.C, &k.H};
+        const zk::DevBuf* slots[5] = {&k.A29, &k.B1_29, &k.B2_29, &k.C29, &k.H29};
+        const int kind = which == 4 ? 1 : 0;
+        const size_t n = which == 4 ? (size_t)k.n : (size_t)k.n_vars + 4;
+        const int tc = k.tab_c[kind];
+        // over the window tables unless a window with another window count is asked for
+        const bool merged = tc && (!window || zk::msm_geom(n, window, false).W == zk::msm_geom(n, tc, false).W);
+        const int cw = merged ? tc : (window ? window : ctx->impl.window_for(kind, n));
+        msm_device(ctx, std_pts[which]->p, d_scalars, n, cw, which == 2, out, is_inf, slots[which]->p, merged);
     });
 }
 

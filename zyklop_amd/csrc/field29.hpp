@@ -467,9 +467,11 @@ ZK_DEV Xyzz29 xyzz29_unpack(const uint32_t w[32]) {
 // P and its endomorphism image lambda*P = (beta*x, y), each as a 64-byte half
 // holding its coordinates in this form packed into 8 words (canonical, < p):
 //   words 0..7 y, 8..15 x | 16..23 y, 24..31 beta*x.
-// Base index i < n reads the first half of slot i; n <= i < 2n the second
-// half of slot i - n. A gather thus touches one aligned 64-byte half.
+// A bucket entry names its base as slot | half << 30 (bit 31 is the digit's
+// sign): half 0 reads the first half of the slot, half 1 the second. A
+// gather thus touches one aligned 64-byte half.
 constexpr int PT29_SLOT_UINT4 = 8;
+constexpr uint32_t PT29_SLOT_MASK = 0x3fffffffu;
 
 ZK_DEV F29 f29_load_packed(const uint4* q) {
     const uint4 a = q[0], b = q[1];
@@ -477,10 +479,9 @@ ZK_DEV F29 f29_load_packed(const uint4* q) {
     return f29_unpack(w);
 }
 
-ZK_DEV void f29_load_point(const void* base, uint32_t idx, uint32_t n, F29& x, F29& y) {
-    const bool phi = idx >= n;
-    const uint4* p = reinterpret_cast<const uint4*>(base) + (size_t)PT29_SLOT_UINT4 * (phi ? idx - n : idx) +
-                     (phi ? 4 : 0);
+ZK_DEV void f29_load_point(const void* base, uint32_t ref, F29& x, F29& y) {
+    const uint4* p = reinterpret_cast<const uint4*>(base) + (size_t)PT29_SLOT_UINT4 * (ref & PT29_SLOT_MASK) +
+                     ((ref >> 28) & 4u);
     y = f29_load_packed(p);
     x = f29_load_packed(p + 2);
 }
@@ -633,14 +634,13 @@ ZK_DEV Xyzz29q xyzz29q_unpack(const uint32_t w[64]) {
 }
 
 // One 256-byte slot per G2 base, halves as for G1 (one 128-byte line per
-// gather): words 0..15 y (c0, c1), 16..31 x | 32..47 y, 48..63 beta*x; base
-// index i >= n reads the second half of slot i - n.
+// gather): words 0..15 y (c0, c1), 16..31 x | 32..47 y, 48..63 beta*x;
+// entries name slot and half as for G1.
 constexpr int PT29Q_SLOT_UINT4 = 16;
 
-ZK_DEV void f29q_load_point(const void* base, uint32_t idx, uint32_t n, F29q& x, F29q& y) {
-    const bool phi = idx >= n;
-    const uint4* p = reinterpret_cast<const uint4*>(base) + (size_t)PT29Q_SLOT_UINT4 * (phi ? idx - n : idx) +
-                     (phi ? 8 : 0);
+ZK_DEV void f29q_load_point(const void* base, uint32_t ref, F29q& x, F29q& y) {
+    const uint4* p = reinterpret_cast<const uint4*>(base) + (size_t)PT29Q_SLOT_UINT4 * (ref & PT29_SLOT_MASK) +
+                     ((ref >> 27) & 8u);
     y.c0 = f29_load_packed(p);
     y.c1 = f29_load_packed(p + 2);
     x.c0 = f29_load_packed(p + 4);

@@ -20,15 +20,15 @@ int msm_default_c(size_t n) {
     return c;
 }
 
-int msm_default_split(size_t n, int c, bool g2) {
-    const MsmGeom g = msm_geom(n, c, g2);
+int msm_default_split(size_t n, int c, bool g2, bool merged) {
+    const MsmGeom g = msm_geom(n, c, g2, merged);
     // resident waves per SIMD: G1 accumulation 6 (80 VGPRs), G2 about 3 (LDS)
     const size_t target = (size_t)1024 * 64 * 2 * (g2 ? 3 : 6);
-    const size_t nb = (size_t)g.W * g.K;
+    const size_t nb = (size_t)g.Wr * g.K;
     return (int)std::max<size_t>(1, std::min<size_t>(3, target / nb));
 }
 
-MsmGeom msm_geom(size_t n, int c, bool g2) {
+MsmGeom msm_geom(size_t n, int c, bool g2, bool merged) {
     if (c <= 0) c = msm_default_c(n);
     ZK_REQUIRE(c >= 4 && c <= 22, ZK_E_ARG, "MSM window must be in [4, 22]");
     MsmGeom g;
@@ -38,16 +38,34 @@ MsmGeom msm_geom(size_t n, int c, bool g2) {
     c = (GLV_BITS + g.W - 1) / g.W;
     g.c = c;
     g.nlo = g.W * c - GLV_BITS;
+    g.merged = merged;
+    g.Wr = merged ? 1 : g.W;
     g.K = 1u << (c - 1);
-    // level-1 segments: 4 G1 buckets up to 2^15 buckets per window (their
+    // level-1 segments: 4 G1 buckets up to 2^18 buckets in all (their
     // chains, split-fold longer, set the reduction's latency), else 8, which
-    // keeps the level-2 partials the host combines to a few dozen per window
-    const uint32_t l1 = (!g2 && g.K <= 32768u) ? 4u : L1_MAX;
+    // keeps the level-2 partials the host combines to a few dozen per set
+    const uint32_t l1 = (!g2 && (size_t)g.Wr * g.K <= (1u << 18)) ? 4u : L1_MAX;
     g.l1 = std::min<uint32_t>(l1, g.K);
     g.nseg = g.K / g.l1;
     uint32_t per_blk = (uint32_t)(g2 ? RedBlk<Fq2>::V : RedBlk<Fp>::V) * L2;
     g.nblk = (g.nseg + per_blk - 1) / per_blk;
     return g;
+}
+
+int msm_table_c(size_t n) {
+    // relative costs: a mixed addition per entry 10 products, a bucket two
+    // general additions of 14
+    int best = 0;
+    double best_cost = 0;
+    for (int c = 11; c <= 22; c++) {
+        const MsmGeom g = msm_geom(n, c, false, true);
+        const double cost = 2.0 * (double)n * g.W * 10 + (double)g.K * 28;
+        if (!best || cost < best_cost) {
+            best = g.c;
+            best_cost = cost;
+        }
+    }
+    return best;
 }
 
 // ---------------------------------------------------------------- GLV split
@@ -108,15 +126,16 @@ struct DigitIter {
 };
 
 // ---------------------------------------------------------------- digit sort
-// Two-pass LDS radix sort of the flat bucket ids b = w*K + |digit| - 1 of all
-// nonzero digits (no global atomics). Buckets are grouped by b >> s.
+// Two-pass LDS radix sort of the flat bucket ids b = w*Kw + |digit| - 1 of all
+// nonzero digits (no global atomics; Kw = K, or 0 when all windows share one
+// set of buckets). Buckets are grouped by b >> s.
 // Pass A, per tile of scalars: an LDS histogram of groups goes to a count
 // matrix [group][tile]; a device scan of the matrix gives every tile its run
 // inside every group; the tile then writes (entry, b) pairs at run + LDS rank.
 // Pass B, per group: LDS counts of its 2^s buckets give the bucket offsets,
 // then the entries go to offset + LDS rank.
 __global__ void __launch_bounds__(1024) k_sort_tile_hist(const uint32_t* __restrict__ sc, size_t n, size_t tile, int c,
-                                                         int nlo, int W, uint32_t K, int s, uint32_t G, uint32_t nt,
+                                                         int nlo, int W, uint32_t Kw, int s, uint32_t G, uint32_t nt,
                                                          uint32_t* __restrict__ matrix) {
     extern __shared__ uint32_t lh[];
     for (uint32_t k = threadIdx.x; k < G; k += blockDim.x) lh[k] = 0;
@@ -128,16 +147,20 @@ __global__ void __launch_bounds__(1024) k_sort_tile_hist(const uint32_t* __restr
         for (int w = 0; w < W; w++) {
             bool ng;
             uint32_t mag = it.next(ng);
-            if (mag) atomicAdd(&lh[((uint32_t)w * K + mag - 1) >> s], 1u);
+            if (mag) atomicAdd(&lh[((uint32_t)w * Kw + mag - 1) >> s], 1u);
         }
     }
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < G; k += blockDim.x) matrix[(size_t)k * nt + blockIdx.x] = lh[k];
 }
 
+// n here counts half scalars (2 per scalar); entry of half i in window w:
+// slot (i mod n/2) + w*sw (sw = n/2 for window tables, else 0), half
+// i >= n/2, sign
 __global__ void __launch_bounds__(1024) k_sort_tile_scatter(const uint32_t* __restrict__ sc, size_t n, size_t tile, int c,
-                                                            int nlo, int W, uint32_t K, int s, uint32_t G, uint32_t nt,
-                                                            const uint32_t* __restrict__ mscan, uint2* __restrict__ pairs) {
+                                                            int nlo, int W, uint32_t Kw, uint32_t sw, int s, uint32_t G,
+                                                            uint32_t nt, const uint32_t* __restrict__ mscan,
+                                                            uint2* __restrict__ pairs) {
     extern __shared__ uint32_t lds[];
     uint32_t* base = lds;       // this tile's run start in each group
     uint32_t* rank = lds + G;   // next rank in each group
@@ -147,16 +170,19 @@ __global__ void __launch_bounds__(1024) k_sort_tile_scatter(const uint32_t* __re
     }
     __syncthreads();
     const size_t i0 = (size_t)blockIdx.x * tile, i1 = i0 + tile < n ? i0 + tile : n;
+    const uint32_t nh = (uint32_t)(n >> 1);
     for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         DigitIter it;
         it.init(sc, i, c, nlo);
+        const bool phi = (uint32_t)i >= nh;
+        const uint32_t ref0 = ((uint32_t)i - (phi ? nh : 0u)) | (phi ? 0x40000000u : 0u);
         for (int w = 0; w < W; w++) {
             bool ng;
             uint32_t mag = it.next(ng);
             if (mag) {
-                const uint32_t b = (uint32_t)w * K + mag - 1, g = b >> s;
+                const uint32_t b = (uint32_t)w * Kw + mag - 1, g = b >> s;
                 const uint32_t pos = base[g] + atomicAdd(&rank[g], 1u);
-                pairs[pos] = make_uint2((uint32_t)i | (ng ? 0x80000000u : 0u), b);
+                pairs[pos] = make_uint2((ref0 + (uint32_t)w * sw) | (ng ? 0x80000000u : 0u), b);
             }
         }
     }
@@ -279,17 +305,20 @@ __global__ void __launch_bounds__(256) k_chunk_list(const uint32_t* __restrict__
 struct SortShape {
     MsmGeom g;
     size_t n2, nb, tile, mlen, ne, max_chunks, max_large;
+    size_t group_max;   // expected entries of the heaviest group (uniform scalars)
     uint32_t G, nt, lmax, nkey;
     int sh, qs;
 };
 
-static SortShape sort_shape(size_t n, int c) {
+static SortShape sort_shape(size_t n, int c, bool merged) {
     ZK_REQUIRE(n < 0x40000000ull, ZK_E_ARG, "MSM size must be < 2^30");
     SortShape z;
-    z.g = msm_geom(n, c, false);
+    z.g = msm_geom(n, c, false, merged);
+    // entries name the slot w*n + i of window w's table in 30 bits
+    ZK_REQUIRE(!merged || (size_t)z.g.W * n < 0x40000000ull, ZK_E_ARG, "MSM over window tables needs W*n < 2^30");
     // the sort runs over the 2n half scalars of the GLV split
     z.n2 = 2 * n;
-    z.nb = (size_t)z.g.W * z.g.K;
+    z.nb = (size_t)z.g.Wr * z.g.K;
     // group size 2^s: at most 4096 groups (LDS histograms of pass A)
     z.sh = 7;
     while (((z.nb + (1ull << z.sh) - 1) >> z.sh) > 4096) z.sh++;
@@ -301,13 +330,23 @@ static SortShape sort_shape(size_t n, int c) {
     z.mlen = (size_t)z.G * z.nt + 1;
     z.ne = std::max<size_t>(1, (size_t)z.g.W * z.n2);
     // accumulation order. Oversized threshold: 8x the mean bucket load
-    // (uniform scalars put about 2n/K entries in a bucket).
-    z.lmax = std::max<uint32_t>(L_MIN_LARGE, 8u * (uint32_t)((z.n2 + z.g.K - 1) / z.g.K));
+    // (uniform scalars put about ne/nb entries in a bucket).
+    z.lmax = std::max<uint32_t>(L_MIN_LARGE, 8u * (uint32_t)((z.ne + z.nb - 1) / z.nb));
     z.qs = 0;
     while ((z.lmax >> z.qs) + 2 > 4096) z.qs++;
     z.nkey = (z.lmax >> z.qs) + 2;
     z.max_chunks = z.ne / CHUNK + z.ne / L_MIN_LARGE + 2;
     z.max_large = z.ne / L_MIN_LARGE + 2;
+    // a window of width wd spreads the 2n digits of its halves over 2^(wd-1)
+    // magnitudes; the lowest buckets carry the most: those of the narrower
+    // windows, or of every window when they share one bucket set
+    double per_b = 0;
+    for (int w = 0; w < z.g.W; w++) {
+        const int wd = w < z.g.nlo ? z.g.c - 1 : z.g.c;
+        const double load = (double)z.n2 / (double)(1ull << (wd - 1));
+        per_b = merged ? per_b + load : std::max(per_b, load);
+    }
+    z.group_max = (size_t)(per_b * (double)(1ull << z.sh));
     return z;
 }
 
@@ -331,20 +370,20 @@ static void sort_reserve(MsmSort& s, const SortShape& z) {
     sort_buffers(s, z, [](DevBuf& b, size_t n) { b.ensure(n); });
 }
 
-size_t MsmSort::reserve(size_t n_, int c) {
-    const SortShape z = sort_shape(n_, c);
+size_t MsmSort::reserve(size_t n_, int c, bool merged) {
+    const SortShape z = sort_shape(n_, c, merged);
     sort_reserve(*this, z);
     return z.max_chunks;
 }
 
-size_t MsmSort::count_reserve(size_t n_, int c, GrowCount& gc) const {
-    const SortShape z = sort_shape(n_, c);
+size_t MsmSort::count_reserve(size_t n_, int c, GrowCount& gc, bool merged) const {
+    const SortShape z = sort_shape(n_, c, merged);
     sort_buffers(*this, z, [&](const DevBuf& b, size_t n) { gc.need(b, n); });
     return z.max_chunks;
 }
 
-void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
-    const SortShape z = sort_shape(n_, c);
+void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st, bool merged) {
+    const SortShape z = sort_shape(n_, c, merged);
     sort_reserve(*this, z);
     n = n_;
     g = z.g;
@@ -363,7 +402,7 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
     uint32_t* mat = matrix.as<uint32_t>();
     uint32_t* msc = mscan.as<uint32_t>();
     if (n) {
-        k_sort_tile_hist<<<nt, 1024, G * 4, st>>>(hs, n2, tile, g.c, g.nlo, g.W, g.K, sh, G, nt, mat);
+        k_sort_tile_hist<<<nt, 1024, G * 4, st>>>(hs, n2, tile, g.c, g.nlo, g.W, merged ? 0u : g.K, sh, G, nt, mat);
         ZK_HIP(hipGetLastError());
         fill_u32(mat + mlen - 1, 0u, 1, st);
     } else {
@@ -371,16 +410,16 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
     }
     exclusive_scan(mat, msc, msc + mlen, mlen, st);
     if (n) {
-        k_sort_tile_scatter<<<nt, 1024, 2 * G * 4, st>>>(hs, n2, tile, g.c, g.nlo, g.W, g.K, sh, G, nt, msc,
-                                                          pairs.as<uint2>());
+        k_sort_tile_scatter<<<nt, 1024, 2 * G * 4, st>>>(hs, n2, tile, g.c, g.nlo, g.W, merged ? 0u : g.K,
+                                                          merged ? (uint32_t)n : 0u, sh, G, nt, msc, pairs.as<uint2>());
         ZK_HIP(hipGetLastError());
     }
-    // stage for about 1.25x the mean group (group sizes are sums of many
-    // digits, so far larger ones are rare and take the global path); above
-    // 64 KiB of LDS (groups of 2^9 buckets at 2^22 points) the kernel is
-    // allowed up to 160 KiB, or keeps 64 KiB if the runtime refuses
+    // stage for the heaviest group uniform scalars give, plus a margin
+    // (group sizes are sums of many digits, so larger ones are rare and take
+    // the global path); above 64 KiB of LDS the kernel is allowed up to
+    // 160 KiB, or keeps 64 KiB if the runtime refuses
     ZK_REQUIRE((4u << sh) < SORT_GROUP_LDS / 2, ZK_E_INTERNAL, "sort group counters exceed the LDS budget");
-    const size_t want = z.ne / G + z.ne / G / 4 + 256;
+    const size_t want = z.group_max + z.group_max / 16 + 256;
     uint32_t lds_budget = SORT_GROUP_LDS;
     if (4 * ((1u << sh) + want) > SORT_GROUP_LDS) {
         if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sort_group), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -411,7 +450,7 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st) {
 }
 
 void msm_reserve_partials(MsmPartials& out, const MsmGeom& g, bool g2) {
-    out.host.ensure(2 * (size_t)g.W * g.nblk * (g2 ? 256 : 128));
+    out.host.ensure(2 * (size_t)g.Wr * g.nblk * (g2 ? 256 : 128));
 }
 
 void msm_points_to29(const void* d_points, size_t n, void* d_out, bool g2, hipStream_t st) {
@@ -419,6 +458,15 @@ void msm_points_to29(const void* d_points, size_t n, void* d_out, bool g2, hipSt
     ZK_REQUIRE(n < 0x40000000ull, ZK_E_ARG, "MSM size must be < 2^30");
     if (g2) msm_points_to29_g2(d_points, n, d_out, st);
     else msm_points_to29_g1(d_points, n, d_out, st);
+}
+
+void msm_points_table(const void* d_points, size_t n, int c, void* d_out, bool g2, hipStream_t st) {
+    if (!n) return;
+    const MsmGeom g = msm_geom(n, c, g2, true);
+    ZK_REQUIRE((size_t)g.W * n < 0x40000000ull, ZK_E_ARG, "window tables need W*n < 2^30");
+    msm_points_to29(d_points, n, d_out, g2, st);
+    if (g2) msm_points_table_g2(d_points, n, g, d_out, st);
+    else msm_points_table_g1(d_points, n, g, d_out, st);
 }
 
 void MsmScratch::enable_timing() {
@@ -437,12 +485,12 @@ MsmScratch::~MsmScratch() {
 template <class S, class F>
 static void scratch_buffers(S& q, const MsmGeom& g, size_t max_chunks, bool g2, int split, F&& f) {
     const size_t xb = g2 ? 256 : 128;
-    const size_t nb = (size_t)g.W * g.K;
+    const size_t nb = (size_t)g.Wr * g.K;
     f(q.buckets, nb * split * xb);
-    f(q.r1, (size_t)g.W * g.nseg * xb);
-    f(q.t1, (size_t)g.W * g.nseg * xb);
-    f(q.r2, (size_t)g.W * g.nblk * xb);
-    f(q.t2, (size_t)g.W * g.nblk * xb);
+    f(q.r1, (size_t)g.Wr * g.nseg * xb);
+    f(q.t1, (size_t)g.Wr * g.nseg * xb);
+    f(q.r2, (size_t)g.Wr * g.nblk * xb);
+    f(q.t2, (size_t)g.Wr * g.nblk * xb);
     f(q.partials, max_chunks * xb);
 }
 
@@ -454,10 +502,11 @@ void MsmScratch::count_ensure(const MsmGeom& g, size_t max_chunks, bool g2, int 
     scratch_buffers(*this, g, max_chunks, g2, split, [&](const DevBuf& b, size_t n) { gc.need(b, n); });
 }
 
-void msm_launch_pre(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, hipStream_t st, int split) {
+void msm_launch_pre(const MsmSort& sort, const void* d_points29, bool g2, MsmScratch& scr, hipStream_t st, int split) {
     ZK_REQUIRE(split >= 1 && split <= MSM_SPLIT_MAX, ZK_E_ARG, "bucket split must be in [1, 4]");
-    if (g2) msm_launch_pre_g2(sort, d_points, scr, st, split);
-    else msm_launch_pre_g1(sort, d_points, scr, st, split);
+    ZK_REQUIRE(sort.n == 0 || d_points29, ZK_E_INTERNAL, "MSM launch without the 29-bit copy of its points");
+    if (g2) msm_launch_pre_g2(sort, d_points29, scr, st, split);
+    else msm_launch_pre_g1(sort, d_points29, scr, st, split);
 }
 
 void msm_launch_main(const MsmSort& sort, bool g2, MsmScratch& scr, MsmPartials& out, hipStream_t st,
@@ -468,9 +517,9 @@ void msm_launch_main(const MsmSort& sort, bool g2, MsmScratch& scr, MsmPartials&
     else msm_launch_main_g1(sort, scr, out, st, d_points29);
 }
 
-void msm_launch(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, MsmPartials& out, hipStream_t st,
+void msm_launch(const MsmSort& sort, bool g2, MsmScratch& scr, MsmPartials& out, hipStream_t st,
                 const void* d_points29, int split) {
-    msm_launch_pre(sort, d_points, g2, scr, st, split);
+    msm_launch_pre(sort, d_points29, g2, scr, st, split);
     msm_launch_main(sort, g2, scr, out, st, d_points29);
 }
 
@@ -503,7 +552,7 @@ zkh::Xyzz<zkh::Fp> msm_g1_noalloc(int device, const void* d_points, const void* 
         if (device >= 0 && device < NOALLOC_MAX_DEVICES) base = g_noalloc_ptr[device];
     }
     ZK_REQUIRE(base != nullptr, ZK_E_STATE, "fixed MSM scratch not set up on this device");
-    const SortShape z = sort_shape(n, window);
+    const SortShape z = sort_shape(n, window, false);
     const MsmGeom& g = z.g;
     const int split = msm_default_split(n, g.c, false);
     // the same buffers the allocating path sizes, laid out in the array
@@ -519,7 +568,7 @@ zkh::Xyzz<zkh::Fp> msm_g1_noalloc(int device, const void* d_points, const void* 
                "MSM of " + std::to_string(n) + " points at window " + std::to_string(g.c) + " needs " +
                    std::to_string(need) + " bytes of workspace; the fixed scratch holds " +
                    std::to_string(MSM_NOALLOC_SCRATCH_BYTES) + " (65536 points at the default window)");
-    const size_t host_need = 2 * (size_t)g.W * g.nblk * 128;
+    const size_t host_need = 2 * (size_t)g.Wr * g.nblk * 128;
     ZK_REQUIRE(host_need <= sizeof g_noalloc_host, ZK_E_ARG, "MSM window too large for the fixed host partials");
     size_t used = 0;
     auto take = [&](DevBuf& b, size_t bytes) {
@@ -535,7 +584,7 @@ zkh::Xyzz<zkh::Fp> msm_g1_noalloc(int device, const void* d_points, const void* 
     part.ext_bytes = sizeof g_noalloc_host;
     if (n) msm_points_to29(d_points, n, p29.p, false, st);
     sort.run(d_scalars, n, window, st);
-    msm_launch(sort, d_points, false, scr, part, st, p29.p, split);
+    msm_launch(sort, false, scr, part, st, p29.p, split);
     ZK_HIP(hipStreamSynchronize(st));
     return msm_finish_g1(part);
 }
@@ -548,14 +597,14 @@ zkh::Fp load_h<zkh::Fp>(const uint64_t* p) { return zkh::Fp::from_raw(p); }
 template <>
 zkh::Fp2 load_h<zkh::Fp2>(const uint64_t* p) { return {zkh::Fp::from_raw(p), zkh::Fp::from_raw(p + 4)}; }
 
-// value of window w from its level-2 partials: sum_blk (T_blk + blk*U*R_blk)
-// with U = 2^logU, by running sums over the blocks
+// value of bucket set w from its level-2 partials: sum_blk (T_blk +
+// blk*U*R_blk) with U = 2^logU, by running sums over the blocks
 template <class HT>
 static zkh::Xyzz<HT> window_value(const MsmPartials& p, int w) {
     const MsmGeom& g = p.g;
     const int lw = sizeof(HT) / 8;  // u64 words per coordinate
     const uint64_t* R = p.data();
-    const uint64_t* Tt = R + (size_t)g.W * g.nblk * 4 * lw;
+    const uint64_t* Tt = R + (size_t)g.Wr * g.nblk * 4 * lw;
     const int logU = ilog2(g.l1) + ilog2(L2) + ilog2(p.g2 ? RedBlk<Fq2>::V : RedBlk<Fp>::V);
     auto ld = [&](const uint64_t* base, size_t i) {
         const uint64_t* q = base + i * 4 * lw;
@@ -571,10 +620,12 @@ static zkh::Xyzz<HT> window_value(const MsmPartials& p, int w) {
     return zkh::xyzz_add(tsum, acc);
 }
 
-// window values (on `tasks` threads when > 1), then Horner across windows
+// window values (on `tasks` threads when > 1), then Horner across windows;
+// over window tables the one bucket set is the whole MSM
 template <class HT>
 static zkh::Xyzz<HT> finish_t(const MsmPartials& p, int tasks) {
     const MsmGeom& g = p.g;
+    if (g.merged) return window_value<HT>(p, 0);
     std::vector<zkh::Xyzz<HT>> S(g.W);
     if (tasks <= 1 || g.W < 2) {
         for (int w = 0; w < g.W; w++) S[w] = window_value<HT>(p, w);

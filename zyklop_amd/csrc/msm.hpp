@@ -18,7 +18,13 @@
 //     whole workgroups (device-side work lists, no host round trip);
 //   * bucket reduction: thread-level running sums over segments, then an LDS
 //     suffix-scan combine per workgroup; the host finishes a few points per
-//     window and runs Horner across windows.
+//     window and runs Horner across windows;
+//   * window tables for bases that stay resident (a circuit's CRS): table w
+//     holds 2^(offset of window w) * P_i for every base, so the digits of all
+//     windows go to ONE set of buckets (sum_w d_w 2^off_w P = sum_w d_w T_w).
+//     One set of 2^(c-1) buckets for W windows makes wider windows cheap:
+//     fewer windows, hence fewer entries, for the same buckets to reduce,
+//     and no Horner.
 #pragma once
 #include <stdint.h>
 
@@ -35,20 +41,31 @@ struct MsmGeom {
     int c;          // window bits: windows w < nlo have c - 1 bits, the others c
     int W;          // windows = ceil(GLV_BITS / requested c); then c = ceil(GLV_BITS / W)
     int nlo;        // W*c - GLV_BITS
-    uint32_t K;     // buckets per window = 2^(c-1)
+    bool merged;    // all windows share one set of buckets (bases from window tables)
+    int Wr;         // bucket sets: W, or 1 when merged
+    uint32_t K;     // buckets per set = 2^(c-1)
     uint32_t l1;    // buckets per level-1 reduction thread
-    uint32_t nseg;  // level-1 segments per window = K / l1
-    uint32_t nblk;  // level-2 blocks per window
+    uint32_t nseg;  // level-1 segments per bucket set = K / l1
+    uint32_t nblk;  // level-2 blocks per bucket set
 };
 
 int msm_default_c(size_t n);
-MsmGeom msm_geom(size_t n, int c, bool g2);
+// The geometry depends on the window count only: every c with the same
+// W = ceil(GLV_BITS / c) gives the same widths.
+MsmGeom msm_geom(size_t n, int c, bool g2, bool merged = false);
+// window of the tables of n resident bases: the c that minimises the
+// accumulation (2n entries per window, one mixed addition each) plus the
+// reduction of one set of 2^(c-1) buckets (two general additions each)
+int msm_table_c(size_t n);
 
 // Per-scalar-vector digit sort (shared by the MSMs over the same scalars).
 // Besides the entries in bucket order it produces the accumulation order: all
-// W*K bucket ids sorted by descending load (quantised by 2^qs), so that the
+// Wr*K bucket ids sorted by descending load (quantised by 2^qs), so that the
 // lanes of one wave walk buckets of nearly equal length. Buckets above lmax
 // entries take the chunked path and come first.
+// An entry is slot | half << 30 | sign << 31: the 29-bit base slot (the
+// scalar's index mod n, plus w*n in window w of a merged sort, whose bases
+// come from window tables), which of its two GLV halves, and the digit's sign.
 struct MsmSort {
     MsmGeom g{};
     size_t n = 0;                 // scalars; the sort itself runs over 2n half scalars
@@ -60,20 +77,19 @@ struct MsmSort {
     size_t n_chunk_max = 0;
     // sizes every buffer a sort of n scalars at window c needs (no launch);
     // returns the chunk-list capacity for MsmScratch::ensure
-    size_t reserve(size_t n, int c);
-    // the device bytes reserve(n, c) would add now (into gc); same return value
-    size_t count_reserve(size_t n, int c, GrowCount& gc) const;
-    // sorts canonical scalars (n x 32 B, device, n < 2^30) into bucket order;
-    // an entry is the index i < 2n of a base (msm_points_to29 order) with the
-    // digit's sign in bit 31
-    void run(const void* d_scalars, size_t n, int c, hipStream_t st);
+    size_t reserve(size_t n, int c, bool merged = false);
+    // the device bytes reserve(n, c, merged) would add now (into gc); same return value
+    size_t count_reserve(size_t n, int c, GrowCount& gc, bool merged = false) const;
+    // sorts canonical scalars (n x 32 B, device, n < 2^30; W*n < 2^30 merged)
+    // into bucket order, entries as above
+    void run(const void* d_scalars, size_t n, int c, hipStream_t st, bool merged = false);
 };
 
 // Result of the device part of one MSM: level-2 partials per window, on the host.
 struct MsmPartials {
     MsmGeom g{};
     bool g2 = false;
-    HostBuf host;       // [W][nblk] R then [W][nblk] T, XYZZ each
+    HostBuf host;       // [Wr][nblk] R then [Wr][nblk] T, XYZZ each
     // caller-owned host memory used instead of `host` when set (ext_bytes long)
     void* ext = nullptr;
     size_t ext_bytes = 0;
@@ -94,25 +110,24 @@ struct MsmScratch {
     void count_ensure(const MsmGeom& g, size_t max_chunks, bool g2, int split, GrowCount& gc) const;
 };
 
-// Launch bucket accumulation + GPU reduction for points (affine, Montgomery,
-// AoS: G1 64 B / G2 128 B per point) against a finished sort; queue the D2H of
-// the partials into out.host on st. The caller synchronises st before finish.
-// d_points29: the base slots of msm_points_to29, in which the buckets are
-// accumulated (d_points serves the oversized-bucket path, which applies the
-// endomorphism itself); may be null only for n = 0.
+// Launch bucket accumulation + GPU reduction against a finished sort; queue
+// the D2H of the partials into out.host on st. The caller synchronises st
+// before finish. d_points29: the base slots of msm_points_to29 (for a merged
+// sort, the W window tables of msm_points_table, table w at slot w*n); may be
+// null only for n = 0.
 // split: threads per bucket in the accumulation (1..MSM_SPLIT_MAX), each over
 // a consecutive part of the bucket's entries; the level-1 reduction adds the
 // parts. More threads than buckets fill the GPU when the buckets are few.
 constexpr int MSM_SPLIT_MAX = 4;
 // split that gives about two rounds of resident waves on a 256-CU GPU
-int msm_default_split(size_t n, int c, bool g2);
-void msm_launch(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, MsmPartials& out,
-                hipStream_t st, const void* d_points29, int split = 1);
+int msm_default_split(size_t n, int c, bool g2, bool merged = false);
+void msm_launch(const MsmSort& sort, bool g2, MsmScratch& scr, MsmPartials& out, hipStream_t st,
+                const void* d_points29, int split = 1);
 // msm_launch in two parts: the chunked path of the oversized buckets, then
 // the accumulation, reductions and the copy of the partials. Queueing every
 // MSM's first part before any accumulation keeps those short launches from
 // waiting behind the accumulations of other streams.
-void msm_launch_pre(const MsmSort& sort, const void* d_points, bool g2, MsmScratch& scr, hipStream_t st, int split);
+void msm_launch_pre(const MsmSort& sort, const void* d_points29, bool g2, MsmScratch& scr, hipStream_t st, int split);
 void msm_launch_main(const MsmSort& sort, bool g2, MsmScratch& scr, MsmPartials& out, hipStream_t st,
                      const void* d_points29);
 
@@ -121,6 +136,10 @@ void msm_launch_main(const MsmSort& sort, bool g2, MsmScratch& scr, MsmPartials&
 // P_i and the x of lambda*P_i: the 2n bases of the GLV split.
 constexpr size_t PT29_BYTES = 128, PT29Q_BYTES = 256;
 void msm_points_to29(const void* d_points, size_t n, void* d_out, bool g2, hipStream_t st);
+// window tables of n affine points for the geometry of window c: W tables of
+// n slots (the form of msm_points_to29), table w holding 2^(offset of window
+// w) * P_i. d_out holds W*n slots; table 0 is msm_points_to29's output.
+void msm_points_table(const void* d_points, size_t n, int c, void* d_out, bool g2, hipStream_t st);
 
 // size out's host buffer for an MSM of geometry g (no launch)
 void msm_reserve_partials(MsmPartials& out, const MsmGeom& g, bool g2);

@@ -154,27 +154,22 @@ ZK_DEV Fp beta_mont(const uint32_t* c) {
     return b;
 }
 
-// base i of an MSM over the 2n points of the GLV split: P_i for i < n, else
-// lambda*P_(i-n) = (beta*x, y) (glv.hpp)
+// the base an entry names (slot | half << 30, field29.hpp), read from the
+// 29-bit slots and returned in the standard form; an all-zero slot is the
+// point at infinity (0, 0)
 template <class T>
-ZK_DEV Aff<T> load_base(const void* pts, uint32_t i, uint32_t n);
+ZK_DEV Aff<T> load_base(const void* pts29, uint32_t ref);
 template <>
-ZK_DEV Aff<Fp> load_base<Fp>(const void* pts, uint32_t i, uint32_t n) {
-    if (i < n) return PointIO<Fp>::load_aff(pts, i);
-    Aff<Fp> a = PointIO<Fp>::load_aff(pts, i - n);
-    if (aff_is_inf(a)) return a;
-    a.x = mul<FpCfg>(a.x, beta_mont(glv::BETA_G1_MONT));
-    return a;
+ZK_DEV Aff<Fp> load_base<Fp>(const void* pts29, uint32_t ref) {
+    F29 x, y;
+    f29_load_point(pts29, ref, x, y);
+    return {f29_to_std(x), f29_to_std(y)};
 }
 template <>
-ZK_DEV Aff<Fq2> load_base<Fq2>(const void* pts, uint32_t i, uint32_t n) {
-    if (i < n) return PointIO<Fq2>::load_aff(pts, i);
-    Aff<Fq2> a = PointIO<Fq2>::load_aff(pts, i - n);
-    if (aff_is_inf(a)) return a;
-    const Fp b = beta_mont(glv::BETA_G2_MONT);
-    a.x.c0 = mul<FpCfg>(a.x.c0, b);
-    a.x.c1 = mul<FpCfg>(a.x.c1, b);
-    return a;
+ZK_DEV Aff<Fq2> load_base<Fq2>(const void* pts29, uint32_t ref) {
+    F29q x, y;
+    f29q_load_point(pts29, ref, x, y);
+    return {f29q_to_std(x), f29q_to_std(y)};
 }
 
 // tree-sum of one XYZZ per thread across the block (blockDim.x must be a power of 2)
@@ -199,9 +194,11 @@ constexpr int CHUNK_THREADS = 64;
 
 // one workgroup per chunk of an oversized bucket (grid-stride over the chunk list)
 template <class T>
-__global__ void __launch_bounds__(CHUNK_THREADS) k_msm_chunks(const uint32_t* __restrict__ entries, const void* __restrict__ points,
-                                                    uint32_t nbase, const uint32_t* __restrict__ counters,
-                                                    const ChunkRec* __restrict__ chunks, void* __restrict__ partials) {
+__global__ void __launch_bounds__(CHUNK_THREADS) k_msm_chunks(const uint32_t* __restrict__ entries,
+                                                             const void* __restrict__ points29,
+                                                             const uint32_t* __restrict__ counters,
+                                                             const ChunkRec* __restrict__ chunks,
+                                                             void* __restrict__ partials) {
     __shared__ Xyzz<T> sh[CHUNK_THREADS];
     const uint32_t nch = counters[0];
     for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
@@ -209,7 +206,7 @@ __global__ void __launch_bounds__(CHUNK_THREADS) k_msm_chunks(const uint32_t* __
         Xyzz<T> acc = xyzz_inf<T>();
         for (uint32_t e = threadIdx.x; e < cr.len; e += blockDim.x) {
             uint32_t ref = entries[cr.start + e];
-            Aff<T> P = load_base<T>(points, ref & 0x7fffffffu, nbase);
+            Aff<T> P = load_base<T>(points29, ref);
             if (aff_is_inf(P)) continue;
             if (ref >> 31) P.y = PointIO<T>::neg_y(P.y);
             xyzz_madd(acc, P);
@@ -345,13 +342,14 @@ __global__ void __launch_bounds__(BLK) k_msm_reduce2(const void* __restrict__ Ri
 // wait behind other streams' accumulations while its own stream's
 // accumulation waits behind it.
 template <class T>
-static void launch_pre_t(const MsmSort& sort, const void* d_points, MsmScratch& scr, hipStream_t st, int split) {
+static void launch_pre_t(const MsmSort& sort, const void* d_points29, MsmScratch& scr, hipStream_t st, int split) {
     const bool g2 = std::is_same<T, Fq2>::value;
-    const MsmGeom g = msm_geom(sort.n, sort.g.c, g2);
+    const MsmGeom g = msm_geom(sort.n, sort.g.c, g2, sort.g.merged);
     scr.ensure(g, sort.n_chunk_max, g2, split);
     scr.split = split;
-    k_msm_chunks<T><<<256, CHUNK_THREADS, 0, st>>>(sort.entries.as<uint32_t>(), d_points, (uint32_t)sort.n,
-                                          sort.counters.as<uint32_t>(), sort.chunks.as<ChunkRec>(), scr.partials.p);
+    k_msm_chunks<T><<<256, CHUNK_THREADS, 0, st>>>(sort.entries.as<uint32_t>(), d_points29,
+                                                   sort.counters.as<uint32_t>(), sort.chunks.as<ChunkRec>(),
+                                                   scr.partials.p);
     ZK_HIP(hipGetLastError());
     k_msm_merge<T><<<64, CHUNK_THREADS, 0, st>>>(sort.counters.as<uint32_t>(), sort.large.as<LargeRec>(), scr.partials.p,
                                          (uint32_t)split, scr.buckets.p);
@@ -366,23 +364,23 @@ template <class T>
 static void launch_main_t(const MsmSort& sort, const void* d_points29, MsmScratch& scr, MsmPartials& out,
                           hipStream_t st) {
     const bool g2 = std::is_same<T, Fq2>::value;
-    const MsmGeom g = msm_geom(sort.n, sort.g.c, g2);
+    const MsmGeom g = msm_geom(sort.n, sort.g.c, g2, sort.g.merged);
     const int split = scr.split;
-    const uint32_t nb = (uint32_t)((size_t)g.W * g.K), nt = nb * (uint32_t)split;
+    const uint32_t nb = (uint32_t)((size_t)g.Wr * g.K);
     if (scr.ev_acc0) ZK_HIP(hipEventRecord(scr.ev_acc0, st));
     launch_accum<T>(sort, nb, split, d_points29, scr.buckets.p, st);
     ZK_HIP(hipGetLastError());
     if (scr.ev_acc1) ZK_HIP(hipEventRecord(scr.ev_acc1, st));
-    uint32_t nseg_all = (uint32_t)g.W * g.nseg;
-    k_msm_reduce1<T><<<cdiv(nseg_all, 128), 128, 0, st>>>(scr.buckets.p, g.K, g.W, g.l1, (uint32_t)split, scr.r1.p,
+    uint32_t nseg_all = (uint32_t)g.Wr * g.nseg;
+    k_msm_reduce1<T><<<cdiv(nseg_all, 128), 128, 0, st>>>(scr.buckets.p, g.K, g.Wr, g.l1, (uint32_t)split, scr.r1.p,
                                                            scr.t1.p);
     ZK_HIP(hipGetLastError());
     constexpr int BLK = RedBlk<T>::V;
-    k_msm_reduce2<T, BLK><<<(unsigned)(g.W * g.nblk), BLK, 0, st>>>(scr.r1.p, scr.t1.p, g.nseg, ilog2(g.l1), g.nblk,
-                                                                     scr.r2.p, scr.t2.p);
+    k_msm_reduce2<T, BLK><<<(unsigned)(g.Wr * g.nblk), BLK, 0, st>>>(scr.r1.p, scr.t1.p, g.nseg, ilog2(g.l1), g.nblk,
+                                                                      scr.r2.p, scr.t2.p);
     ZK_HIP(hipGetLastError());
     size_t xb = g2 ? 256 : 128;
-    size_t half = (size_t)g.W * g.nblk * xb;
+    size_t half = (size_t)g.Wr * g.nblk * xb;
     out.g = g;
     out.g2 = g2;
     void* dst = out.ext;
@@ -399,11 +397,13 @@ static void launch_main_t(const MsmSort& sort, const void* d_points29, MsmScratc
 
 
 // per-curve entry points, defined in msm_g1.hip and msm_g2.hip
-void msm_launch_pre_g1(const MsmSort& sort, const void* d_points, MsmScratch& scr, hipStream_t st, int split);
+void msm_launch_pre_g1(const MsmSort& sort, const void* d_points29, MsmScratch& scr, hipStream_t st, int split);
 void msm_launch_main_g1(const MsmSort& sort, MsmScratch& scr, MsmPartials& out, hipStream_t st, const void* d_points29);
 void msm_points_to29_g1(const void* d_points, size_t n, void* d_out, hipStream_t st);
-void msm_launch_pre_g2(const MsmSort& sort, const void* d_points, MsmScratch& scr, hipStream_t st, int split);
+void msm_points_table_g1(const void* d_points, size_t n, const MsmGeom& g, void* d_out, hipStream_t st);
+void msm_launch_pre_g2(const MsmSort& sort, const void* d_points29, MsmScratch& scr, hipStream_t st, int split);
 void msm_launch_main_g2(const MsmSort& sort, MsmScratch& scr, MsmPartials& out, hipStream_t st, const void* d_points29);
 void msm_points_to29_g2(const void* d_points, size_t n, void* d_out, hipStream_t st);
+void msm_points_table_g2(const void* d_points, size_t n, const MsmGeom& g, void* d_out, hipStream_t st);
 
 }  // namespace zk

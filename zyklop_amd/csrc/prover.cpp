@@ -141,17 +141,42 @@ static void finish_circuit_bases(Circuit& c, const uint8_t* A, const uint8_t* B1
     build_ext(c.B2, nullptr, 0, B2, nv, exB2, 128);
     build_ext(c.C, nullptr, l + 1, C, nv - l - 1, exC, 64);
     upload(c.H, H, (size_t)c.n * 64);
-    const size_t ne = nv + 4;
-    hipStream_t st = c.ctx->stream;
-    for (auto pr : {std::make_pair(&c.A, &c.A29), std::make_pair(&c.B1, &c.B1_29), std::make_pair(&c.C, &c.C29)}) {
-        pr.second->ensure(ne * PT29_BYTES);
-        msm_points_to29(pr.first->p, ne, pr.second->p, false, st);
+    circuit_set_tables(c, c.ctx->tables_mode);
+}
+
+void circuit_set_tables(Circuit& c, int mode) {
+    const size_t ne = (size_t)c.n_vars + 4, n = c.n;
+    const int cw = msm_table_c(ne), ch = msm_table_c(n);
+    const int Ww = msm_geom(ne, cw, false, true).W, Wh = msm_geom(n, ch, false, true).W;
+    bool on = mode > 0;
+    if (mode < 0) {
+        const size_t extra = (size_t)(Ww - 1) * ne * (3 * PT29_BYTES + PT29Q_BYTES) + (size_t)(Wh - 1) * n * PT29_BYTES;
+        size_t free_b = 0, total_b = 0;
+        ZK_HIP(hipMemGetInfo(&free_b, &total_b));
+        on = extra <= free_b / 3;
     }
-    c.B2_29.ensure(ne * PT29Q_BYTES);
-    msm_points_to29(c.B2.p, ne, c.B2_29.p, true, st);
-    c.H29.ensure((size_t)c.n * PT29_BYTES);
-    msm_points_to29(c.H.p, c.n, c.H29.p, false, st);
+    // entries name a table slot in 30 bits
+    const bool fits = (size_t)Ww * ne < 0x40000000ull && (size_t)Wh * n < 0x40000000ull;
+    ZK_REQUIRE(!(mode > 0 && !fits), ZK_E_ARG, "window tables need W*n < 2^30 slots");
+    on = on && fits;
+    // the arrays are reallocated: nothing may still read them
+    ZK_HIP(hipDeviceSynchronize());
+    hipStream_t st = c.ctx->stream;
+    const size_t kw = on ? (size_t)Ww : 1, kh = on ? (size_t)Wh : 1;
+    for (auto pr : {std::make_pair(&c.A, &c.A29), std::make_pair(&c.B1, &c.B1_29), std::make_pair(&c.C, &c.C29)}) {
+        pr.second->alloc(kw * ne * PT29_BYTES);
+        if (on) msm_points_table(pr.first->p, ne, cw, pr.second->p, false, st);
+        else msm_points_to29(pr.first->p, ne, pr.second->p, false, st);
+    }
+    c.B2_29.alloc(kw * ne * PT29Q_BYTES);
+    if (on) msm_points_table(c.B2.p, ne, cw, c.B2_29.p, true, st);
+    else msm_points_to29(c.B2.p, ne, c.B2_29.p, true, st);
+    c.H29.alloc(kh * n * PT29_BYTES);
+    if (on) msm_points_table(c.H.p, n, ch, c.H29.p, false, st);
+    else msm_points_to29(c.H.p, n, c.H29.p, false, st);
     ZK_HIP(hipStreamSynchronize(st));
+    c.tab_c[0] = on ? cw : 0;
+    c.tab_c[1] = on ? ch : 0;
 }
 
 std::unique_ptr<Circuit> circuit_from_zkey(Context& ctx, const std::string& path) {
@@ -341,13 +366,13 @@ void ProveSlot::reserve(Context& ctx, const Circuit& c, bool host_witness) {
     const size_t nS = (size_t)c.n_vars + 4, n = c.n;
     prepare(ctx, c);
     if (host_witness) staging.ensure((size_t)c.n_vars * 32);
-    const int cw = ctx.window_for(0, nS), ch = ctx.window_for(1, n);
-    const size_t kw = sortW.reserve(nS, cw), kh = sortH.reserve(n, ch);
-    const int s1 = ctx.split_for(0, nS, false), s2 = ctx.split_for(0, nS, true), sh = ctx.split_for(1, n, false);
-    const MsmGeom gw1 = msm_geom(nS, cw, false), gw2 = msm_geom(nS, cw, true), gh = msm_geom(n, ch, false);
-    for (MsmScratch* q : {&scrA, &scrB1, &scrC}) q->ensure(gw1, kw, false, s1);
-    scrB2.ensure(gw2, kw, true, s2);
-    scrH.ensure(gh, kh, false, sh);
+    const MsmPlan pl = ctx.msm_plan(c);
+    const size_t kw = sortW.reserve(nS, pl.cw, pl.mw), kh = sortH.reserve(n, pl.ch, pl.mh);
+    const MsmGeom gw1 = msm_geom(nS, pl.cw, false, pl.mw), gw2 = msm_geom(nS, pl.cw, true, pl.mw),
+                  gh = msm_geom(n, pl.ch, false, pl.mh);
+    for (MsmScratch* q : {&scrA, &scrB1, &scrC}) q->ensure(gw1, kw, false, pl.sp_w1);
+    scrB2.ensure(gw2, kw, true, pl.sp_w2);
+    scrH.ensure(gh, kh, false, pl.sp_h);
     for (MsmPartials* q : {&parts.a, &parts.b1, &parts.c}) msm_reserve_partials(*q, gw1, false);
     msm_reserve_partials(parts.b2, gw2, true);
     msm_reserve_partials(parts.h, gh, false);
@@ -380,9 +405,10 @@ void ProveSlot::enqueue(Context& ctx, const Circuit& c, const uint64_t* witness,
 void ProveSlot::body(Context& ctx, const Circuit& c, ProofParts& out, bool alone) {
     const size_t nS = (size_t)c.n_vars + 4, n = c.n;
     NttPlan& plan = ctx.ntt_plan(c.log_n);
-    window_w = ctx.window_for(0, nS);
-    window_h = ctx.window_for(1, n);
-    const int sp_w1 = ctx.split_for(0, nS, false), sp_w2 = ctx.split_for(0, nS, true), sp_h = ctx.split_for(1, n, false);
+    const MsmPlan pl = ctx.msm_plan(c);
+    window_w = pl.cw;
+    window_h = pl.ch;
+    const int sp_w1 = pl.sp_w1, sp_w2 = pl.sp_w2, sp_h = pl.sp_h;
     ZK_HIP(hipEventRecord(ev_scal, s0));
     // witness MSMs first: the G2 one is the longest chain of the proof, so its
     // sort and accumulation are queued before the H pipeline. One shared
@@ -401,15 +427,15 @@ void ProveSlot::body(Context& ctx, const Circuit& c, ProofParts& out, bool alone
     hipStream_t s_sort = alone ? s2 : s1, s_a = alone ? s2 : s1;
     ZK_HIP(hipStreamWaitEvent(s_sort, ev_scal, 0));
     if (timed) ZK_HIP(hipEventRecord(ev[4], s_sort));
-    sortW.run(scal.p, nS, window_w, s_sort);
+    sortW.run(scal.p, nS, window_w, s_sort, pl.mw);
     ZK_HIP(hipEventRecord(ev_done1, s_sort));
     if (timed) ZK_HIP(hipEventRecord(ev[5], s_sort));
     for (hipStream_t p : {s1, s2, s3, s4})
         if (p != s_sort) ZK_HIP(hipStreamWaitEvent(p, ev_done1, 0));
-    msm_launch_pre(sortW, c.B2.p, true, scrB2, s4, sp_w2);
-    msm_launch_pre(sortW, c.B1.p, false, scrB1, s2, sp_w1);
-    msm_launch_pre(sortW, c.C.p, false, scrC, s3, sp_w1);
-    msm_launch_pre(sortW, c.A.p, false, scrA, s_a, sp_w1);
+    msm_launch_pre(sortW, c.B2_29.p, true, scrB2, s4, sp_w2);
+    msm_launch_pre(sortW, c.B1_29.p, false, scrB1, s2, sp_w1);
+    msm_launch_pre(sortW, c.C29.p, false, scrC, s3, sp_w1);
+    msm_launch_pre(sortW, c.A29.p, false, scrA, s_a, sp_w1);
     msm_launch_main(sortW, true, scrB2, out.b2, s4, c.B2_29.p);
     if (!alone) msm_launch_main(sortW, false, scrA, out.a, s_a, c.A29.p);
     msm_launch_main(sortW, false, scrB1, out.b1, s2, c.B1_29.p);
@@ -426,8 +452,8 @@ void ProveSlot::body(Context& ctx, const Circuit& c, ProofParts& out, bool alone
     ntt_run(plan, tmp, abc, 3, false, nullptr, nullptr, s0);          // values at w_2n^(2j+1)
     if (timed) ZK_HIP(hipEventRecord(ev[3], s0));
     h_final(aw.p, bw.p, cw.p, h.p, n, s0);
-    sortH.run(h.p, n, window_h, s0);
-    msm_launch(sortH, c.H.p, false, scrH, out.h, s0, c.H29.p, sp_h);
+    sortH.run(h.p, n, window_h, s0, pl.mh);
+    msm_launch(sortH, false, scrH, out.h, s0, c.H29.p, sp_h);
     ZK_HIP(hipEventRecord(ev_h, s0));
     hipStream_t br[4] = {s1, s2, s3, s4};
     for (int k = 0; k < 4; k++) {
@@ -466,16 +492,16 @@ size_t ProveSlot::count_bytes(const Context& ctx, const Circuit& c, const ProveS
     } else {
         g.bytes += 7 * n * 32;
     }
-    const int cw = ctx.window_for(0, nS), ch = ctx.window_for(1, n);
-    const size_t kw = (sl ? sl->sortW : no_sort).count_reserve(nS, cw, g);
-    const size_t kh = (sl ? sl->sortH : no_sort).count_reserve(n, ch, g);
-    const int s1 = ctx.split_for(0, nS, false), s2 = ctx.split_for(0, nS, true), sh = ctx.split_for(1, n, false);
-    const MsmGeom gw1 = msm_geom(nS, cw, false), gw2 = msm_geom(nS, cw, true), gh = msm_geom(n, ch, false);
-    (sl ? sl->scrA : no_scr).count_ensure(gw1, kw, false, s1, g);
-    (sl ? sl->scrB1 : no_scr).count_ensure(gw1, kw, false, s1, g);
-    (sl ? sl->scrC : no_scr).count_ensure(gw1, kw, false, s1, g);
-    (sl ? sl->scrB2 : no_scr).count_ensure(gw2, kw, true, s2, g);
-    (sl ? sl->scrH : no_scr).count_ensure(gh, kh, false, sh, g);
+    const MsmPlan pl = ctx.msm_plan(c);
+    const size_t kw = (sl ? sl->sortW : no_sort).count_reserve(nS, pl.cw, g, pl.mw);
+    const size_t kh = (sl ? sl->sortH : no_sort).count_reserve(n, pl.ch, g, pl.mh);
+    const MsmGeom gw1 = msm_geom(nS, pl.cw, false, pl.mw), gw2 = msm_geom(nS, pl.cw, true, pl.mw),
+                  gh = msm_geom(n, pl.ch, false, pl.mh);
+    (sl ? sl->scrA : no_scr).count_ensure(gw1, kw, false, pl.sp_w1, g);
+    (sl ? sl->scrB1 : no_scr).count_ensure(gw1, kw, false, pl.sp_w1, g);
+    (sl ? sl->scrC : no_scr).count_ensure(gw1, kw, false, pl.sp_w1, g);
+    (sl ? sl->scrB2 : no_scr).count_ensure(gw2, kw, true, pl.sp_w2, g);
+    (sl ? sl->scrH : no_scr).count_ensure(gh, kh, false, pl.sp_h, g);
     if (!ctx.ntt_plans.count(c.log_n)) g.bytes += NttPlan::bytes_for(c.log_n);
     return g.bytes;
 }
@@ -561,7 +587,7 @@ void ProveSlot::collect_stats() {
         stats[10 + i] = el(m[i]->ev_acc0, m[i]->ev_end);
     }
     uint32_t ew = 0, eh = 0;
-    size_t nbw = (size_t)sortW.g.W * sortW.g.K, nbh = (size_t)sortH.g.W * sortH.g.K;
+    size_t nbw = (size_t)sortW.g.Wr * sortW.g.K, nbh = (size_t)sortH.g.Wr * sortH.g.K;
     ZK_HIP(hipMemcpy(&ew, sortW.offsets.as<uint32_t>() + nbw, 4, hipMemcpyDeviceToHost));
     ZK_HIP(hipMemcpy(&eh, sortH.offsets.as<uint32_t>() + nbh, 4, hipMemcpyDeviceToHost));
     stats[15] = ew;

@@ -41,8 +41,12 @@ struct Circuit {
     // extended bases, n_vars + 4 points each (see header comment); H: n points
     DevBuf A, B1, B2, C, H;
     // the bases again in the 29-bit-limb form of the bucket accumulation,
-    // each slot also holding the x of its endomorphism image (msm.hpp)
+    // each slot also holding the x of its endomorphism image (msm.hpp); with
+    // window tables, W tables of the array one after the other
     DevBuf A29, B1_29, B2_29, C29, H29;
+    // window of the tables (msm_points_table) of the witness arrays (A', B1',
+    // C', B2') and of H; 0: no tables, one slot per base
+    int tab_c[2] = {0, 0};
     ZkeyHeader hdr;               // VK in file layout
     std::vector<uint8_t> coefs;   // §4 bytes, kept to write the circuit back out
     std::vector<uint64_t> ic;     // (l+1) x 8
@@ -54,6 +58,12 @@ struct Circuit {
     std::vector<uint32_t> wit_levels;
     size_t device_bytes() const;
 };
+
+// Window tables of c's resident bases (msm.hpp): mode 1 builds them (window
+// msm_table_c per array size), 0 frees them, -1 builds them when they take
+// at most a third of the device memory still free. Waits for the device
+// first: no proof of c may be in flight.
+void circuit_set_tables(Circuit& c, int mode);
 
 // Device results of one proof's five MSMs (level-2 partials, host memory).
 struct ProofParts {
@@ -132,6 +142,13 @@ struct ProveSlot {
 constexpr int TUNE_C_MIN = 11, TUNE_C_MAX = 19;  // autotuner window range (c = 19: 7 windows)
 int size_class(size_t n);                         // ceil(log2 n)
 
+// windows and bucket splits of one proof's MSMs on a circuit
+struct MsmPlan {
+    int cw = 0, ch = 0;            // windows of the witness sort and of the h sort
+    bool mw = false, mh = false;   // over the circuit's window tables (one bucket set)
+    int sp_w1 = 1, sp_w2 = 1, sp_h = 1;   // splits: witness G1, witness G2, H
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -144,20 +161,25 @@ struct Context {
     std::unique_ptr<ProveSlot> slot;   // slot used by the one-shot prove entry point
     DevBuf comb_g1, comb_g2;           // fixed-base tables for synthetic setup
     // MSM windows: forced (nonzero) or tuned per (kind, ceil log2 n), else the
-    // size default. kind 0 = witness sort (A', B1', C', B2'), 1 = H. With a
-    // tuned window come the tuned bucket splits of its G1 and G2 MSMs; else
-    // msm_default_split.
+    // size default. kind 0 = witness sort (A', B1', C', B2'), 1 = H; kinds 2
+    // and 3 are the same MSMs over window tables, whose window is the
+    // tables' (only their splits are tuned). With a tuned window come the
+    // tuned bucket splits of its G1 and G2 MSMs; else msm_default_split.
     struct MsmChoice {
         int c, split_g1, split_g2;   // split 0: msm_default_split
     };
     int force_w = 0, force_h = 0;
     std::map<std::pair<int, int>, MsmChoice> tuned;
+    int tables_mode = -1;   // circuit_set_tables mode for circuits loaded from now on
     explicit Context(int dev);
     ~Context();
     NttPlan& ntt_plan(int log_n);
     void ensure_comb();
     int window_for(int kind, size_t n) const;
-    int split_for(int kind, size_t n, bool g2) const;
+    int split_for(int kind, size_t n, int c, bool g2) const;
+    // windows and splits of a proof of c: its window tables unless a window
+    // with another window count is forced
+    MsmPlan msm_plan(const Circuit& c) const;
     std::string arch() const;
     void autotune(const Circuit& c, bool force, int reps, int chosen[2]);
     int tune_load(const std::string& path);

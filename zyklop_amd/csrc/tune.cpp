@@ -12,6 +12,8 @@
 //   kind 0: the witness sort over n_vars + 4 scalars and its four MSMs
 //           (A', B1', C' in G1, B2' in G2), the cost one proof pays for c_w;
 //   kind 1: the sort of h over n scalars and the H MSM (c_h).
+// A circuit with window tables fixes the window (the tables'): for it only
+// the splits are tuned, recorded under kind 2 (witness) or 3 (H).
 // Every proof on the context then uses the table unless a window is forced.
 #include <stdio.h>
 #include <string.h>
@@ -38,14 +40,42 @@ int Context::window_for(int kind, size_t n) const {
     return it != tuned.end() ? it->second.c : msm_default_c(n);
 }
 
-int Context::split_for(int kind, size_t n, bool g2) const {
-    const bool forced = kind == 0 ? force_w != 0 : force_h != 0;
+int Context::split_for(int kind, size_t n, int c, bool g2) const {
+    const bool forced = (kind & 1) == 0 ? force_w != 0 : force_h != 0;
     auto it = tuned.find({kind, size_class(n)});
-    if (!forced && it != tuned.end()) {
+    if (!forced && it != tuned.end() && it->second.c == c) {
         const int sp = g2 ? it->second.split_g2 : it->second.split_g1;
         if (sp > 0) return sp;
     }
-    return msm_default_split(n, window_for(kind, n), g2);
+    return msm_default_split(n, c, g2, kind >= 2);
+}
+
+MsmPlan Context::msm_plan(const Circuit& circ) const {
+    MsmPlan p;
+    for (int k = 0; k < 2; k++) {
+        const size_t n = k ? (size_t)circ.n : (size_t)circ.n_vars + 4;
+        const int forced = k ? force_h : force_w, tc = circ.tab_c[k];
+        bool merged = false;
+        int c;
+        if (tc && (!forced || msm_geom(n, forced, false).W == msm_geom(n, tc, false).W)) {
+            c = tc;
+            merged = true;
+        } else {
+            c = window_for(k, n);
+        }
+        const int kind = k + (merged ? 2 : 0);
+        if (k == 0) {
+            p.cw = c;
+            p.mw = merged;
+            p.sp_w1 = split_for(kind, n, c, false);
+            p.sp_w2 = split_for(kind, n, c, true);
+        } else {
+            p.ch = c;
+            p.mh = merged;
+            p.sp_h = split_for(kind, n, c, false);
+        }
+    }
+    return p;
 }
 
 std::string Context::arch() const {
@@ -83,30 +113,29 @@ float elapsed(TuneBufs& tb) {
 // Device ms at window c, mean over reps after one warm-up: the sort, and each
 // MSM of `kind` at each split 1..MSM_SPLIT_MAX (msm[m][split-1]; splits that
 // would start more than 2^21 threads are skipped and left at -1).
-void time_window(Context& ctx, const Circuit& circ, int kind, int c, const void* d_sc, size_t n, int reps,
-                 TuneBufs& tb, float& sort_ms, float msm[4][MSM_SPLIT_MAX]) {
+void time_window(Context& ctx, const Circuit& circ, int kind, int c, bool merged, const void* d_sc, size_t n,
+                 int reps, TuneBufs& tb, float& sort_ms, float msm[4][MSM_SPLIT_MAX]) {
     hipStream_t st = ctx.stream;
-    struct M { const void* pts; const void* p29; bool g2; };
-    const M w4[4] = {{circ.A.p, circ.A29.p, false}, {circ.B1.p, circ.B1_29.p, false}, {circ.C.p, circ.C29.p, false},
-                     {circ.B2.p, circ.B2_29.p, true}};
-    const M h1[1] = {{circ.H.p, circ.H29.p, false}};
+    struct M { const void* p29; bool g2; };
+    const M w4[4] = {{circ.A29.p, false}, {circ.B1_29.p, false}, {circ.C29.p, false}, {circ.B2_29.p, true}};
+    const M h1[1] = {{circ.H29.p, false}};
     const M* ms = kind == 0 ? w4 : h1;
     const int nm = kind == 0 ? 4 : 1;
-    const MsmGeom g = msm_geom(n, c, false);
+    const MsmGeom g = msm_geom(n, c, false, merged);
     sort_ms = 0;
     for (int m = 0; m < 4; m++)
         for (int s = 0; s < MSM_SPLIT_MAX; s++) msm[m][s] = -1;
     for (int it = 0; it <= reps; it++) {
         ZK_HIP(hipEventRecord(tb.e0, st));
-        tb.sort.run(d_sc, n, c, st);
+        tb.sort.run(d_sc, n, c, st, merged);
         ZK_HIP(hipEventRecord(tb.e1, st));
         const float t_sort = elapsed(tb);
         if (it > 0) sort_ms += t_sort / reps;
         for (int m = 0; m < nm; m++)
             for (int s = 1; s <= MSM_SPLIT_MAX; s++) {
-                if ((size_t)g.W * g.K * s > ((size_t)1 << 21) && s > 1) continue;
+                if ((size_t)g.Wr * g.K * s > ((size_t)1 << 21) && s > 1) continue;
                 ZK_HIP(hipEventRecord(tb.e0, st));
-                msm_launch(tb.sort, ms[m].pts, ms[m].g2, ms[m].g2 ? tb.g2 : tb.g1, tb.part, st, ms[m].p29, s);
+                msm_launch(tb.sort, ms[m].g2, ms[m].g2 ? tb.g2 : tb.g1, tb.part, st, ms[m].p29, s);
                 ZK_HIP(hipEventRecord(tb.e1, st));
                 const float t = elapsed(tb);
                 if (it > 0) msm[m][s - 1] = (msm[m][s - 1] < 0 ? 0 : msm[m][s - 1]) + t / reps;
@@ -125,7 +154,8 @@ void Context::autotune(const Circuit& circ, bool force, int reps, int chosen[2])
     for (int kind = 0; kind < 2; kind++) {
         const size_t n = sizes[kind];
         const int cls = size_class(n);
-        auto key = std::make_pair(kind, cls);
+        const int tc = circ.tab_c[kind];
+        auto key = std::make_pair(kind + (tc ? 2 : 0), cls);
         if (!force && tuned.count(key)) {
             chosen[kind] = tuned[key].c;
             continue;
@@ -135,21 +165,25 @@ void Context::autotune(const Circuit& circ, bool force, int reps, int chosen[2])
         DevBuf d_sc;
         d_sc.ensure(n * 32);
         ZK_HIP(hipMemcpy(d_sc.p, sc.data(), n * 32, hipMemcpyHostToDevice));
-        MsmChoice best{msm_default_c(n), 1, 1};
+        MsmChoice best{tc ? tc : msm_default_c(n), 1, 1};
         float best_ms = -1;
         // candidate windows around the size (wide for large MSMs, narrow for
         // small), one per window count: widths with the same count give the
-        // same geometry (msm_geom)
+        // same geometry (msm_geom); over window tables, the tables' window
         const int hi = std::min(TUNE_C_MAX, std::max(TUNE_C_MIN, cls - 2));
         const int lo = std::max(TUNE_C_MIN, std::min(cls - 8, hi - 5));
         std::vector<int> cands;
-        for (int c = lo; c <= hi; c++) {
-            const MsmGeom g = msm_geom(n, c, false);
-            if (cands.empty() || msm_geom(n, cands.back(), false).W != g.W) cands.push_back(g.c);
+        if (tc) {
+            cands.push_back(tc);
+        } else {
+            for (int c = lo; c <= hi; c++) {
+                const MsmGeom g = msm_geom(n, c, false);
+                if (cands.empty() || msm_geom(n, cands.back(), false).W != g.W) cands.push_back(g.c);
+            }
         }
         for (int c : cands) {
             float sort_ms, msm[4][MSM_SPLIT_MAX];
-            time_window(*this, circ, kind, c, d_sc.p, n, reps, tb, sort_ms, msm);
+            time_window(*this, circ, kind, c, tc != 0, d_sc.p, n, reps, tb, sort_ms, msm);
             float total = sort_ms;
             int sp[4] = {1, 1, 1, 1};
             // a larger split adds level-1 work (one addition per extra part
@@ -173,8 +207,9 @@ void Context::autotune(const Circuit& circ, bool force, int reps, int chosen[2])
     }
 }
 
-// text table, one entry per line: "<arch> <kind> <log2 n> <c> [<split G1> <split G2>]";
-// '#' starts a comment; an entry without splits uses msm_default_split
+// text table, one entry per line: "<arch> <kind> <log2 n> <c> [<split G1> <split G2>]"
+// (kind 0 witness, 1 h, 2 and 3 the same over window tables); '#' starts a
+// comment; an entry without splits uses msm_default_split
 int Context::tune_load(const std::string& path) {
     std::ifstream f(path);
     ZK_REQUIRE(f.good(), ZK_E_IO, "cannot open tuning table " + path);
@@ -188,7 +223,7 @@ int Context::tune_load(const std::string& path) {
         int kind, cls, c, s1 = 0, s2 = 0;
         ZK_REQUIRE(bool(is >> a >> kind >> cls >> c), ZK_E_FORMAT, "bad tuning table line: " + line);
         if (is >> s1) ZK_REQUIRE(bool(is >> s2), ZK_E_FORMAT, "bad tuning table line: " + line);
-        ZK_REQUIRE(kind >= 0 && kind <= 1 && cls >= 0 && cls <= 40 && c >= 4 && c <= 22 && s1 >= 0 &&
+        ZK_REQUIRE(kind >= 0 && kind <= 3 && cls >= 0 && cls <= 40 && c >= 4 && c <= 22 && s1 >= 0 &&
                        s1 <= MSM_SPLIT_MAX && s2 >= 0 && s2 <= MSM_SPLIT_MAX,
                    ZK_E_FORMAT, "bad tuning table entry: " + line);
         if (a != me) continue;
@@ -201,7 +236,8 @@ int Context::tune_load(const std::string& path) {
 void Context::tune_save(const std::string& path) const {
     std::ofstream f(path, std::ios::trunc);
     ZK_REQUIRE(f.good(), ZK_E_IO, "cannot write tuning table " + path);
-    f << "# zyklop_amd MSM windows: arch kind(0 witness, 1 h) ceil-log2(n) c split-G1 split-G2\n";
+    f << "# zyklop_amd MSM windows: arch kind(0 witness, 1 h, 2/3 over window tables) ceil-log2(n) c split-G1 "
+         "split-G2\n";
     const std::string me = arch();
     for (auto& kv : tuned)
         f << me << ' ' << kv.first.first << ' ' << kv.first.second << ' ' << kv.second.c << ' ' << kv.second.split_g1
