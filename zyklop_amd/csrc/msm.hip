@@ -196,7 +196,10 @@ constexpr uint32_t SORT_GROUP_LDS = 65536 - 256;   // dynamic LDS bytes of k_sor
 // gfx950 gives one workgroup up to 160 KiB of LDS once the kernel asks for
 // more than the default 64 KiB
 constexpr uint32_t SORT_GROUP_LDS_MAX = 160 * 1024 - 1024;
-__global__ void __launch_bounds__(256) k_sort_group(const uint2* __restrict__ pairs, const uint32_t* __restrict__ mscan,
+// 16 waves per group: the LDS stage often leaves room for one workgroup per
+// CU, whose waves then carry the pass's LDS atomics and loads alone
+constexpr int SORT_GROUP_THREADS = 1024;
+__global__ void __launch_bounds__(1024) k_sort_group(const uint2* __restrict__ pairs, const uint32_t* __restrict__ mscan,
                                                     uint32_t nt, int s, uint32_t nb, uint32_t stage_cap,
                                                     uint32_t* __restrict__ offsets, uint32_t* __restrict__ entries) {
     extern __shared__ uint32_t cnt[];   // 2^s counters, then the stage
@@ -323,9 +326,10 @@ static SortShape sort_shape(size_t n, int c, bool merged) {
     z.sh = 7;
     while (((z.nb + (1ull << z.sh) - 1) >> z.sh) > 4096) z.sh++;
     z.G = (uint32_t)((z.nb + (1ull << z.sh) - 1) >> z.sh);
-    // tiles of pass A: multiples of 4096 scalars (long runs per group for the
-    // pair writes), at most 2048 tiles
-    z.tile = (size_t)4096 * std::max<size_t>(1, (z.n2 + 4096 * 2048 - 1) / (4096 * 2048));
+    // tiles of pass A: multiples of 4096 scalars, at most 512 tiles (two
+    // workgroups per CU): longer runs per group and tile for the pair writes,
+    // whose short runs wrote several times their bytes
+    z.tile = (size_t)4096 * std::max<size_t>(1, (z.n2 + 4096 * 512 - 1) / (4096 * 512));
     z.nt = (uint32_t)std::max<size_t>(1, (z.n2 + z.tile - 1) / z.tile);
     z.mlen = (size_t)z.G * z.nt + 1;
     z.ne = std::max<size_t>(1, (size_t)z.g.W * z.n2);
@@ -429,7 +433,7 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st, bool 
             (void)hipGetLastError();
     }
     const uint32_t stage_cap = (uint32_t)std::min<size_t>((lds_budget - (4u << sh)) / 4, want);
-    k_sort_group<<<G, 256, 4u * ((1u << sh) + stage_cap), st>>>(pairs.as<uint2>(), msc, nt, sh, (uint32_t)nb, stage_cap,
+    k_sort_group<<<G, SORT_GROUP_THREADS, 4u * ((1u << sh) + stage_cap), st>>>(pairs.as<uint2>(), msc, nt, sh, (uint32_t)nb, stage_cap,
                                                                 offsets.as<uint32_t>(), entries.as<uint32_t>());
     ZK_HIP(hipGetLastError());
     const uint32_t* off = offsets.as<uint32_t>();
