@@ -73,18 +73,26 @@ ZK_DEV F29 f29_const(const uint32_t (&c)[9]) {
     return r;
 }
 
-// (lo * (-p^-1)) mod 2^29 with one full-rate v_mad_u64_u32 (a 32-bit
+// (lo * np) mod 2^29 with one full-rate v_mad_u64_u32 (a 32-bit
 // v_mul_lo_u32 would issue at quarter rate); its carry-out goes to an SGPR
-// pair that nothing reads
+// pair that nothing reads. np = -m^-1 mod 2^29 for the modulus m.
+template <uint32_t NPV = f29c::NP>
 ZK_DEV uint32_t f29_qword(uint32_t lo) {
     uint64_t t, cc;
-    asm("v_mad_u64_u32 %0, %1, %2, %3, 0" : "=v"(t), "=s"(cc) : "v"(lo), "s"(f29c::NP));
+    asm("v_mad_u64_u32 %0, %1, %2, %3, 0" : "=v"(t), "=s"(cc) : "v"(lo), "s"(NPV));
     (void)cc;
     return (uint32_t)t & f29c::MASK;
 }
 
-// Montgomery product a*b/R' mod p (product scanning, one 64-bit accumulator)
-ZK_DEV F29 f29_mul(const F29& a, const F29& b) {
+// moduli of the 29-bit Montgomery product: limbs of m and -m^-1 mod 2^29
+struct F29ModP {
+    static constexpr const uint32_t* L = f29c::P;
+    static constexpr uint32_t NEG_INV = f29c::NP;
+};
+
+// Montgomery product a*b/R' mod m (product scanning, one 64-bit accumulator)
+template <class M>
+ZK_DEV F29 f29_mul_mod(const F29& a, const F29& b) {
     using namespace f29c;
     uint32_t m[9];
     F29 t;
@@ -99,11 +107,11 @@ ZK_DEV F29 f29_mul(const F29& a, const F29& b) {
 #pragma unroll
         for (int i = 0; i < 9; i++) {
             const int j = k - i;
-            if (i < k && j >= 0 && j < 9) acc += (uint64_t)m[i] * P[j];
+            if (i < k && j >= 0 && j < 9) acc += (uint64_t)m[i] * M::L[j];
         }
         if (k < 9) {
-            m[k] = f29_qword((uint32_t)acc);
-            acc += (uint64_t)m[k] * P[0];
+            m[k] = f29_qword<M::NEG_INV>((uint32_t)acc);
+            acc += (uint64_t)m[k] * M::L[0];
         } else {
             t.l[k - 9] = (uint32_t)acc & MASK;
         }
@@ -116,6 +124,8 @@ ZK_DEV F29 f29_mul(const F29& a, const F29& b) {
     t.l[8] = (uint32_t)acc;
     return t;
 }
+
+ZK_DEV F29 f29_mul(const F29& a, const F29& b) { return f29_mul_mod<F29ModP>(a, b); }
 
 // Montgomery square a*a/R' mod p for normalized a: 45 partial products
 // (the off-diagonal a_i*a_j, i < j, taken once against 2*a_j, which stays
@@ -484,6 +494,81 @@ ZK_DEV void f29_load_point(const void* base, uint32_t ref, F29& x, F29& y) {
                      ((ref >> 28) & 4u);
     y = f29_load_packed(p);
     x = f29_load_packed(p + 2);
+}
+
+// ---------------------------------------------------------------- Fr (NTT)
+// BN254's group order r in the same limbs, for the NTT. Its data stay in the
+// standard form (x*2^256 mod r, canonical, 8 words in memory); its tables are
+// prescaled by 2^5 = R'/2^256, so that a*b/R' of a standard-form a and a
+// prescaled b is the standard-form product and no element changes form.
+// Bounds (r ~ 2^253.6, R'/r ~ 169): for b < r and any a < 2^261 whose limbs
+// are below 2^31, a*b/R' < r*(a/R') + r < 2r, and a column sums at most 9
+// products below 2^60, 9 below 2^58 and the carry, under 2^64.
+namespace fr29c {
+constexpr uint32_t R[9] = {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u,
+                           0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu};
+constexpr uint32_t NR = 0x0fffffffu;   // -r^-1 mod 2^29
+// 2^k r with every limb but the top one in [2^29, 2^30) (borrowed from the
+// next limb), so that 2^k r - b never borrows for normalized b < 2^k r
+constexpr uint32_t R2[9] = {0x20000002u, 0x3e1f593eu, 0x3cb848a0u, 0x2fa121e5u, 0x2b0ba505u,
+                            0x25b68180u, 0x214dc281u, 0x3cb84c67u, 0x0060c89bu};
+constexpr uint32_t R4[9] = {0x20000004u, 0x3c3eb27du, 0x39709142u, 0x3f4243ccu, 0x36174a0bu,
+                            0x2b6d0301u, 0x229b8503u, 0x397098cfu, 0x00c19138u};
+constexpr uint32_t R8[9] = {0x20000008u, 0x387d64fbu, 0x32e12286u, 0x3e84879au, 0x2c2e9418u,
+                            0x36da0604u, 0x25370a07u, 0x32e1319fu, 0x01832272u};
+constexpr uint32_t R16[9] = {0x20000010u, 0x30fac9f7u, 0x25c2450eu, 0x3d090f36u, 0x385d2832u,
+                             0x2db40c09u, 0x2a6e1410u, 0x25c2633fu, 0x030644e6u};
+constexpr uint32_t RTOP1 = 3171407u;   // (r >> 232) + 1
+template <int K> struct ROff;          // 2^K r, borrowed
+template <> struct ROff<1> { static constexpr const uint32_t* L = R2; };
+template <> struct ROff<2> { static constexpr const uint32_t* L = R4; };
+template <> struct ROff<3> { static constexpr const uint32_t* L = R8; };
+template <> struct ROff<4> { static constexpr const uint32_t* L = R16; };
+}  // namespace fr29c
+
+struct F29ModR {
+    static constexpr const uint32_t* L = fr29c::R;
+    static constexpr uint32_t NEG_INV = fr29c::NR;
+};
+
+ZK_DEV F29 fr29_mul(const F29& a, const F29& b) { return f29_mul_mod<F29ModR>(a, b); }
+
+// a - b + 2^K r limb-wise, for normalized b < 2^K r: limbs below 2^31 (the
+// top one wraps below zero only when a carry from below makes it up)
+template <int K>
+ZK_DEV F29 fr29_sub(const F29& a, const F29& b) {
+    F29 r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.l[i] = a.l[i] + fr29c::ROff<K>::L[i] - b.l[i];
+    return r;
+}
+
+// normalized value < 2^261 -> value < 2r (subtract q*r, q from the top limb:
+// q <= value/r and value - q*r < r + value/(r >> 232) < 2r)
+ZK_DEV F29 fr29_reduce(const F29& a) {
+    const uint32_t q = a.l[8] / fr29c::RTOP1;
+    F29 r;
+    int64_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const int64_t x = (int64_t)a.l[i] - (int64_t)((uint64_t)q * fr29c::R[i]) + c;
+        r.l[i] = (uint32_t)x & f29c::MASK;
+        c = x >> 29;
+    }
+    return r;
+}
+
+// normalized value < 2r -> canonical (< r)
+ZK_DEV F29 fr29_canon(const F29& a) {
+    F29 d;
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const int32_t x = (int32_t)a.l[i] - (int32_t)fr29c::R[i] + c;
+        d.l[i] = (uint32_t)x & f29c::MASK;
+        c = x >> 29;
+    }
+    return c < 0 ? a : d;
 }
 
 // ---------------------------------------------------------------- Fq2 (G2)

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "field.hpp"
+#include "field29.hpp"
 #include "host_field.hpp"
 #include "ntt.hpp"
 
@@ -18,7 +19,8 @@ struct NttPtrs {
     void* out[NTT_MAX_BATCH];
 };
 
-// w_R^i for i < R/2 (Montgomery), passed by value so they live in SGPRs
+// w_R^i for i < R/2 (standard form, prescaled by 2^5: field29.hpp "Fr"),
+// passed by value so they live in SGPRs
 struct RootsR {
     Fr w[8];
 };
@@ -34,38 +36,74 @@ constexpr int bitrev_c(int x, int bits) {
     return r;
 }
 
+// ---- elements: 9 x 29-bit limbs in registers, 8 canonical words in memory ----
+// A product by a prescaled table value keeps the standard form (field29.hpp).
+ZK_DEV F29 ld29(const void* base, size_t i) {
+    const uint4* p = reinterpret_cast<const uint4*>(base) + 2 * i;
+    const uint4 a = p[0], b = p[1];
+    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    return f29_unpack(w);
+}
+// x normalized and below 2r
+ZK_DEV void st29(void* base, size_t i, const F29& x) {
+    uint32_t w[8];
+    f29_pack(fr29_canon(x), w);
+    uint4* p = reinterpret_cast<uint4*>(base) + 2 * i;
+    p[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    p[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+// limbs below 2^31, value below 2^261 -> normalized, below r + 2^240
+ZK_DEV F29 settle29(const F29& x) { return fr29_reduce(f29_norm(x)); }
+
 // one radix-2 DIT stage (span 2^(S-1)) of a 2^LR-point DFT held in registers;
-// every loop has a constant trip count so the whole DFT unrolls to registers
+// every loop has a constant trip count so the whole DFT unrolls to registers.
+// Bounds (worst cases followed stage by stage in an exact model): the DFT's
+// inputs are below 1.2r (canonical loads, products by a prescaled table
+// value, below 1.01r, or settled values, below r + 2^240); stage S then
+// sees b below 2^S r with a top limb below that of the borrowed 2^S r, so
+// a - b + 2^S r has no negative limb; outputs stay below 32r (radix 16),
+// limbs below 2^31. Between stages the limbs are normalized; the last
+// stage's outputs go straight to a product (limbs below 2^31 are fine
+// there) or to settle29.
 template <int LR, int S>
 struct DftStage {
-    ZK_DEV static void run(Fr (&x)[1 << LR], const RootsR& rt) {
+    ZK_DEV static void run(F29 (&x)[1 << LR], const F29 (&w)[8]) {
         constexpr int R = 1 << LR, half = 1 << (S - 1);
 #pragma unroll
         for (int q = 0; q < R / 2; q++) {
             const int p = q & (half - 1);
             const int i0 = (q >> (S - 1)) * 2 * half + p, i1 = i0 + half;
-            Fr a = x[i0], b = x[i1];
-            if (p) b = mul<FrCfg>(b, rt.w[p * (R >> S)]);
-            x[i0] = add<FrCfg>(a, b);
-            x[i1] = sub<FrCfg>(a, b);
+            const F29 a = x[i0];
+            F29 b = x[i1];
+            if (p) b = fr29_mul(b, w[p * (R >> S)]);
+            x[i0] = f29_add(a, b);
+            x[i1] = fr29_sub<S>(a, b);
         }
-        DftStage<LR, S + 1>::run(x, rt);
+        if (S < LR) {
+#pragma unroll
+            for (int i = 0; i < R; i++) x[i] = f29_norm(x[i]);
+        }
+        DftStage<LR, S + 1>::run(x, w);
     }
 };
 template <int LR>
 struct DftStage<LR, LR + 1> {
-    ZK_DEV static void run(Fr (&)[1 << LR], const RootsR&) {}
+    ZK_DEV static void run(F29 (&)[1 << LR], const F29 (&)[8]) {}
 };
 
 // radix-2^LR DFT in registers, X_k = sum_b x_b w_R^(b k); the caller loads
 // input b into x[bitrev(b)] and gets natural-order output
 template <int LR>
-ZK_DEV void dft_reg(Fr (&x)[1 << LR], const RootsR& rt) {
-    DftStage<LR, 1>::run(x, rt);
+ZK_DEV void dft_reg(F29 (&x)[1 << LR], const RootsR& rt) {
+    F29 w[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) w[i] = f29_unpack(rt.w[i].v);
+    DftStage<LR, 1>::run(x, w);
 }
 
 // In-place pass j < np-1 over blocks of M = R*S elements: thread (beta, c)
-// transforms x[beta*M + b*S + c], b < R, then multiplies output k by w_M^(c k).
+// transforms x[beta*M + b*S + c], b < R, then multiplies output k by w_M^(c k)
+// (the table holds w_M^0 for c = 0, so every output k >= 1 is a product).
 template <int LR>
 __global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass(NttPtrs ptrs, int logM, size_t n, RootsR rt,
                                                           const void* __restrict__ tw, const void* __restrict__ pre) {
@@ -76,20 +114,19 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass(NttPtrs ptrs, int logM
     const int logS = logM - LR;
     const size_t c = tid & (((size_t)1 << logS) - 1);
     const size_t base = (tid >> logS) << logM;
-    Fr x[R];
+    F29 x[R];
 #pragma unroll
-    for (int b = 0; b < R; b++) x[bitrev_c(b, LR)] = load<FrCfg>(data, base + ((size_t)b << logS) + c);
-    if (pre) {
-#pragma unroll
-        for (int b = 0; b < R; b++)
-            x[bitrev_c(b, LR)] = mul<FrCfg>(x[bitrev_c(b, LR)], load<FrCfg>(pre, base + ((size_t)b << logS) + c));
+    for (int b = 0; b < R; b++) {
+        const size_t i = base + ((size_t)b << logS) + c;
+        F29 v = ld29(data, i);
+        if (pre) v = fr29_mul(v, ld29(pre, i));
+        x[bitrev_c(b, LR)] = v;
     }
     dft_reg<LR>(x, rt);
 #pragma unroll
     for (int k = 0; k < R; k++) {
-        Fr y = x[k];
-        if (k && c) y = mul<FrCfg>(y, load<FrCfg>(tw, ((size_t)k << logS) + c));
-        store<FrCfg>(data, base + ((size_t)k << logS) + c, y);
+        const F29 y = k ? fr29_mul(x[k], ld29(tw, ((size_t)k << logS) + c)) : settle29(x[k]);
+        st29(data, base + ((size_t)k << logS) + c, y);
     }
 }
 
@@ -104,14 +141,15 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass(NttPtrs ptrs, int logM
 //           multiplies output k2 by w_S^(c2 k2) and stores x[beta*M + k*S +
 //           k2*S' + c2].
 // Consecutive lanes take consecutive units, so global accesses stay coalesced
-// (runs of min(S', U) elements).
+// (runs of min(S', U) elements). The LDS holds the 9 limbs of each staged
+// element (normalized, below 1.2r), limb-major.
 template <int LR1, int LR2, int U>
 __global__ void __launch_bounds__((1 << LR2) * U) k_ntt_pass2(NttPtrs ptrs, int logM, size_t n, RootsR rt1, RootsR rt2,
                                                               const void* __restrict__ tw1, const void* __restrict__ tw2,
                                                               const void* __restrict__ pre) {
-    constexpr int R1 = 1 << LR1, R2 = 1 << LR2;
+    constexpr int R1 = 1 << LR1, R2 = 1 << LR2, NS = R1 * R2 * U;
     static_assert(R1 == R2, "one task per thread in both steps");
-    __shared__ Fr lds[R1 * R2 * U];
+    __shared__ uint32_t lds[9 * NS];
     void* data = ptrs.in[blockIdx.y];
     const int logS = logM - LR1, logS2 = logS - LR2;
     const size_t S2 = (size_t)1 << logS2;
@@ -123,35 +161,39 @@ __global__ void __launch_bounds__((1 << LR2) * U) k_ntt_pass2(NttPtrs ptrs, int 
     if (live) {
         // step 1: pass j over b for column c = q*S' + c2
         const size_t c = ((size_t)q << logS2) + c2;
-        Fr x[R1];
+        F29 x[R1];
 #pragma unroll
-        for (int b = 0; b < R1; b++) x[bitrev_c(b, LR1)] = load<FrCfg>(data, base + ((size_t)b << logS) + c);
-        if (pre) {
-#pragma unroll
-            for (int b = 0; b < R1; b++)
-                x[bitrev_c(b, LR1)] = mul<FrCfg>(x[bitrev_c(b, LR1)], load<FrCfg>(pre, base + ((size_t)b << logS) + c));
+        for (int b = 0; b < R1; b++) {
+            const size_t i = base + ((size_t)b << logS) + c;
+            F29 v = ld29(data, i);
+            if (pre) v = fr29_mul(v, ld29(pre, i));
+            x[bitrev_c(b, LR1)] = v;
         }
         dft_reg<LR1>(x, rt1);
 #pragma unroll
         for (int k = 0; k < R1; k++) {
-            Fr y = x[k];
-            if (k && c) y = mul<FrCfg>(y, load<FrCfg>(tw1, ((size_t)k << logS) + c));
-            lds[(k * R2 + q) * U + ul] = y;
+            const F29 y = k ? fr29_mul(x[k], ld29(tw1, ((size_t)k << logS) + c)) : settle29(x[k]);
+#pragma unroll
+            for (int l = 0; l < 9; l++) lds[l * NS + (k * R2 + q) * U + ul] = y.l[l];
         }
     }
     __syncthreads();
     if (!live) return;
     // step 2: pass j+1 over b2 for sub-block (beta, k = q), column c2
-    Fr x[R2];
+    F29 x[R2];
 #pragma unroll
-    for (int b2 = 0; b2 < R2; b2++) x[bitrev_c(b2, LR2)] = lds[(q * R2 + b2) * U + ul];
+    for (int b2 = 0; b2 < R2; b2++) {
+        F29 v;
+#pragma unroll
+        for (int l = 0; l < 9; l++) v.l[l] = lds[l * NS + (q * R2 + b2) * U + ul];
+        x[bitrev_c(b2, LR2)] = v;
+    }
     dft_reg<LR2>(x, rt2);
     const size_t sub = base + ((size_t)q << logS);
 #pragma unroll
     for (int k2 = 0; k2 < R2; k2++) {
-        Fr y = x[k2];
-        if (k2 && c2) y = mul<FrCfg>(y, load<FrCfg>(tw2, ((size_t)k2 << logS2) + c2));
-        store<FrCfg>(data, sub + ((size_t)k2 << logS2) + c2, y);
+        const F29 y = k2 ? fr29_mul(x[k2], ld29(tw2, ((size_t)k2 << logS2) + c2)) : settle29(x[k2]);
+        st29(data, sub + ((size_t)k2 << logS2) + c2, y);
     }
 }
 
@@ -172,21 +214,20 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_final(NttPtrs ptrs, NttDigi
         r >>= dg.lr[d];
         row = (row << dg.lr[d]) | digit;
     }
-    Fr x[R];
+    F29 x[R];
 #pragma unroll
-    for (int b = 0; b < R; b++) x[bitrev_c(b, LR)] = load<FrCfg>(in, (row << LR) + b);
-    if (pre && dg.np == 1) {
-#pragma unroll
-        for (int b = 0; b < R; b++)
-            x[bitrev_c(b, LR)] = mul<FrCfg>(x[bitrev_c(b, LR)], load<FrCfg>(pre, (row << LR) + b));
+    for (int b = 0; b < R; b++) {
+        const size_t i = (row << LR) + b;
+        F29 v = ld29(in, i);
+        if (pre && dg.np == 1) v = fr29_mul(v, ld29(pre, i));
+        x[bitrev_c(b, LR)] = v;
     }
     dft_reg<LR>(x, rt);
 #pragma unroll
     for (int k = 0; k < R; k++) {
-        size_t o = rv + (size_t)k * O;
-        Fr y = x[k];
-        if (post) y = mul<FrCfg>(y, load<FrCfg>(post, o));
-        store<FrCfg>(out, o, y);
+        const size_t o = rv + (size_t)k * O;
+        const F29 y = post ? fr29_mul(x[k], ld29(post, o)) : settle29(x[k]);
+        st29(out, o, y);
     }
 }
 
@@ -275,13 +316,18 @@ void NttPlan::build(int ln, hipStream_t st) {
     std::vector<zkh::Fr> lo_f, hi_f, lo_i, hi_i;
     host_pow_tables(w2n, ln + 1, lo_f, hi_f);
     host_pow_tables(w2n_inv, ln + 1, lo_i, hi_i);
-    DevBuf d_lo_f(4096 * 32), d_hi_f(hi_f.size() * 32), d_lo_i(4096 * 32), d_hi_i(hi_i.size() * 32), d_scale(32);
+    DevBuf d_lo_f(4096 * 32), d_hi_f(hi_f.size() * 32), d_lo_i(4096 * 32), d_hi_i(hi_i.size() * 32), d_scale(32),
+        d_s32(32);
     ZK_HIP(hipMemcpyAsync(d_lo_f.p, lo_f.data(), 4096 * 32, hipMemcpyHostToDevice, st));
     ZK_HIP(hipMemcpyAsync(d_hi_f.p, hi_f.data(), hi_f.size() * 32, hipMemcpyHostToDevice, st));
     ZK_HIP(hipMemcpyAsync(d_lo_i.p, lo_i.data(), 4096 * 32, hipMemcpyHostToDevice, st));
     ZK_HIP(hipMemcpyAsync(d_hi_i.p, hi_i.data(), hi_i.size() * 32, hipMemcpyHostToDevice, st));
-    zkh::Fr ninv = zkh::Fr::from_u64(n).inv();
+    // every table value the kernels multiply by is prescaled by 2^5 (the
+    // 29-bit product's R'/2^256, field29.hpp "Fr")
+    const zkh::Fr s32 = zkh::Fr::from_u64(32);
+    const zkh::Fr ninv = zkh::Fr::from_u64(n).inv() * s32;
     ZK_HIP(hipMemcpyAsync(d_scale.p, &ninv, 32, hipMemcpyHostToDevice, st));
+    ZK_HIP(hipMemcpyAsync(d_s32.p, &s32, 32, hipMemcpyHostToDevice, st));
     const uint64_t mask2n = (2 * (uint64_t)n) - 1;
 
     // radix roots w_R^i = w_2n^(i * 2n / R), forward and inverse, on the host
@@ -289,7 +335,7 @@ void NttPlan::build(int ln, hipStream_t st) {
         int R = 1 << lr[j];
         for (int i = 0; i < 8; i++) {
             uint64_t e = i < R / 2 ? ((uint64_t)i * ((2 * (uint64_t)n) >> lr[j])) & mask2n : 0;
-            zkh::Fr f = lo_f[e & 4095] * hi_f[e >> 12], v = lo_i[e & 4095] * hi_i[e >> 12];
+            zkh::Fr f = lo_f[e & 4095] * hi_f[e >> 12] * s32, v = lo_i[e & 4095] * hi_i[e >> 12] * s32;
             memcpy(roots_fwd[j] + 4 * i, f.v, 32);  // 4 words per element
             memcpy(roots_inv[j] + 4 * i, v.v, 32);
         }
@@ -305,9 +351,9 @@ void NttPlan::build(int ln, hipStream_t st) {
         uint64_t step_M = (uint64_t)1 << (ln + 1 - logM);  // w_M = w_2n^(2n/M)
         size_t cnt = (size_t)1 << logM;
         k_pow_table<<<cdiv(cnt, 256), 256, 0, st>>>((char*)tw_fwd.p + tw_off[j] * 32, cnt, d_lo_f.p, d_hi_f.p, 1, step_M,
-                                                   logS, mask2n, nullptr);
+                                                   logS, mask2n, d_s32.p);
         k_pow_table<<<cdiv(cnt, 256), 256, 0, st>>>((char*)tw_inv.p + tw_off[j] * 32, cnt, d_lo_i.p, d_hi_i.p, 1, step_M,
-                                                   logS, mask2n, nullptr);
+                                                   logS, mask2n, d_s32.p);
         consumed += lr[j];
     }
     // element-wise tables
@@ -315,7 +361,7 @@ void NttPlan::build(int ln, hipStream_t st) {
     inv_coset.ensure(n * 32);
     inv_plain.ensure(n * 32);
     inv_uncoset.ensure(n * 32);
-    k_pow_table<<<cdiv(n, 256), 256, 0, st>>>(coset_fwd.p, n, d_lo_f.p, d_hi_f.p, 0, 1, 0, mask2n, nullptr);
+    k_pow_table<<<cdiv(n, 256), 256, 0, st>>>(coset_fwd.p, n, d_lo_f.p, d_hi_f.p, 0, 1, 0, mask2n, d_s32.p);
     k_pow_table<<<cdiv(n, 256), 256, 0, st>>>(inv_coset.p, n, d_lo_f.p, d_hi_f.p, 0, 1, 0, mask2n, d_scale.p);
     k_pow_table<<<cdiv(n, 256), 256, 0, st>>>(inv_plain.p, n, d_lo_f.p, d_hi_f.p, 0, 0, 0, mask2n, d_scale.p);
     k_pow_table<<<cdiv(n, 256), 256, 0, st>>>(inv_uncoset.p, n, d_lo_i.p, d_hi_i.p, 0, 1, 0, mask2n, d_scale.p);
@@ -330,7 +376,7 @@ static void launch_pass(const NttPtrs& ptrs, int batch, int logM, size_t n, cons
     k_ntt_pass<LR><<<dim3(grid, batch), NTT_THREADS, 0, st>>>(ptrs, logM, n, rt, tw, pre);
 }
 
-constexpr int NTT2_UNITS = 32;
+constexpr int NTT2_UNITS = 16;   // 36 KiB of LDS per workgroup (9 limbs x 64 points x 16 units)
 
 static void launch_pass2(const NttPtrs& ptrs, int batch, int logM, size_t n, const RootsR& rt1, const RootsR& rt2,
                          const void* tw1, const void* tw2, const void* pre, hipStream_t st) {
