@@ -56,7 +56,7 @@ HBM_BYTES_PER_ENTRY_G2 = (2.3905e9 + 0.0671e9) / 14737409
 ALGO_BYTES_PER_ENTRY_G1 = 68            # y and x (or beta*x) of the base, 32 B each packed, + the 4 B entry
 ALGO_BYTES_PER_ENTRY_G2 = 132           # the same over Fp2, + the 4 B entry
 # NTT of 2^22, per transform (three pass pairs + the last pass)
-HBM_BYTES_NTT_2P22 = {"forward": 1.2245e9, "inverse": 1.3620e9, "coset": 1.3616e9}
+HBM_BYTES_NTT_2P22 = {"forward": 1.2290e9, "inverse": 1.3581e9, "coset": 1.3592e9}
 # SpMV of the 2^20 circuit (k_spmv_abc)
 HBM_BYTES_SPMV_2P20 = 1.3491e9 + 0.2017e9
 
@@ -357,8 +357,8 @@ def section_probes(args, zk, ctx, circ):
         "frac": round(gbs["forward"] / PEAK_HBM_GBS, 4), "traffic": int(HBM_BYTES_NTT_2P22["forward"]),
         "traffic_by_kind": {k: int(v) for k, v in HBM_BYTES_NTT_2P22.items()},
         "traffic_GBps": {k: round(HBM_BYTES_NTT_2P22[k] / (v / 1e3) / 1e9, 1) for k, v in ntt.items()},
-        "note": ("the passes are bound by Fr products on the VALU, not by HBM: the profiler's traffic runs at "
-                 "under a quarter of the HBM peak"),
+        "note": ("the passes wait on memory latency at 2 waves per SIMD (their LDS stage sets that), not on HBM "
+                 "bandwidth: the profiler's traffic runs at under a quarter of the HBM peak"),
         "avg_launch_ms": r3(ntt["forward"]), "algorithmic_bytes": 64 * n,
         "ms": {k: r3(v) for k, v in ntt.items()},
         "GBps": {k: round(v, 1) for k, v in gbs.items()},
