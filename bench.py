@@ -7,7 +7,8 @@ proofs of the same inputs):
 
   headline (2^20): the seeded synthetic R1CS with m = 2^20 constraints and 4096
       public inputs (3 terms per A/B row, C_i = aux_i; domain n = 2^21),
-      trapdoor CRS resident in HBM.
+      trapdoor CRS resident in HBM with the window tables of its bases
+      (--tables on|off|auto).
         plain   : one proof at a time (zk_prove_device, stream-synchronised)
         batched : the same proofs with several in flight, as streams
                   (zk_prove_batch_device); the captured-graph variant

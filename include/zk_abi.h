@@ -96,8 +96,9 @@ int zk_set_windows(zk_ctx* ctx, int window_witness, int window_h);
  * Costs (W - 1) more copies of the 29-bit base arrays in device memory
  * (W = 6..10, by size). Proofs are the same either way.
  * zk_set_window_tables: mode for circuits loaded on ctx from now on: 1 on,
- * 0 off, -1 (default) on when the tables take at most a third of the device
- * memory still free.
+ * 0 off, -1 on when the tables take at most a third of the device memory
+ * still free. A new context takes its mode from the environment variable
+ * ZK_WINDOW_TABLES (-1, 0 or 1) when it is set, else -1.
  * zk_circuit_window_tables: builds (1) or frees (0) c's tables, or decides
  * as above (-1), or only reports (-2); windows (may be NULL) receives the
  * tables' (witness, h) windows, 0 when c has none. Waits for the device: no

@@ -16,6 +16,7 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope="module")
 def ctx():
     c = zk.Context(0)
+    c.set_window_tables(1)   # whatever ZK_WINDOW_TABLES says for the other suites
     yield c
     c.close()
 
@@ -45,10 +46,14 @@ def check_all_arrays(ctx, circ, make_scalars, window=0):
             ctx.dev_free(d_sc)
 
 
-def test_tables_built_by_default(ctx):
-    circ = ctx.synth_circuit(41, 3000, 8)
-    cw, ch = ctx.circuit_window_tables(circ)
-    assert cw >= 11 and ch >= 11
+def test_tables_built_in_auto_mode(ctx):
+    try:
+        ctx.set_window_tables(-1)
+        circ = ctx.synth_circuit(41, 3000, 8)
+        cw, ch = ctx.circuit_window_tables(circ)
+        assert cw >= 11 and ch >= 11
+    finally:
+        ctx.set_window_tables(1)
 
 
 @pytest.mark.parametrize("m", [300, 5000])
@@ -151,4 +156,4 @@ def test_tables_mode_for_new_circuits(ctx):
         r, s = zk.synth_blinding(47)
         assert np.array_equal(ctx.prove(circ, w, r, s).words(), ctx.prove(circ1, w, r, s).words())
     finally:
-        ctx.set_window_tables(-1)
+        ctx.set_window_tables(1)

@@ -1,5 +1,6 @@
 #include "prover.hpp"
 
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -27,6 +28,12 @@ Context::Context(int dev) : device(dev) {
     ZK_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     ZK_HIP(hipStreamCreateWithFlags(&aux_stream, hipStreamNonBlocking));
     msm_noalloc_setup(dev);   // device address of the fixed MSM scratch, once per device
+    // default window-table mode for circuits loaded on this context
+    if (const char* e = getenv("ZK_WINDOW_TABLES")) {
+        const int m = atoi(e);
+        ZK_REQUIRE(m >= -1 && m <= 1, ZK_E_ARG, "ZK_WINDOW_TABLES must be -1, 0 or 1");
+        tables_mode = m;
+    }
 }
 
 Context::~Context() {
