@@ -121,7 +121,8 @@ int zk_circuit_window_tables(zk_ctx* ctx, zk_circuit* c, int mode, int windows[2
  * the splits are tuned.
  * windows (may be NULL) receives the (witness, h) windows now in effect. */
 int zk_autotune(zk_ctx* ctx, const zk_circuit* c, int force, int windows[2]);
-/* window a proof on ctx would use for an MSM of kind (0 witness, 1 h) and size n */
+/* window a proof on ctx would use for an MSM of kind (0 witness, 1 h) and size n
+ * over a circuit without window tables (with them: zk_circuit_window_tables) */
 int zk_tune_get(zk_ctx* ctx, int kind, uint64_t n, int* window);
 /* merge / write the tuning table as text ("<arch> <kind> <ceil log2 n> <c>"
  * per line); entries for other GPU architectures are skipped on load */
