@@ -358,8 +358,8 @@ def section_probes(args, zk, ctx, circ):
         "frac": round(gbs["forward"] / PEAK_HBM_GBS, 4), "traffic": int(HBM_BYTES_NTT_2P22["forward"]),
         "traffic_by_kind": {k: int(v) for k, v in HBM_BYTES_NTT_2P22.items()},
         "traffic_GBps": {k: round(HBM_BYTES_NTT_2P22[k] / (v / 1e3) / 1e9, 1) for k, v in ntt.items()},
-        "note": ("the passes wait on memory latency at 2 waves per SIMD (their LDS stage sets that), not on HBM "
-                 "bandwidth: the profiler's traffic runs at under a quarter of the HBM peak"),
+        "note": ("the passes are mostly instruction-bound (29-bit Fr products), the rest memory latency at 2 waves "
+                 "per SIMD, not HBM bandwidth: the profiler's traffic runs at about a quarter of the HBM peak"),
         "avg_launch_ms": r3(ntt["forward"]), "algorithmic_bytes": 64 * n,
         "ms": {k: r3(v) for k, v in ntt.items()},
         "GBps": {k: round(v, 1) for k, v in gbs.items()},
