@@ -175,7 +175,9 @@ int zk_prove_stats(zk_ctx* ctx, double* out, int n);
  * per process, src/main.cu:115-171) ----
  * A worker thread admits submitted requests into workspace slots while other
  * proofs run (continuous batching), creating or growing a slot only if free
- * HBM stays above reserve_bytes. Circuits must outlive their requests.
+ * HBM stays above reserve_bytes. Requests over a domain of 2^23 or more
+ * (m >= 2^22), each enough to fill the GPU, run one at a time; smaller ones
+ * may pass a waiting one. Circuits must outlive their requests.
  * zk_server_destroy finishes every submitted request first. Destroying the
  * context first does the same for each of its servers, after which their
  * handles only accept zk_server_destroy (other calls return ZK_E_STATE). */

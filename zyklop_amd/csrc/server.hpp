@@ -2,7 +2,8 @@
 //
 // Requests (circuit resident in HBM, witness, r, s) enter a queue. A worker
 // thread admits them into workspace slots as slots free up and as free HBM
-// allows (a slot is created or grown only if hipMemGetInfo leaves the
+// allows, at most one proof over a domain of 2^23 or more at a time (smaller
+// requests may pass a waiting one) (a slot is created or grown only if hipMemGetInfo leaves the
 // reserve intact), so requests that arrive while others run on the GPU are
 // overlapped with them. Completion is detected without blocking (stream
 // queries); the host tail of a finished proof runs while other slots keep the
