@@ -160,7 +160,10 @@ void circuit_set_tables(Circuit& c, int mode) {
         const size_t extra = (size_t)(Ww - 1) * ne * (3 * PT29_BYTES + PT29Q_BYTES) + (size_t)(Wh - 1) * n * PT29_BYTES;
         size_t free_b = 0, total_b = 0;
         ZK_HIP(hipMemGetInfo(&free_b, &total_b));
-        on = extra <= free_b / 3;
+        // tables the circuit holds now are freed before the new ones are made
+        const size_t held = c.A29.bytes + c.B1_29.bytes + c.C29.bytes + c.B2_29.bytes + c.H29.bytes;
+        const size_t base = ne * (3 * PT29_BYTES + PT29Q_BYTES) + n * PT29_BYTES;
+        on = extra <= (free_b + (held > base ? held - base : 0)) / 3;
     }
     // entries name a table slot in 30 bits
     const bool fits = (size_t)Ww * ne < 0x40000000ull && (size_t)Wh * n < 0x40000000ull;
