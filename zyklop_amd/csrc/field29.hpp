@@ -269,6 +269,24 @@ ZK_DEV F29 f29_neg(const F29& y) {
     return f29_norm(r);
 }
 
+// -y without carry propagation, as an operand of the products below: 2p - y
+// for y < p (value in (p, 2p]) or 4p - y for normalized y < 2p (value in
+// (2p, 4p]); every limb lies in (0, 2^30) (the constants borrow into
+// [2^29, 2^30) and the top limb of y stays below theirs), so a product
+// column that takes one such factor per term sums terms below 2^59
+ZK_DEV F29 f29_neg_lazy2(const F29& y) {
+    F29 r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.l[i] = f29c::P2[i] - y.l[i];
+    return r;
+}
+ZK_DEV F29 f29_neg_lazy4(const F29& y) {
+    F29 r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.l[i] = f29c::P4[i] - y.l[i];
+    return r;
+}
+
 // limb-wise sum (no carry propagation)
 ZK_DEV F29 f29_add(const F29& a, const F29& b) {
     F29 r;
@@ -323,14 +341,15 @@ struct Xyzz29 {
     bool inf;
 };
 
-// madd-2008-s, acc += (x2, y2) with x2 < p, y2 < 2p (normalized). The equal-x
+// madd-2008-s, acc += (x2, y2) with x2 < p and y2 < 2p, normalized or the
+// unnormalized negation of a value below p (f29_neg_lazy2). The equal-x
 // cases (doubling, or the sum is infinity) are rare: the doubling goes through
 // the standard-form formulas.
 ZK_DEV void f29_madd(Xyzz29& A, const F29& x2, const F29& y2) {
     using namespace f29c;
     if (A.inf) {
         A.X = x2;
-        A.Y = y2;
+        A.Y = f29_norm(y2);
         A.ZZ = f29_const(ONE);
         A.ZZZ = f29_const(ONE);
         A.inf = false;
@@ -364,10 +383,11 @@ ZK_DEV void f29_madd(Xyzz29& A, const F29& x2, const F29& y2) {
 #pragma unroll
     for (int i = 0; i < 9; i++) x3.l[i] = (P12[i] - PPP.l[i] - 2 * Q.l[i]) + RR.l[i];
     const F29 X3 = f29_reduce(f29_norm(x3));
-    // Y3 = R*(Q - X3) - Y1*PPP with one reduction: R, Q - X3 < 6p, Y1 and
-    // -PPP <= 2p, so the sum of products stays under the f29_mulsum bound
+    // Y3 = R*(Q - X3) - Y1*PPP with one reduction: R, Q - X3 < 6p, Y1 < 2p
+    // and -PPP <= 4p (f29_neg_lazy4): the products sum below 20p^2 and a
+    // column below 36 * 2^58
     const F29 T = f29_norm(f29_sub4(Q, X3));
-    const F29 ya[2] = {R_, A.Y}, yb[2] = {T, f29_neg(PPP)};
+    const F29 ya[2] = {R_, A.Y}, yb[2] = {T, f29_neg_lazy4(PPP)};
     A.Y = f29_mulsum<2>(ya, yb);
     A.X = X3;
     A.ZZ = f29_mul(A.ZZ, PP);
@@ -403,7 +423,7 @@ ZK_DEV Xyzz29 f29_xyzz_dbl(const Xyzz29& p) {
     for (int i = 0; i < 9; i++) x3.l[i] = (P8[i] - S.l[i] - S.l[i]) + MM.l[i];
     const F29 X3 = f29_reduce(f29_norm(x3));
     const F29 T = f29_norm(f29_sub4(S, X3));
-    const F29 ya[2] = {M, Wv}, yb[2] = {T, f29_neg(p.Y)};
+    const F29 ya[2] = {M, Wv}, yb[2] = {T, f29_neg_lazy4(p.Y)};
     Xyzz29 r;
     r.X = X3;
     r.Y = f29_mulsum<2>(ya, yb);
@@ -437,7 +457,7 @@ ZK_DEV Xyzz29 f29_xyzz_add(const Xyzz29& p, const Xyzz29& q) {
     for (int i = 0; i < 9; i++) x3.l[i] = (P12[i] - PPP.l[i] - 2 * Q.l[i]) + RR.l[i];
     const F29 X3 = f29_reduce(f29_norm(x3));
     const F29 T = f29_norm(f29_sub4(Q, X3));
-    const F29 ya[2] = {R_, S1}, yb[2] = {T, f29_neg(PPP)};
+    const F29 ya[2] = {R_, S1}, yb[2] = {T, f29_neg_lazy4(PPP)};
     Xyzz29 r;
     r.X = X3;
     r.Y = f29_mulsum<2>(ya, yb);
@@ -577,14 +597,15 @@ struct F29q {
     F29 c0, c1;
 };
 
-// a: components normalized with value < 6p; b: components N. Each component
-// is a sum of two products with one reduction (f29_mulsum): c0 = a0*b0 +
-// a1*(-b1), c1 = a0*b1 + a1*b0; result N. Four products and two reductions
+// a: components normalized with value < 6p, or lazy negations of values below
+// p (f29_neg_lazy2); b: components N. Each component is a sum of two products
+// with one reduction (f29_mulsum): c0 = a0*b0 + a1*(-b1), c1 = a0*b1 + a1*b0,
+// -b1 = 4p - b1 unnormalized (a column stays below 63 * 2^58); result N. Four products and two reductions
 // cost fewer instructions than Karatsuba's three products, three reductions
 // and the normalisations of its differences.
 ZK_DEV F29q f29q_mul(const F29q& a, const F29q& b) {
     const F29 x[2] = {a.c0, a.c1};
-    const F29 y0[2] = {b.c0, f29_neg(b.c1)}, y1[2] = {b.c1, b.c0};
+    const F29 y0[2] = {b.c0, f29_neg_lazy4(b.c1)}, y1[2] = {b.c1, b.c0};
     return {f29_mulsum<2>(x, y0), f29_mulsum<2>(x, y1)};
 }
 
@@ -611,9 +632,11 @@ ZK_DEV F29q f29q_reduce(const F29q& a) { return {f29_reduce(a.c0), f29_reduce(a.
 ZK_DEV bool f29q_sub_is_zero(const F29q& x) { return f29_sub4_is_zero(x.c0) && f29_sub4_is_zero(x.c1); }
 
 // T*A - B*C, each component one sum of four products with one reduction:
-// T normalized < 6p, A, B, C N (below 32p^2 in all: the f29_mulsum bound)
+// T normalized < 6p, A, B, C N; the negations are 4p - x unnormalized (two of
+// the four terms of a column carry one, so a column stays below 63 * 2^58),
+// and the products sum below 52p^2
 ZK_DEV F29q f29q_mul_sub_mul(const F29q& T, const F29q& A, const F29q& B, const F29q& C) {
-    const F29 nb0 = f29_neg(B.c0), nb1 = f29_neg(B.c1), na1 = f29_neg(A.c1);
+    const F29 nb0 = f29_neg_lazy4(B.c0), nb1 = f29_neg_lazy4(B.c1), na1 = f29_neg_lazy4(A.c1);
     const F29 a0[4] = {T.c0, T.c1, nb0, B.c1}, b0[4] = {A.c0, na1, C.c0, C.c1};
     const F29 a1[4] = {T.c0, T.c1, nb0, nb1}, b1[4] = {A.c1, A.c0, C.c1, C.c0};
     return {f29_mulsum<4>(a0, b0), f29_mulsum<4>(a1, b1)};
@@ -770,7 +793,8 @@ struct AccLds {
 };
 
 // madd-2008-s over Fp2 on that accumulator, acc += (x2, y2) with components
-// of x2 < p and y2 < 2p (normalized); inf tracks the empty/infinite sum. The
+// of x2 < p and y2 < 2p, normalized or f29_neg_lazy2 of values below p; inf
+// tracks the empty/infinite sum. The
 // equal-x cases go through the standard-form formulas.
 template <int NT>
 ZK_DEV void f29q_madd(AccLds<NT>& A, bool& inf, const F29q& x2, const F29q& y2) {
@@ -779,7 +803,7 @@ ZK_DEV void f29q_madd(AccLds<NT>& A, bool& inf, const F29q& x2, const F29q& y2) 
     if (inf) {
         const F29q one = {f29_const(ONE), F29{}};
         A.st(L::X, x2);
-        A.st(L::Y, y2);
+        A.st(L::Y, F29q{f29_norm(y2.c0), f29_norm(y2.c1)});
         A.st(L::ZZ, one);
         A.st(L::ZZZ, one);
         inf = false;
@@ -824,11 +848,13 @@ ZK_DEV void f29q_madd(AccLds<NT>& A, bool& inf, const F29q& x2, const F29q& y2) 
     }
     A.st(L::X, X3);
     // Y3 = (Q - X3)*R - Y1*PPP, each Fp2 component as one sum of four
-    // products with one reduction: T = Q - X3 < 6p, R, Y1, PPP and their
-    // negations <= 2p, so the products sum below 32p^2 (f29_mulsum bound)
+    // products with one reduction: T = Q - X3 < 6p, R, Y1, PPP < 2p and the
+    // negations 4p - x unnormalized (f29_neg_lazy4), so the products sum
+    // below 48p^2 and a column, two of whose four terms carry a negation,
+    // below 63 * 2^58
     const F29q T = f29q_sub_nr(Q, X3);
     const F29q Y1 = A.ld(L::Y);
-    const F29 nr1 = f29_neg(R_.c1), ny0 = f29_neg(Y1.c0), ny1 = f29_neg(Y1.c1);
+    const F29 nr1 = f29_neg_lazy4(R_.c1), ny0 = f29_neg_lazy4(Y1.c0), ny1 = f29_neg_lazy4(Y1.c1);
     const F29 a0[4] = {T.c0, T.c1, ny0, Y1.c1}, b0[4] = {R_.c0, nr1, PPP.c0, PPP.c1};
     const F29 a1[4] = {T.c0, T.c1, ny0, ny1}, b1[4] = {R_.c1, R_.c0, PPP.c1, PPP.c0};
     A.st(L::Y, F29q{f29_mulsum<4>(a0, b0), f29_mulsum<4>(a1, b1)});

@@ -21,7 +21,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         F29 x, y;
         f29_load_point(points29, ref, x, y);
         if (f29_is_zero(x) && f29_is_zero(y)) continue;   // infinity
-        if (ref >> 31) y = f29_neg(y);
+        if (ref >> 31) y = f29_neg_lazy2(y);   // f29_madd takes it unnormalized
         f29_madd(acc, x, y);
     }
     red_store<Fp>(buckets, slot, acc);

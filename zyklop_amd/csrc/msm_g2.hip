@@ -29,7 +29,7 @@ __global__ void __launch_bounds__(ACC29Q_THREADS) __attribute__((amdgpu_waves_pe
         F29q x, y;
         f29q_load_point(points29, ref, x, y);
         if (f29_is_zero(x.c0) && f29_is_zero(x.c1) && f29_is_zero(y.c0) && f29_is_zero(y.c1)) continue;
-        if (ref >> 31) y = {f29_neg(y.c0), f29_neg(y.c1)};
+        if (ref >> 31) y = {f29_neg_lazy2(y.c0), f29_neg_lazy2(y.c1)};   // f29q_madd takes it unnormalized
         f29q_madd(A, inf, x, y);
     }
     using L = AccLds<ACC29Q_THREADS>;
