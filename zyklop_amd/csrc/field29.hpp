@@ -383,10 +383,12 @@ ZK_DEV void f29_madd(Xyzz29& A, const F29& x2, const F29& y2) {
 #pragma unroll
     for (int i = 0; i < 9; i++) x3.l[i] = (P12[i] - PPP.l[i] - 2 * Q.l[i]) + RR.l[i];
     const F29 X3 = f29_reduce(f29_norm(x3));
-    // Y3 = R*(Q - X3) - Y1*PPP with one reduction: R, Q - X3 < 6p, Y1 < 2p
-    // and -PPP <= 4p (f29_neg_lazy4): the products sum below 20p^2 and a
-    // column below 36 * 2^58
-    const F29 T = f29_norm(f29_sub4(Q, X3));
+    // Y3 = R*(Q - X3) - Y1*PPP with one reduction: R and T = Q - X3 below 6p,
+    // Y1 below 2p and -PPP <= 4p (f29_neg_lazy4), so the products sum below
+    // 44p^2. T stays unnormalized (limbs below 2^31: X3 < 2p keeps every limb
+    // of Q + 4p - X3 non-negative), so a column sums 9 terms below 2^60, 9
+    // below 2^59 and 9 below 2^58: under 63 * 2^58.
+    const F29 T = f29_sub4(Q, X3);
     const F29 ya[2] = {R_, A.Y}, yb[2] = {T, f29_neg_lazy4(PPP)};
     A.Y = f29_mulsum<2>(ya, yb);
     A.X = X3;
@@ -406,8 +408,8 @@ ZK_DEV Xyzz29 xyzz29_inf() {
 // M = 3X^2, X3 = M^2 - 2S, Y3 = M*(S - X3) - W*Y, ZZ3 = V*ZZ, ZZZ3 = W*ZZZ.
 // Bounds: U normalized < 4p, M normalized < 6p (both fine for f29_sqr /
 // f29_mul); M^2 + 8p - 2S lies in (4p, 10p) with every limb of 8p - 2S
-// positive; Y3 sums M*(S - X3) < 36p^2 and W*(-Y) <= 4p^2 (f29_mulsum<2>
-// bound). BN254 G1 has no point of order 2, so Y = 0 only at infinity.
+// positive; Y3 sums M*(S - X3) < 36p^2 and W*(4p - Y) < 8p^2 (f29_mulsum<2>
+// bound, columns as in f29_madd). BN254 G1 has no point of order 2, so Y = 0 only at infinity.
 ZK_DEV Xyzz29 f29_xyzz_dbl(const Xyzz29& p) {
     using namespace f29c;
     if (p.inf) return p;
@@ -422,7 +424,7 @@ ZK_DEV Xyzz29 f29_xyzz_dbl(const Xyzz29& p) {
 #pragma unroll
     for (int i = 0; i < 9; i++) x3.l[i] = (P8[i] - S.l[i] - S.l[i]) + MM.l[i];
     const F29 X3 = f29_reduce(f29_norm(x3));
-    const F29 T = f29_norm(f29_sub4(S, X3));
+    const F29 T = f29_sub4(S, X3);   // unnormalized, as in f29_madd
     const F29 ya[2] = {M, Wv}, yb[2] = {T, f29_neg_lazy4(p.Y)};
     Xyzz29 r;
     r.X = X3;
@@ -456,7 +458,7 @@ ZK_DEV Xyzz29 f29_xyzz_add(const Xyzz29& p, const Xyzz29& q) {
 #pragma unroll
     for (int i = 0; i < 9; i++) x3.l[i] = (P12[i] - PPP.l[i] - 2 * Q.l[i]) + RR.l[i];
     const F29 X3 = f29_reduce(f29_norm(x3));
-    const F29 T = f29_norm(f29_sub4(Q, X3));
+    const F29 T = f29_sub4(Q, X3);   // unnormalized, as in f29_madd
     const F29 ya[2] = {R_, S1}, yb[2] = {T, f29_neg_lazy4(PPP)};
     Xyzz29 r;
     r.X = X3;
