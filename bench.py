@@ -52,8 +52,8 @@ CALIB_BYTES = 384 << 20                 # device copy before the probes (profile
 # calibration copy; profiles/probe_kernels_hbm_counters.md
 # G1 / G2 accumulation over the window tables of the 2^20 circuit (7 windows
 # of 18-19 bits over the GLV halves sharing one bucket set), 14,737,409 entries:
-HBM_BYTES_PER_ENTRY_G1 = (2.7032e9 + 0.1070e9) / 14737409
-HBM_BYTES_PER_ENTRY_G2 = (2.3905e9 + 0.0671e9) / 14737409
+HBM_BYTES_PER_ENTRY_G1 = (2.7586e9 + 0.1070e9) / 14737409
+HBM_BYTES_PER_ENTRY_G2 = (2.3723e9 + 0.0671e9) / 14737409
 ALGO_BYTES_PER_ENTRY_G1 = 68            # y and x (or beta*x) of the base, 32 B each packed, + the 4 B entry
 ALGO_BYTES_PER_ENTRY_G2 = 132           # the same over Fp2, + the 4 B entry
 # NTT of 2^22, per transform (three pass pairs + the last pass)
