@@ -71,10 +71,16 @@ int msm_table_c(size_t n) {
 // ---------------------------------------------------------------- GLV split
 // Every scalar k_i becomes two half scalars (glv.hpp): half[i] = k1 for P_i and
 // half[n + i] = k2 for lambda*P_i, each a 127-bit magnitude with its sign in
-// bit 127 (4 x u32 LE).
-__global__ void k_glv_split(const uint32_t* __restrict__ sc, size_t n, uint4* __restrict__ half) {
+// bit 127 (4 x u32 LE). A scalar whose skip byte is set splits into two
+// zero halves (no digits, so no entries).
+__global__ void k_glv_split(const uint32_t* __restrict__ sc, size_t n, const uint8_t* __restrict__ skip,
+                            uint4* __restrict__ half) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (skip && skip[i]) {
+        half[i] = half[n + i] = make_uint4(0, 0, 0, 0);
+        return;
+    }
     const uint4* p = reinterpret_cast<const uint4*>(sc) + 2 * i;
     const uint4 a = p[0], b = p[1];
     const uint64_t k[4] = {a.x | (uint64_t)a.y << 32, a.z | (uint64_t)a.w << 32, b.x | (uint64_t)b.y << 32,
@@ -386,7 +392,7 @@ size_t MsmSort::count_reserve(size_t n_, int c, GrowCount& gc, bool merged) cons
     return z.max_chunks;
 }
 
-void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st, bool merged) {
+void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st, bool merged, const uint8_t* skip) {
     const SortShape z = sort_shape(n_, c, merged);
     sort_reserve(*this, z);
     n = n_;
@@ -399,7 +405,7 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st, bool 
     const uint32_t G = z.G, nt = z.nt;
     const int sh = z.sh;
     if (n) {
-        k_glv_split<<<cdiv(n, 256), 256, 0, st>>>((const uint32_t*)d_scalars, n, half.as<uint4>());
+        k_glv_split<<<cdiv(n, 256), 256, 0, st>>>((const uint32_t*)d_scalars, n, skip, half.as<uint4>());
         ZK_HIP(hipGetLastError());
     }
     const uint32_t* hs = half.as<uint32_t>();

@@ -81,8 +81,10 @@ struct MsmSort {
     // the device bytes reserve(n, c, merged) would add now (into gc); same return value
     size_t count_reserve(size_t n, int c, GrowCount& gc, bool merged = false) const;
     // sorts canonical scalars (n x 32 B, device, n < 2^30; W*n < 2^30 merged)
-    // into bucket order, entries as above
-    void run(const void* d_scalars, size_t n, int c, hipStream_t st, bool merged = false);
+    // into bucket order, entries as above; scalar i with skip[i] != 0 (skip:
+    // n device bytes, optional) is taken as zero, so it gets no entries
+    void run(const void* d_scalars, size_t n, int c, hipStream_t st, bool merged = false,
+             const uint8_t* skip = nullptr);
 };
 
 // Result of the device part of one MSM: level-2 partials per window, on the host.

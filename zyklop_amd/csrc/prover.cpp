@@ -102,7 +102,8 @@ void Context::ensure_comb() {
 // ------------------------------------------------------------------ circuits
 size_t Circuit::device_bytes() const {
     return rowA.bytes + colA.bytes + valA.bytes + rowB.bytes + colB.bytes + valB.bytes + A.bytes + B1.bytes + B2.bytes +
-           C.bytes + H.bytes + A29.bytes + B1_29.bytes + B2_29.bytes + C29.bytes + H29.bytes + wit_rows.bytes;
+           C.bytes + H.bytes + A29.bytes + B1_29.bytes + B2_29.bytes + C29.bytes + H29.bytes + wit_rows.bytes +
+           b_skip.bytes;
 }
 
 static void upload(DevBuf& d, const void* src, size_t bytes) {
@@ -135,6 +136,31 @@ static void build_ext(DevBuf& d, const uint8_t* lead_zeros_then, size_t lead_zer
     upload(d, h.data(), h.size());
 }
 
+static bool words_zero(const uint8_t* p, size_t bytes) {
+    uint64_t v[16];
+    memcpy(v, p, bytes);
+    uint64_t o = 0;
+    for (size_t k = 0; k < bytes / 8; k++) o |= v[k];
+    return o == 0;
+}
+
+// Circuit::b_skip from the extended B1'/B2' arrays (bases, then the four extra
+// points, a null extra being infinity): set where both points are infinite
+static void build_b_skip(Circuit& c, const uint8_t* B1, const uint8_t* B2, const uint64_t* const exB1[4],
+                         const uint64_t* const exB2[4]) {
+    const size_t nv = c.n_vars, ne = nv + 4;
+    std::vector<uint8_t> sk(ne);
+    size_t cnt = 0;
+    for (size_t i = 0; i < ne; i++) {
+        const uint8_t* p1 = i < nv ? B1 + 64 * i : reinterpret_cast<const uint8_t*>(exB1[i - nv]);
+        const uint8_t* p2 = i < nv ? B2 + 128 * i : reinterpret_cast<const uint8_t*>(exB2[i - nv]);
+        sk[i] = (!p1 || words_zero(p1, 64)) && (!p2 || words_zero(p2, 128));
+        cnt += sk[i];
+    }
+    if (cnt * 8 >= ne) upload(c.b_skip, sk.data(), ne);
+    else c.b_skip.release();
+}
+
 static void finish_circuit_bases(Circuit& c, const uint8_t* A, const uint8_t* B1, const uint8_t* B2, const uint8_t* C,
                                  const uint8_t* H) {
     const ZkeyHeader& h = c.hdr;
@@ -146,6 +172,7 @@ static void finish_circuit_bases(Circuit& c, const uint8_t* A, const uint8_t* B1
     build_ext(c.A, nullptr, 0, A, nv, exA, 64);
     build_ext(c.B1, nullptr, 0, B1, nv, exB1, 64);
     build_ext(c.B2, nullptr, 0, B2, nv, exB2, 128);
+    build_b_skip(c, B1, B2, exB1, exB2);
     build_ext(c.C, nullptr, l + 1, C, nv - l - 1, exC, 64);
     upload(c.H, H, (size_t)c.n * 64);
     circuit_set_tables(c, c.ctx->tables_mode);
@@ -320,6 +347,7 @@ ProveSlot::ProveSlot() {
     for (hipStream_t* p : {&s0, &s1, &s2, &s3, &s4}) ZK_HIP(hipStreamCreateWithFlags(p, hipStreamNonBlocking));
     ZK_HIP(hipEventCreateWithFlags(&ev_scal, hipEventDisableTiming));
     ZK_HIP(hipEventCreateWithFlags(&ev_done1, hipEventDisableTiming));
+    ZK_HIP(hipEventCreateWithFlags(&ev_doneB, hipEventDisableTiming));
     for (auto& e : ev_join) ZK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     ZK_HIP(hipEventCreateWithFlags(&ev_h, hipEventDisableTiming));
 }
@@ -338,6 +366,7 @@ ProveSlot::~ProveSlot() {
         if (p) { (void)hipStreamSynchronize(p); (void)hipStreamDestroy(p); }
     if (ev_scal) (void)hipEventDestroy(ev_scal);
     if (ev_done1) (void)hipEventDestroy(ev_done1);
+    if (ev_doneB) (void)hipEventDestroy(ev_doneB);
     if (ev_h) (void)hipEventDestroy(ev_h);
     for (auto& e : ev_join)
         if (e) (void)hipEventDestroy(e);
@@ -345,7 +374,7 @@ ProveSlot::~ProveSlot() {
 
 size_t ProveSlot::device_bytes() const {
     size_t b = scal.bytes + aw.bytes + bw.bytes + cw.bytes + t0.bytes + t1.bytes + t2.bytes + h.bytes;
-    for (const MsmSort* q : {&sortW, &sortH})
+    for (const MsmSort* q : {&sortW, &sortH, &sortB})
         b += q->half.bytes + q->matrix.bytes + q->mscan.bytes + q->pairs.bytes + q->offsets.bytes + q->entries.bytes +
              q->order.bytes + q->lkeys.bytes + q->chunks.bytes + q->large.bytes + q->counters.bytes;
     for (const MsmScratch* s : {&scrA, &scrB1, &scrC, &scrB2, &scrH})
@@ -377,7 +406,9 @@ void ProveSlot::reserve(Context& ctx, const Circuit& c, bool host_witness) {
     prepare(ctx, c);
     if (host_witness) staging.ensure((size_t)c.n_vars * 32);
     const MsmPlan pl = ctx.msm_plan(c);
-    const size_t kw = sortW.reserve(nS, pl.cw, pl.mw), kh = sortH.reserve(n, pl.ch, pl.mh);
+    size_t kw = sortW.reserve(nS, pl.cw, pl.mw);
+    const size_t kh = sortH.reserve(n, pl.ch, pl.mh);
+    if (c.b_skip.bytes) kw = std::max(kw, sortB.reserve(nS, pl.cw, pl.mw));
     const MsmGeom gw1 = msm_geom(nS, pl.cw, false, pl.mw), gw2 = msm_geom(nS, pl.cw, true, pl.mw),
                   gh = msm_geom(n, pl.ch, false, pl.mh);
     for (MsmScratch* q : {&scrA, &scrB1, &scrC}) q->ensure(gw1, kw, false, pl.sp_w1);
@@ -433,22 +464,39 @@ void ProveSlot::body(Context& ctx, const Circuit& c, ProofParts& out, bool alone
     //    pipeline, the proof's longest chain, behind it (plain proofs
     //    13.8 -> 13.4 ms; in batches the other slots fill that gap and the
     //    extra MSM stream does better).
+    //
+    // A circuit whose B matrix leaves wires out (b_skip) gets a second sort
+    // of the same scalars with those wires left out, on s4 ahead of B2: B1
+    // and B2 then never read an infinite base. An infinite base in the shared
+    // entries costs its lane a whole mixed addition, since the other lanes of
+    // the wave are adding; a quarter of the wires of the synthetic circuits
+    // are such (row i draws its wires uniformly from the first ones).
     alone_layout = alone;
     hipStream_t s_sort = alone ? s2 : s1, s_a = alone ? s2 : s1;
+    const bool sep_b = c.b_skip.bytes != 0;
+    const MsmSort& sB = sep_b ? sortB : sortW;
+    if (sep_b) {
+        ZK_HIP(hipStreamWaitEvent(s4, ev_scal, 0));
+        sortB.run(scal.p, nS, window_w, s4, pl.mw, c.b_skip.as<uint8_t>());
+        ZK_HIP(hipEventRecord(ev_doneB, s4));
+    }
     ZK_HIP(hipStreamWaitEvent(s_sort, ev_scal, 0));
     if (timed) ZK_HIP(hipEventRecord(ev[4], s_sort));
     sortW.run(scal.p, nS, window_w, s_sort, pl.mw);
     ZK_HIP(hipEventRecord(ev_done1, s_sort));
     if (timed) ZK_HIP(hipEventRecord(ev[5], s_sort));
-    for (hipStream_t p : {s1, s2, s3, s4})
+    for (hipStream_t p : {s1, s2, s3, s4}) {
+        if (sep_b && (p == s2 || p == s4)) continue;   // B1 and B2 read the B sort
         if (p != s_sort) ZK_HIP(hipStreamWaitEvent(p, ev_done1, 0));
-    msm_launch_pre(sortW, c.B2_29.p, true, scrB2, s4, sp_w2);
-    msm_launch_pre(sortW, c.B1_29.p, false, scrB1, s2, sp_w1);
+    }
+    if (sep_b) ZK_HIP(hipStreamWaitEvent(s2, ev_doneB, 0));
+    msm_launch_pre(sB, c.B2_29.p, true, scrB2, s4, sp_w2);
+    msm_launch_pre(sB, c.B1_29.p, false, scrB1, s2, sp_w1);
     msm_launch_pre(sortW, c.C29.p, false, scrC, s3, sp_w1);
     msm_launch_pre(sortW, c.A29.p, false, scrA, s_a, sp_w1);
-    msm_launch_main(sortW, true, scrB2, out.b2, s4, c.B2_29.p);
+    msm_launch_main(sB, true, scrB2, out.b2, s4, c.B2_29.p);
     if (!alone) msm_launch_main(sortW, false, scrA, out.a, s_a, c.A29.p);
-    msm_launch_main(sortW, false, scrB1, out.b1, s2, c.B1_29.p);
+    msm_launch_main(sB, false, scrB1, out.b1, s2, c.B1_29.p);
     msm_launch_main(sortW, false, scrC, out.c, s3, c.C29.p);
     if (alone) msm_launch_main(sortW, false, scrA, out.a, s_a, c.A29.p);
     // H pipeline on s0
@@ -503,7 +551,8 @@ size_t ProveSlot::count_bytes(const Context& ctx, const Circuit& c, const ProveS
         g.bytes += 7 * n * 32;
     }
     const MsmPlan pl = ctx.msm_plan(c);
-    const size_t kw = (sl ? sl->sortW : no_sort).count_reserve(nS, pl.cw, g, pl.mw);
+    size_t kw = (sl ? sl->sortW : no_sort).count_reserve(nS, pl.cw, g, pl.mw);
+    if (c.b_skip.bytes) kw = std::max(kw, (sl ? sl->sortB : no_sort).count_reserve(nS, pl.cw, g, pl.mw));
     const size_t kh = (sl ? sl->sortH : no_sort).count_reserve(n, pl.ch, g, pl.mh);
     const MsmGeom gw1 = msm_geom(nS, pl.cw, false, pl.mw), gw2 = msm_geom(nS, pl.cw, true, pl.mw),
                   gh = msm_geom(n, pl.ch, false, pl.mh);
