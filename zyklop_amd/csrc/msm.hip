@@ -71,16 +71,10 @@ int msm_table_c(size_t n) {
 // ---------------------------------------------------------------- GLV split
 // Every scalar k_i becomes two half scalars (glv.hpp): half[i] = k1 for P_i and
 // half[n + i] = k2 for lambda*P_i, each a 127-bit magnitude with its sign in
-// bit 127 (4 x u32 LE). A scalar whose skip byte is set splits into two
-// zero halves (no digits, so no entries).
-__global__ void k_glv_split(const uint32_t* __restrict__ sc, size_t n, const uint8_t* __restrict__ skip,
-                            uint4* __restrict__ half) {
+// bit 127 (4 x u32 LE).
+__global__ void k_glv_split(const uint32_t* __restrict__ sc, size_t n, uint4* __restrict__ half) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (skip && skip[i]) {
-        half[i] = half[n + i] = make_uint4(0, 0, 0, 0);
-        return;
-    }
     const uint4* p = reinterpret_cast<const uint4*>(sc) + 2 * i;
     const uint4 a = p[0], b = p[1];
     const uint64_t k[4] = {a.x | (uint64_t)a.y << 32, a.z | (uint64_t)a.w << 32, b.x | (uint64_t)b.y << 32,
@@ -166,7 +160,7 @@ __global__ void __launch_bounds__(1024) k_sort_tile_hist(const uint32_t* __restr
 __global__ void __launch_bounds__(1024) k_sort_tile_scatter(const uint32_t* __restrict__ sc, size_t n, size_t tile, int c,
                                                             int nlo, int W, uint32_t Kw, uint32_t sw, int s, uint32_t G,
                                                             uint32_t nt, const uint32_t* __restrict__ mscan,
-                                                            uint2* __restrict__ pairs) {
+                                                            const uint8_t* __restrict__ cls, uint2* __restrict__ pairs) {
     extern __shared__ uint32_t lds[];
     uint32_t* base = lds;       // this tile's run start in each group
     uint32_t* rank = lds + G;   // next rank in each group
@@ -182,13 +176,14 @@ __global__ void __launch_bounds__(1024) k_sort_tile_scatter(const uint32_t* __re
         it.init(sc, i, c, nlo);
         const bool phi = (uint32_t)i >= nh;
         const uint32_t ref0 = ((uint32_t)i - (phi ? nh : 0u)) | (phi ? 0x40000000u : 0u);
+        const uint32_t ctag = cls ? (uint32_t)cls[(uint32_t)i - (phi ? nh : 0u)] << 30 : 0u;
         for (int w = 0; w < W; w++) {
             bool ng;
             uint32_t mag = it.next(ng);
             if (mag) {
                 const uint32_t b = (uint32_t)w * Kw + mag - 1, g = b >> s;
                 const uint32_t pos = base[g] + atomicAdd(&rank[g], 1u);
-                pairs[pos] = make_uint2((ref0 + (uint32_t)w * sw) | (ng ? 0x80000000u : 0u), b);
+                pairs[pos] = make_uint2((ref0 + (uint32_t)w * sw) | (ng ? 0x80000000u : 0u), b | ctag);
             }
         }
     }
@@ -205,9 +200,15 @@ constexpr uint32_t SORT_GROUP_LDS_MAX = 160 * 1024 - 1024;
 // 16 waves per group: the LDS stage often leaves room for one workgroup per
 // CU, whose waves then carry the pass's LDS atomics and loads alone
 constexpr int SORT_GROUP_THREADS = 1024;
+//
+// With scalar classes (cls_off non-null; the class is in bits 30..31 of a
+// pair's bucket word) the entries go out class by class: one pass over the
+// group's pairs per class, each bucket's cursor then standing where the next
+// class starts, which cls_off records.
 __global__ void __launch_bounds__(1024) k_sort_group(const uint2* __restrict__ pairs, const uint32_t* __restrict__ mscan,
                                                     uint32_t nt, int s, uint32_t nb, uint32_t stage_cap,
-                                                    uint32_t* __restrict__ offsets, uint32_t* __restrict__ entries) {
+                                                    uint32_t* __restrict__ offsets, uint32_t* __restrict__ entries,
+                                                    uint32_t* __restrict__ cls_off) {
     extern __shared__ uint32_t cnt[];   // 2^s counters, then the stage
     __shared__ uint32_t wsum[16];
     const uint32_t g = blockIdx.x, F = 1u << s, mask = F - 1;
@@ -235,52 +236,75 @@ __global__ void __launch_bounds__(1024) k_sort_group(const uint2* __restrict__ p
     }
     if (g == gridDim.x - 1 && threadIdx.x == 0) offsets[nb] = end;
     __syncthreads();
-    if (end - start <= stage_cap) {
-        uint32_t* stage = cnt + F;
+    const bool staged = end - start <= stage_cap;
+    uint32_t* stage = cnt + F;
+    const int ncls = cls_off ? 4 : 1;
+    for (int k = 0; k < ncls; k++) {
+        if (k) {
+            // every entry of classes < k is placed: cnt[f] is where class k starts
+            __syncthreads();
+            for (uint32_t f = threadIdx.x; f < F; f += blockDim.x) {
+                const uint32_t b = g * F + f;
+                if (b < nb) cls_off[3 * (size_t)b + (k - 1)] = cnt[f];
+            }
+            __syncthreads();
+        }
         for (uint32_t e = start + threadIdx.x; e < end; e += blockDim.x) {
             const uint2 p = pairs[e];
-            stage[atomicAdd(&cnt[p.y & mask], 1u) - start] = p.x;
+            if (cls_off && (int)(p.y >> 30) != k) continue;
+            const uint32_t pos = atomicAdd(&cnt[p.y & mask], 1u);
+            if (staged) stage[pos - start] = p.x;
+            else entries[pos] = p.x;
         }
+    }
+    if (staged) {
         __syncthreads();
         for (uint32_t e = threadIdx.x; e < end - start; e += blockDim.x) entries[start + e] = stage[e];
-    } else {
-        for (uint32_t e = start + threadIdx.x; e < end; e += blockDim.x) {
-            const uint2 p = pairs[e];
-            entries[atomicAdd(&cnt[p.y & mask], 1u)] = p.x;
-        }
     }
 }
 
-// accumulation-order key of bucket b: 0 = oversized, then descending load,
-// nkey-1 = empty
-__device__ inline uint32_t load_key(const uint32_t* __restrict__ off, uint32_t b, uint32_t lmax, int qs, uint32_t nkey) {
-    uint32_t len = off[b + 1] - off[b];
-    return len > lmax ? 0u : (nkey - 1) - (len >> qs);
+// accumulation-order key of bucket b: 0 = oversized (by the whole bucket),
+// then descending length of the part the MSM reads (MsmRange over a sort with
+// classes: the lanes of a wave get parts of nearly equal length), nkey-1 =
+// empty
+__device__ inline uint32_t load_key(const uint32_t* __restrict__ off, const uint32_t* __restrict__ cls_off, int range,
+                                    uint32_t b, uint32_t lmax, int qs, uint32_t nkey) {
+    uint32_t s0 = off[b], e0 = off[b + 1];
+    if (e0 - s0 > lmax) return 0u;
+    if (range == MSM_RANGE_A) {
+        s0 = cls_off[3 * (size_t)b];
+        e0 = cls_off[3 * (size_t)b + 2];
+    } else if (range == MSM_RANGE_B) {
+        e0 = cls_off[3 * (size_t)b + 1];
+    }
+    return (nkey - 1) - ((e0 - s0) >> qs);
 }
 
-__global__ void __launch_bounds__(1024) k_load_hist(const uint32_t* __restrict__ off, uint32_t nb, uint32_t lmax, int qs,
-                                                    uint32_t nkey, uint32_t* __restrict__ hist) {
+__global__ void __launch_bounds__(1024) k_load_hist(const uint32_t* __restrict__ off, const uint32_t* __restrict__ cls_off,
+                                                    int range, uint32_t nb, uint32_t lmax, int qs, uint32_t nkey,
+                                                    uint32_t* __restrict__ hist) {
     extern __shared__ uint32_t lh[];
     for (uint32_t k = threadIdx.x; k < nkey; k += blockDim.x) lh[k] = 0;
     __syncthreads();
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < nb) atomicAdd(&lh[load_key(off, b, lmax, qs, nkey)], 1u);
+    if (b < nb) atomicAdd(&lh[load_key(off, cls_off, range, b, lmax, qs, nkey)], 1u);
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < nkey; k += blockDim.x)
         if (lh[k]) atomicAdd(&hist[k], lh[k]);
 }
 
 // each block reserves one range per key, then ranks its buckets inside it in LDS
-__global__ void __launch_bounds__(1024) k_load_scatter(const uint32_t* __restrict__ off, uint32_t nb, uint32_t lmax, int qs,
-                                                       uint32_t nkey, uint32_t* __restrict__ cursor,
-                                                       uint32_t* __restrict__ order) {
+__global__ void __launch_bounds__(1024) k_load_scatter(const uint32_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ cls_off, int range, uint32_t nb,
+                                                       uint32_t lmax, int qs, uint32_t nkey,
+                                                       uint32_t* __restrict__ cursor, uint32_t* __restrict__ order) {
     extern __shared__ uint32_t lh[];
     for (uint32_t k = threadIdx.x; k < nkey; k += blockDim.x) lh[k] = 0;
     __syncthreads();
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t key = 0, rank = 0;
     if (b < nb) {
-        key = load_key(off, b, lmax, qs, nkey);
+        key = load_key(off, cls_off, range, b, lmax, qs, nkey);
         rank = atomicAdd(&lh[key], 1u);
     }
     __syncthreads();
@@ -376,26 +400,38 @@ static void sort_buffers(S& s, const SortShape& z, F&& f) {
     f(s.counters, (size_t)64);
 }
 
+// the buffers of a sort with scalar classes over nb buckets
+template <class S, class F>
+static void class_buffers(S& s, size_t nb, F&& f) {
+    f(s.cls_off, nb * 12);
+    f(s.order_a, nb * 4);
+    f(s.order_b, nb * 4);
+}
+
 static void sort_reserve(MsmSort& s, const SortShape& z) {
     sort_buffers(s, z, [](DevBuf& b, size_t n) { b.ensure(n); });
 }
 
-size_t MsmSort::reserve(size_t n_, int c, bool merged) {
+size_t MsmSort::reserve(size_t n_, int c, bool merged, bool with_classes) {
     const SortShape z = sort_shape(n_, c, merged);
     sort_reserve(*this, z);
+    if (with_classes) class_buffers(*this, z.nb, [](DevBuf& b, size_t n) { b.ensure(n); });
     return z.max_chunks;
 }
 
-size_t MsmSort::count_reserve(size_t n_, int c, GrowCount& gc, bool merged) const {
+size_t MsmSort::count_reserve(size_t n_, int c, GrowCount& gc, bool merged, bool with_classes) const {
     const SortShape z = sort_shape(n_, c, merged);
     sort_buffers(*this, z, [&](const DevBuf& b, size_t n) { gc.need(b, n); });
+    if (with_classes) class_buffers(*this, z.nb, [&](const DevBuf& b, size_t n) { gc.need(b, n); });
     return z.max_chunks;
 }
 
-void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st, bool merged, const uint8_t* skip) {
+void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st, bool merged, const uint8_t* cls) {
     const SortShape z = sort_shape(n_, c, merged);
     sort_reserve(*this, z);
+    if (cls) class_buffers(*this, z.nb, [](DevBuf& b, size_t n) { b.ensure(n); });
     n = n_;
+    classes = cls != nullptr && n_ != 0;
     g = z.g;
     lmax = z.lmax;
     qs = z.qs;
@@ -405,7 +441,7 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st, bool 
     const uint32_t G = z.G, nt = z.nt;
     const int sh = z.sh;
     if (n) {
-        k_glv_split<<<cdiv(n, 256), 256, 0, st>>>((const uint32_t*)d_scalars, n, skip, half.as<uint4>());
+        k_glv_split<<<cdiv(n, 256), 256, 0, st>>>((const uint32_t*)d_scalars, n, half.as<uint4>());
         ZK_HIP(hipGetLastError());
     }
     const uint32_t* hs = half.as<uint32_t>();
@@ -421,7 +457,8 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st, bool 
     exclusive_scan(mat, msc, msc + mlen, mlen, st);
     if (n) {
         k_sort_tile_scatter<<<nt, 1024, 2 * G * 4, st>>>(hs, n2, tile, g.c, g.nlo, g.W, merged ? 0u : g.K,
-                                                          merged ? (uint32_t)n : 0u, sh, G, nt, msc, pairs.as<uint2>());
+                                                          merged ? (uint32_t)n : 0u, sh, G, nt, msc, cls,
+                                                          pairs.as<uint2>());
         ZK_HIP(hipGetLastError());
     }
     // stage for the heaviest group uniform scalars give, plus a margin
@@ -440,18 +477,29 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st, bool 
     }
     const uint32_t stage_cap = (uint32_t)std::min<size_t>((lds_budget - (4u << sh)) / 4, want);
     k_sort_group<<<G, SORT_GROUP_THREADS, 4u * ((1u << sh) + stage_cap), st>>>(pairs.as<uint2>(), msc, nt, sh, (uint32_t)nb, stage_cap,
-                                                                offsets.as<uint32_t>(), entries.as<uint32_t>());
+                                                                offsets.as<uint32_t>(), entries.as<uint32_t>(),
+                                                                cls ? cls_off.as<uint32_t>() : nullptr);
     ZK_HIP(hipGetLastError());
     const uint32_t* off = offsets.as<uint32_t>();
     uint32_t* hist = lkeys.as<uint32_t>();
     uint32_t* cur = hist + nkey;
-    fill_u32(hist, 0u, nkey, st);
     const unsigned nblk_o = (unsigned)cdiv(nb, 1024);
-    k_load_hist<<<nblk_o, 1024, nkey * 4, st>>>(off, (uint32_t)nb, lmax, qs, nkey, hist);
-    ZK_HIP(hipGetLastError());
-    exclusive_scan(hist, cur, cur + nkey, nkey, st);
-    k_load_scatter<<<nblk_o, 1024, nkey * 4, st>>>(off, (uint32_t)nb, lmax, qs, nkey, cur, order.as<uint32_t>());
-    ZK_HIP(hipGetLastError());
+    const uint32_t* co = classes ? cls_off.as<uint32_t>() : nullptr;
+    auto make_order = [&](int range, uint32_t* ord) {
+        fill_u32(hist, 0u, nkey, st);
+        k_load_hist<<<nblk_o, 1024, nkey * 4, st>>>(off, co, range, (uint32_t)nb, lmax, qs, nkey, hist);
+        ZK_HIP(hipGetLastError());
+        exclusive_scan(hist, cur, cur + nkey, nkey, st);
+        k_load_scatter<<<nblk_o, 1024, nkey * 4, st>>>(off, co, range, (uint32_t)nb, lmax, qs, nkey, cur, ord);
+        ZK_HIP(hipGetLastError());
+    };
+    // the parts' orders first: the chunk lists below read the whole buckets'
+    // histogram
+    if (classes) {
+        make_order(MSM_RANGE_A, order_a.as<uint32_t>());
+        make_order(MSM_RANGE_B, order_b.as<uint32_t>());
+    }
+    make_order(MSM_RANGE_ALL, order.as<uint32_t>());
     // chunk lists of the oversized buckets, shared by every MSM over this sort
     fill_u32(counters.as<uint32_t>(), 0u, 16, st);
     k_chunk_list<<<64, 256, 0, st>>>(order.as<uint32_t>(), off, hist, counters.as<uint32_t>(), chunks.as<ChunkRec>(),
@@ -520,11 +568,13 @@ void msm_launch_pre(const MsmSort& sort, const void* d_points29, bool g2, MsmScr
 }
 
 void msm_launch_main(const MsmSort& sort, bool g2, MsmScratch& scr, MsmPartials& out, hipStream_t st,
-                     const void* d_points29) {
+                     const void* d_points29, int range) {
     ZK_REQUIRE(sort.n == 0 || d_points29, ZK_E_INTERNAL, "MSM launch without the 29-bit copy of its points");
     ZK_REQUIRE(scr.split >= 1, ZK_E_INTERNAL, "MSM accumulation queued before its chunked path");
-    if (g2) msm_launch_main_g2(sort, scr, out, st, d_points29);
-    else msm_launch_main_g1(sort, scr, out, st, d_points29);
+    ZK_REQUIRE(range >= MSM_RANGE_ALL && range <= MSM_RANGE_B, ZK_E_INTERNAL, "unknown MSM entry range");
+    if (!sort.classes) range = MSM_RANGE_ALL;
+    if (g2) msm_launch_main_g2(sort, scr, out, st, d_points29, range);
+    else msm_launch_main_g1(sort, scr, out, st, d_points29, range);
 }
 
 void msm_launch(const MsmSort& sort, bool g2, MsmScratch& scr, MsmPartials& out, hipStream_t st,

@@ -66,25 +66,48 @@ int msm_table_c(size_t n);
 // An entry is slot | half << 30 | sign << 31: the 29-bit base slot (the
 // scalar's index mod n, plus w*n in window w of a merged sort, whose bases
 // come from window tables), which of its two GLV halves, and the digit's sign.
+//
+// Scalar classes (optional, one byte per scalar, 0..3): the entries of every
+// bucket are then ordered by the class of their scalar, and cls_off[3b + k]
+// is where class k + 1 starts in bucket b. With the classes of the witness
+// (MSM_CLS_*), each MSM over the sort reads one contiguous range of every
+// bucket (MsmRange) that leaves out the scalars whose bases are infinite.
+enum : uint8_t {
+    MSM_CLS_A_INF = 0,   // A' infinite, B1'/B2' not
+    MSM_CLS_NONE = 1,    // no infinite base
+    MSM_CLS_B_INF = 2,   // B1' and B2' infinite, A' not
+    MSM_CLS_AB_INF = 3,  // A', B1' and B2' infinite
+};
+enum MsmRange {
+    MSM_RANGE_ALL = 0,   // every entry
+    MSM_RANGE_A = 1,     // classes 1..2 (A' finite)
+    MSM_RANGE_B = 2,     // classes 0..1 (B1', B2' finite)
+};
 struct MsmSort {
     MsmGeom g{};
     size_t n = 0;                 // scalars; the sort itself runs over 2n half scalars
     uint32_t lmax = 0, nkey = 0;
     int qs = 0;
     DevBuf half, matrix, mscan, pairs, offsets, entries, order, lkeys;
+    // with classes: cls_off, and the accumulation orders of MSM_RANGE_A / _B
+    // (by the length of those parts)
+    DevBuf cls_off, order_a, order_b;
+    bool classes = false;         // the last run() had scalar classes
     // chunk lists of the oversized buckets (above lmax entries), built by run()
     DevBuf chunks, large, counters;
     size_t n_chunk_max = 0;
-    // sizes every buffer a sort of n scalars at window c needs (no launch);
-    // returns the chunk-list capacity for MsmScratch::ensure
-    size_t reserve(size_t n, int c, bool merged = false);
-    // the device bytes reserve(n, c, merged) would add now (into gc); same return value
-    size_t count_reserve(size_t n, int c, GrowCount& gc, bool merged = false) const;
+    // sizes every buffer a sort of n scalars at window c needs (no launch),
+    // with_classes: also cls_off; returns the chunk-list capacity for
+    // MsmScratch::ensure
+    size_t reserve(size_t n, int c, bool merged = false, bool with_classes = false);
+    // the device bytes reserve(n, c, merged, with_classes) would add now (into
+    // gc); same return value
+    size_t count_reserve(size_t n, int c, GrowCount& gc, bool merged = false, bool with_classes = false) const;
     // sorts canonical scalars (n x 32 B, device, n < 2^30; W*n < 2^30 merged)
-    // into bucket order, entries as above; scalar i with skip[i] != 0 (skip:
-    // n device bytes, optional) is taken as zero, so it gets no entries
+    // into bucket order, entries as above; cls: n device bytes, the scalar
+    // classes (optional)
     void run(const void* d_scalars, size_t n, int c, hipStream_t st, bool merged = false,
-             const uint8_t* skip = nullptr);
+             const uint8_t* cls = nullptr);
 };
 
 // Result of the device part of one MSM: level-2 partials per window, on the host.
@@ -130,8 +153,10 @@ void msm_launch(const MsmSort& sort, bool g2, MsmScratch& scr, MsmPartials& out,
 // MSM's first part before any accumulation keeps those short launches from
 // waiting behind the accumulations of other streams.
 void msm_launch_pre(const MsmSort& sort, const void* d_points29, bool g2, MsmScratch& scr, hipStream_t st, int split);
+// range: the part of every bucket this MSM reads (MsmRange; a sort without
+// classes reads all of it)
 void msm_launch_main(const MsmSort& sort, bool g2, MsmScratch& scr, MsmPartials& out, hipStream_t st,
-                     const void* d_points29);
+                     const void* d_points29, int range = MSM_RANGE_ALL);
 
 // n affine points (standard form, G1 64 B / G2 128 B each) -> n slots of the
 // 29-bit-limb form (G1 128 B, G2 256 B; layout in field29.hpp), each holding

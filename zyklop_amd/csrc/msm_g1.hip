@@ -7,12 +7,14 @@ namespace zk {
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_msm_accum29(const uint32_t* __restrict__ entries,
                                                      const uint32_t* __restrict__ offsets,
-                                                     const uint32_t* __restrict__ order, uint32_t nb, uint32_t split,
-                                                     uint32_t lmax,
+                                                     const uint32_t* __restrict__ order,
+                                                     const uint32_t* __restrict__ cls_off, int range, uint32_t nb,
+                                                     uint32_t split, uint32_t lmax,
                                                      const void* __restrict__ points29,
                                                      void* __restrict__ buckets) {
     uint32_t st, en, slot;
-    if (!acc_range<Fp>(blockIdx.x * blockDim.x + threadIdx.x, nb, split, order, offsets, lmax, buckets, st, en, slot))
+    if (!acc_range<Fp>(blockIdx.x * blockDim.x + threadIdx.x, nb, split, order, offsets, cls_off, range, lmax, buckets,
+                       st, en, slot))
         return;
     Xyzz29 acc;
     acc.inf = true;
@@ -78,10 +80,13 @@ __global__ void k_points_table(const void* __restrict__ pts, uint32_t n, int W, 
 }
 
 template <>
-void launch_accum<Fp>(const MsmSort& sort, uint32_t nb, int split, const void* d_points29, void* buckets, hipStream_t st) {
+void launch_accum<Fp>(const MsmSort& sort, uint32_t nb, int split, const void* d_points29, void* buckets, hipStream_t st,
+                       int range) {
     const uint32_t nt = nb * (uint32_t)split;
     k_msm_accum29<<<cdiv(nt, 128), 128, 0, st>>>(sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(),
-                                                 sort.order.as<uint32_t>(), nb, (uint32_t)split, sort.lmax, d_points29,
+                                                 msm_range_order(sort, range),
+                                                 range != MSM_RANGE_ALL ? sort.cls_off.as<uint32_t>() : nullptr, range,
+                                                 nb, (uint32_t)split, sort.lmax, d_points29,
                                                  buckets);
 }
 
@@ -89,8 +94,9 @@ void msm_launch_pre_g1(const MsmSort& sort, const void* d_points29, MsmScratch& 
     launch_pre_t<Fp>(sort, d_points29, scr, st, split);
 }
 
-void msm_launch_main_g1(const MsmSort& sort, MsmScratch& scr, MsmPartials& out, hipStream_t st, const void* d_points29) {
-    launch_main_t<Fp>(sort, d_points29, scr, out, st);
+void msm_launch_main_g1(const MsmSort& sort, MsmScratch& scr, MsmPartials& out, hipStream_t st, const void* d_points29,
+                        int range) {
+    launch_main_t<Fp>(sort, d_points29, scr, out, st, range);
 }
 
 void msm_points_to29_g1(const void* d_points, size_t n, void* d_out, hipStream_t st) {

@@ -14,13 +14,15 @@ namespace zk {
 constexpr int ACC29Q_THREADS = 64;
 __global__ void __launch_bounds__(ACC29Q_THREADS) __attribute__((amdgpu_waves_per_eu(2))) k_msm_accum29q(const uint32_t* __restrict__ entries,
                                                                  const uint32_t* __restrict__ offsets,
-                                                                 const uint32_t* __restrict__ order, uint32_t nb,
-                                                                 uint32_t split, uint32_t lmax,
+                                                                 const uint32_t* __restrict__ order,
+                                                                 const uint32_t* __restrict__ cls_off, int range,
+                                                                 uint32_t nb, uint32_t split, uint32_t lmax,
                                                                  const void* __restrict__ points29,
                                                                  void* __restrict__ buckets) {
     __shared__ uint32_t lds[AccLds<ACC29Q_THREADS>::LDS_WORDS];
     uint32_t st, en, slot;
-    if (!acc_range<Fq2>(blockIdx.x * blockDim.x + threadIdx.x, nb, split, order, offsets, lmax, buckets, st, en, slot))
+    if (!acc_range<Fq2>(blockIdx.x * blockDim.x + threadIdx.x, nb, split, order, offsets, cls_off, range, lmax,
+                        buckets, st, en, slot))
         return;
     AccLds<ACC29Q_THREADS> A{lds + threadIdx.x, {}};
     bool inf = true;
@@ -86,19 +88,22 @@ __global__ void k_points_table_q(const void* __restrict__ pts, uint32_t n, int W
 }
 
 template <>
-void launch_accum<Fq2>(const MsmSort& sort, uint32_t nb, int split, const void* d_points29, void* buckets, hipStream_t st) {
+void launch_accum<Fq2>(const MsmSort& sort, uint32_t nb, int split, const void* d_points29, void* buckets, hipStream_t st,
+                       int range) {
     const uint32_t nt = nb * (uint32_t)split;
     k_msm_accum29q<<<cdiv(nt, ACC29Q_THREADS), ACC29Q_THREADS, 0, st>>>(
-        sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(), sort.order.as<uint32_t>(), nb, (uint32_t)split,
-        sort.lmax, d_points29, buckets);
+        sort.entries.as<uint32_t>(), sort.offsets.as<uint32_t>(), msm_range_order(sort, range),
+        range != MSM_RANGE_ALL ? sort.cls_off.as<uint32_t>() : nullptr, range, nb, (uint32_t)split, sort.lmax,
+        d_points29, buckets);
 }
 
 void msm_launch_pre_g2(const MsmSort& sort, const void* d_points29, MsmScratch& scr, hipStream_t st, int split) {
     launch_pre_t<Fq2>(sort, d_points29, scr, st, split);
 }
 
-void msm_launch_main_g2(const MsmSort& sort, MsmScratch& scr, MsmPartials& out, hipStream_t st, const void* d_points29) {
-    launch_main_t<Fq2>(sort, d_points29, scr, out, st);
+void msm_launch_main_g2(const MsmSort& sort, MsmScratch& scr, MsmPartials& out, hipStream_t st, const void* d_points29,
+                        int range) {
+    launch_main_t<Fq2>(sort, d_points29, scr, out, st, range);
 }
 
 void msm_points_to29_g2(const void* d_points, size_t n, void* d_out, hipStream_t st) {

@@ -126,21 +126,29 @@ ZK_DEV void store_bucket(void* buckets, size_t slot, const Xyzz<T>& p) {
 // Such a bucket is summed by the chunked path before this launch
 // (k_msm_chunks, k_msm_merge into slot b*split); its other parts store
 // infinity here. Otherwise [st, en) are the thread's entries and slot its
-// bucket slot.
+// bucket slot. With cls_off (a sort with scalar classes) the parts cover the
+// bucket's entries of MsmRange `range` only; the chunked path reads whole
+// buckets and passes over the infinite bases there.
 template <class T>
 ZK_DEV bool acc_range(uint32_t tid, uint32_t nb, uint32_t split, const uint32_t* __restrict__ order,
-                      const uint32_t* __restrict__ offsets, uint32_t lmax, void* __restrict__ buckets, uint32_t& st,
-                      uint32_t& en, uint32_t& slot) {
+                      const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ cls_off, int range,
+                      uint32_t lmax, void* __restrict__ buckets, uint32_t& st, uint32_t& en, uint32_t& slot) {
     if (tid >= nb * split) return false;
     const uint32_t ob = tid / split, h = tid - ob * split;
     const uint32_t b = order[ob];
-    const uint32_t s0 = offsets[b], e0 = offsets[b + 1];
-    const uint32_t cnt = e0 - s0;
+    uint32_t s0 = offsets[b], e0 = offsets[b + 1];
     slot = b * split + h;
-    if (cnt > lmax) {
+    if (e0 - s0 > lmax) {
         if (h != 0) store_bucket<T>(buckets, slot, xyzz_inf<T>());
         return false;
     }
+    if (cls_off && range == MSM_RANGE_A) {
+        s0 = cls_off[3 * (size_t)b];
+        e0 = cls_off[3 * (size_t)b + 2];
+    } else if (cls_off && range == MSM_RANGE_B) {
+        e0 = cls_off[3 * (size_t)b + 1];
+    }
+    const uint32_t cnt = e0 - s0;
     st = s0 + (uint32_t)(((uint64_t)cnt * h) / split);
     en = s0 + (uint32_t)(((uint64_t)cnt * (h + 1)) / split);
     return true;
@@ -356,19 +364,26 @@ static void launch_pre_t(const MsmSort& sort, const void* d_points29, MsmScratch
     ZK_HIP(hipGetLastError());
 }
 
+// accumulation order of the MSMs that read MsmRange `range` of the sort
+inline const uint32_t* msm_range_order(const MsmSort& sort, int range) {
+    const DevBuf& o = range == MSM_RANGE_A ? sort.order_a : range == MSM_RANGE_B ? sort.order_b : sort.order;
+    return o.as<uint32_t>();
+}
+
 // the bucket-accumulation kernel of curve T (msm_g1.hip / msm_g2.hip)
 template <class T>
-void launch_accum(const MsmSort& sort, uint32_t nb, int split, const void* d_points29, void* buckets, hipStream_t st);
+void launch_accum(const MsmSort& sort, uint32_t nb, int split, const void* d_points29, void* buckets, hipStream_t st,
+                  int range);
 
 template <class T>
 static void launch_main_t(const MsmSort& sort, const void* d_points29, MsmScratch& scr, MsmPartials& out,
-                          hipStream_t st) {
+                          hipStream_t st, int range) {
     const bool g2 = std::is_same<T, Fq2>::value;
     const MsmGeom g = msm_geom(sort.n, sort.g.c, g2, sort.g.merged);
     const int split = scr.split;
     const uint32_t nb = (uint32_t)((size_t)g.Wr * g.K);
     if (scr.ev_acc0) ZK_HIP(hipEventRecord(scr.ev_acc0, st));
-    launch_accum<T>(sort, nb, split, d_points29, scr.buckets.p, st);
+    launch_accum<T>(sort, nb, split, d_points29, scr.buckets.p, st, range);
     ZK_HIP(hipGetLastError());
     if (scr.ev_acc1) ZK_HIP(hipEventRecord(scr.ev_acc1, st));
     uint32_t nseg_all = (uint32_t)g.Wr * g.nseg;
@@ -398,11 +413,13 @@ static void launch_main_t(const MsmSort& sort, const void* d_points29, MsmScratc
 
 // per-curve entry points, defined in msm_g1.hip and msm_g2.hip
 void msm_launch_pre_g1(const MsmSort& sort, const void* d_points29, MsmScratch& scr, hipStream_t st, int split);
-void msm_launch_main_g1(const MsmSort& sort, MsmScratch& scr, MsmPartials& out, hipStream_t st, const void* d_points29);
+void msm_launch_main_g1(const MsmSort& sort, MsmScratch& scr, MsmPartials& out, hipStream_t st, const void* d_points29,
+                        int range);
 void msm_points_to29_g1(const void* d_points, size_t n, void* d_out, hipStream_t st);
 void msm_points_table_g1(const void* d_points, size_t n, const MsmGeom& g, void* d_out, hipStream_t st);
 void msm_launch_pre_g2(const MsmSort& sort, const void* d_points29, MsmScratch& scr, hipStream_t st, int split);
-void msm_launch_main_g2(const MsmSort& sort, MsmScratch& scr, MsmPartials& out, hipStream_t st, const void* d_points29);
+void msm_launch_main_g2(const MsmSort& sort, MsmScratch& scr, MsmPartials& out, hipStream_t st, const void* d_points29,
+                        int range);
 void msm_points_to29_g2(const void* d_points, size_t n, void* d_out, hipStream_t st);
 void msm_points_table_g2(const void* d_points, size_t n, const MsmGeom& g, void* d_out, hipStream_t st);
 

@@ -103,7 +103,7 @@ void Context::ensure_comb() {
 size_t Circuit::device_bytes() const {
     return rowA.bytes + colA.bytes + valA.bytes + rowB.bytes + colB.bytes + valB.bytes + A.bytes + B1.bytes + B2.bytes +
            C.bytes + H.bytes + A29.bytes + B1_29.bytes + B2_29.bytes + C29.bytes + H29.bytes + wit_rows.bytes +
-           b_skip.bytes;
+           wire_cls.bytes;
 }
 
 static void upload(DevBuf& d, const void* src, size_t bytes) {
@@ -144,21 +144,24 @@ static bool words_zero(const uint8_t* p, size_t bytes) {
     return o == 0;
 }
 
-// Circuit::b_skip from the extended B1'/B2' arrays (bases, then the four extra
-// points, a null extra being infinity): set where both points are infinite
-static void build_b_skip(Circuit& c, const uint8_t* B1, const uint8_t* B2, const uint64_t* const exB1[4],
-                         const uint64_t* const exB2[4]) {
+// Circuit::wire_cls from the extended A', B1', B2' arrays (bases, then the
+// four extra points, a null extra being infinity)
+static void build_wire_cls(Circuit& c, const uint8_t* A, const uint8_t* B1, const uint8_t* B2,
+                           const uint64_t* const exA[4], const uint64_t* const exB1[4], const uint64_t* const exB2[4]) {
     const size_t nv = c.n_vars, ne = nv + 4;
-    std::vector<uint8_t> sk(ne);
-    size_t cnt = 0;
+    auto inf = [&](const uint8_t* base, const uint64_t* const ex[4], size_t bytes, size_t i) {
+        const uint8_t* p = i < nv ? base + bytes * i : reinterpret_cast<const uint8_t*>(ex[i - nv]);
+        return !p || words_zero(p, bytes);
+    };
+    std::vector<uint8_t> cls(ne);
+    size_t n_inf = 0;
     for (size_t i = 0; i < ne; i++) {
-        const uint8_t* p1 = i < nv ? B1 + 64 * i : reinterpret_cast<const uint8_t*>(exB1[i - nv]);
-        const uint8_t* p2 = i < nv ? B2 + 128 * i : reinterpret_cast<const uint8_t*>(exB2[i - nv]);
-        sk[i] = (!p1 || words_zero(p1, 64)) && (!p2 || words_zero(p2, 128));
-        cnt += sk[i];
+        const bool a = inf(A, exA, 64, i), b = inf(B1, exB1, 64, i) && inf(B2, exB2, 128, i);
+        cls[i] = a ? (b ? MSM_CLS_AB_INF : MSM_CLS_A_INF) : (b ? MSM_CLS_B_INF : MSM_CLS_NONE);
+        n_inf += (size_t)a + (size_t)b;
     }
-    if (cnt * 8 >= ne) upload(c.b_skip, sk.data(), ne);
-    else c.b_skip.release();
+    if (n_inf * 8 >= ne) upload(c.wire_cls, cls.data(), ne);
+    else c.wire_cls.release();
 }
 
 static void finish_circuit_bases(Circuit& c, const uint8_t* A, const uint8_t* B1, const uint8_t* B2, const uint8_t* C,
@@ -172,7 +175,7 @@ static void finish_circuit_bases(Circuit& c, const uint8_t* A, const uint8_t* B1
     build_ext(c.A, nullptr, 0, A, nv, exA, 64);
     build_ext(c.B1, nullptr, 0, B1, nv, exB1, 64);
     build_ext(c.B2, nullptr, 0, B2, nv, exB2, 128);
-    build_b_skip(c, B1, B2, exB1, exB2);
+    build_wire_cls(c, A, B1, B2, exA, exB1, exB2);
     build_ext(c.C, nullptr, l + 1, C, nv - l - 1, exC, 64);
     upload(c.H, H, (size_t)c.n * 64);
     circuit_set_tables(c, c.ctx->tables_mode);
@@ -347,7 +350,6 @@ ProveSlot::ProveSlot() {
     for (hipStream_t* p : {&s0, &s1, &s2, &s3, &s4}) ZK_HIP(hipStreamCreateWithFlags(p, hipStreamNonBlocking));
     ZK_HIP(hipEventCreateWithFlags(&ev_scal, hipEventDisableTiming));
     ZK_HIP(hipEventCreateWithFlags(&ev_done1, hipEventDisableTiming));
-    ZK_HIP(hipEventCreateWithFlags(&ev_doneB, hipEventDisableTiming));
     for (auto& e : ev_join) ZK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     ZK_HIP(hipEventCreateWithFlags(&ev_h, hipEventDisableTiming));
 }
@@ -366,7 +368,6 @@ ProveSlot::~ProveSlot() {
         if (p) { (void)hipStreamSynchronize(p); (void)hipStreamDestroy(p); }
     if (ev_scal) (void)hipEventDestroy(ev_scal);
     if (ev_done1) (void)hipEventDestroy(ev_done1);
-    if (ev_doneB) (void)hipEventDestroy(ev_doneB);
     if (ev_h) (void)hipEventDestroy(ev_h);
     for (auto& e : ev_join)
         if (e) (void)hipEventDestroy(e);
@@ -374,9 +375,10 @@ ProveSlot::~ProveSlot() {
 
 size_t ProveSlot::device_bytes() const {
     size_t b = scal.bytes + aw.bytes + bw.bytes + cw.bytes + t0.bytes + t1.bytes + t2.bytes + h.bytes;
-    for (const MsmSort* q : {&sortW, &sortH, &sortB})
+    for (const MsmSort* q : {&sortW, &sortH})
         b += q->half.bytes + q->matrix.bytes + q->mscan.bytes + q->pairs.bytes + q->offsets.bytes + q->entries.bytes +
-             q->order.bytes + q->lkeys.bytes + q->chunks.bytes + q->large.bytes + q->counters.bytes;
+             q->order.bytes + q->lkeys.bytes + q->chunks.bytes + q->large.bytes + q->counters.bytes + q->cls_off.bytes +
+             q->order_a.bytes + q->order_b.bytes;
     for (const MsmScratch* s : {&scrA, &scrB1, &scrC, &scrB2, &scrH})
         b += s->buckets.bytes + s->r1.bytes + s->t1.bytes + s->r2.bytes + s->t2.bytes + s->partials.bytes;
     return b;
@@ -406,9 +408,7 @@ void ProveSlot::reserve(Context& ctx, const Circuit& c, bool host_witness) {
     prepare(ctx, c);
     if (host_witness) staging.ensure((size_t)c.n_vars * 32);
     const MsmPlan pl = ctx.msm_plan(c);
-    size_t kw = sortW.reserve(nS, pl.cw, pl.mw);
-    const size_t kh = sortH.reserve(n, pl.ch, pl.mh);
-    if (c.b_skip.bytes) kw = std::max(kw, sortB.reserve(nS, pl.cw, pl.mw));
+    const size_t kw = sortW.reserve(nS, pl.cw, pl.mw, c.wire_cls.bytes != 0), kh = sortH.reserve(n, pl.ch, pl.mh);
     const MsmGeom gw1 = msm_geom(nS, pl.cw, false, pl.mw), gw2 = msm_geom(nS, pl.cw, true, pl.mw),
                   gh = msm_geom(n, pl.ch, false, pl.mh);
     for (MsmScratch* q : {&scrA, &scrB1, &scrC}) q->ensure(gw1, kw, false, pl.sp_w1);
@@ -465,40 +465,31 @@ void ProveSlot::body(Context& ctx, const Circuit& c, ProofParts& out, bool alone
     //    13.8 -> 13.4 ms; in batches the other slots fill that gap and the
     //    extra MSM stream does better).
     //
-    // A circuit whose B matrix leaves wires out (b_skip) gets a second sort
-    // of the same scalars with those wires left out, on s4 ahead of B2: B1
-    // and B2 then never read an infinite base. An infinite base in the shared
-    // entries costs its lane a whole mixed addition, since the other lanes of
-    // the wave are adding; a quarter of the wires of the synthetic circuits
-    // are such (row i draws its wires uniformly from the first ones).
+    // With wire classes (Circuit::wire_cls) the A MSM and the B1, B2 MSMs
+    // read only the entries of their finite bases. An infinite base in the
+    // entries would cost its lane a whole mixed addition, since the other
+    // lanes of the wave are adding; a quarter of the A' and of the B' bases
+    // of the synthetic circuits are infinite (row i draws its wires uniformly
+    // from the first ones, so late wires are rarely used).
     alone_layout = alone;
     hipStream_t s_sort = alone ? s2 : s1, s_a = alone ? s2 : s1;
-    const bool sep_b = c.b_skip.bytes != 0;
-    const MsmSort& sB = sep_b ? sortB : sortW;
-    if (sep_b) {
-        ZK_HIP(hipStreamWaitEvent(s4, ev_scal, 0));
-        sortB.run(scal.p, nS, window_w, s4, pl.mw, c.b_skip.as<uint8_t>());
-        ZK_HIP(hipEventRecord(ev_doneB, s4));
-    }
+    const uint8_t* cls = c.wire_cls.bytes ? c.wire_cls.as<uint8_t>() : nullptr;
     ZK_HIP(hipStreamWaitEvent(s_sort, ev_scal, 0));
     if (timed) ZK_HIP(hipEventRecord(ev[4], s_sort));
-    sortW.run(scal.p, nS, window_w, s_sort, pl.mw);
+    sortW.run(scal.p, nS, window_w, s_sort, pl.mw, cls);
     ZK_HIP(hipEventRecord(ev_done1, s_sort));
     if (timed) ZK_HIP(hipEventRecord(ev[5], s_sort));
-    for (hipStream_t p : {s1, s2, s3, s4}) {
-        if (sep_b && (p == s2 || p == s4)) continue;   // B1 and B2 read the B sort
+    for (hipStream_t p : {s1, s2, s3, s4})
         if (p != s_sort) ZK_HIP(hipStreamWaitEvent(p, ev_done1, 0));
-    }
-    if (sep_b) ZK_HIP(hipStreamWaitEvent(s2, ev_doneB, 0));
-    msm_launch_pre(sB, c.B2_29.p, true, scrB2, s4, sp_w2);
-    msm_launch_pre(sB, c.B1_29.p, false, scrB1, s2, sp_w1);
+    msm_launch_pre(sortW, c.B2_29.p, true, scrB2, s4, sp_w2);
+    msm_launch_pre(sortW, c.B1_29.p, false, scrB1, s2, sp_w1);
     msm_launch_pre(sortW, c.C29.p, false, scrC, s3, sp_w1);
     msm_launch_pre(sortW, c.A29.p, false, scrA, s_a, sp_w1);
-    msm_launch_main(sB, true, scrB2, out.b2, s4, c.B2_29.p);
-    if (!alone) msm_launch_main(sortW, false, scrA, out.a, s_a, c.A29.p);
-    msm_launch_main(sB, false, scrB1, out.b1, s2, c.B1_29.p);
+    msm_launch_main(sortW, true, scrB2, out.b2, s4, c.B2_29.p, MSM_RANGE_B);
+    if (!alone) msm_launch_main(sortW, false, scrA, out.a, s_a, c.A29.p, MSM_RANGE_A);
+    msm_launch_main(sortW, false, scrB1, out.b1, s2, c.B1_29.p, MSM_RANGE_B);
     msm_launch_main(sortW, false, scrC, out.c, s3, c.C29.p);
-    if (alone) msm_launch_main(sortW, false, scrA, out.a, s_a, c.A29.p);
+    if (alone) msm_launch_main(sortW, false, scrA, out.a, s_a, c.A29.p, MSM_RANGE_A);
     // H pipeline on s0
     spmv_abc(c.rowA.as<uint32_t>(), c.colA.as<uint32_t>(), c.valA.p, c.n_rowsA, c.rowB.as<uint32_t>(),
              c.colB.as<uint32_t>(), c.valB.p, c.n_rowsB, scal.p, n, aw.p, bw.p, cw.p, s0);
@@ -551,8 +542,7 @@ size_t ProveSlot::count_bytes(const Context& ctx, const Circuit& c, const ProveS
         g.bytes += 7 * n * 32;
     }
     const MsmPlan pl = ctx.msm_plan(c);
-    size_t kw = (sl ? sl->sortW : no_sort).count_reserve(nS, pl.cw, g, pl.mw);
-    if (c.b_skip.bytes) kw = std::max(kw, (sl ? sl->sortB : no_sort).count_reserve(nS, pl.cw, g, pl.mw));
+    const size_t kw = (sl ? sl->sortW : no_sort).count_reserve(nS, pl.cw, g, pl.mw, c.wire_cls.bytes != 0);
     const size_t kh = (sl ? sl->sortH : no_sort).count_reserve(n, pl.ch, g, pl.mh);
     const MsmGeom gw1 = msm_geom(nS, pl.cw, false, pl.mw), gw2 = msm_geom(nS, pl.cw, true, pl.mw),
                   gh = msm_geom(n, pl.ch, false, pl.mh);

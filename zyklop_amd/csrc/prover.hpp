@@ -47,11 +47,12 @@ struct Circuit {
     // window of the tables (msm_points_table) of the witness arrays (A', B1',
     // C', B2') and of H; 0: no tables, one slot per base
     int tab_c[2] = {0, 0};
-    // one byte per witness scalar (n_vars + 4), set where the B1' and B2'
-    // bases are both infinite (a wire no row of B uses): the B1 and B2 MSMs
-    // sort their digits apart from A's and C's with these scalars left out.
-    // Empty when fewer than 1/8 of the wires qualify (one shared sort then).
-    DevBuf b_skip;
+    // class of every witness scalar (n_vars + 4 bytes, MSM_CLS_*): which of
+    // its A', B1'/B2' bases are infinite (a wire that no row of A, or of B,
+    // uses). The witness sort orders each bucket by class, so that the A MSM
+    // and the B1, B2 MSMs read only entries with finite bases. Empty when
+    // fewer than 1/8 of the A' and B' bases are infinite.
+    DevBuf wire_cls;
     ZkeyHeader hdr;               // VK in file layout
     std::vector<uint8_t> coefs;   // §4 bytes, kept to write the circuit back out
     std::vector<uint64_t> ic;     // (l+1) x 8
@@ -85,15 +86,14 @@ double assemble_proof(const ProofParts& p, const uint64_t r[4], const uint64_t s
                       const hipEvent_t* ready = nullptr);
 
 struct ProveSlot {
-    // s0: H pipeline + H MSM; s1: witness sort + A; s2: B1; s3: C; s4: B2
-    // (and the B sort, when the circuit has one). Each proof forks s1..s4
-    // from s0 and joins them back into s0 at its end.
+    // s0: H pipeline + H MSM; s1: witness sort + A; s2: B1; s3: C; s4: B2.
+    // Each proof forks s1..s4 from s0 and joins them back into s0 at its end.
     hipStream_t s0 = nullptr, s1 = nullptr, s2 = nullptr, s3 = nullptr, s4 = nullptr;
-    hipEvent_t ev_scal = nullptr, ev_done1 = nullptr, ev_doneB = nullptr, ev_join[4] = {}, ev_h = nullptr;
+    hipEvent_t ev_scal = nullptr, ev_done1 = nullptr, ev_join[4] = {}, ev_h = nullptr;
     bool alone_layout = false;   // the layout body() used for the queued proof
     DevBuf scal, aw, bw, cw, t0, t1, t2, h, wtmp;
     HostBuf staging;                 // pinned copy of a host witness (async upload)
-    MsmSort sortW, sortH, sortB;   // witness (A, C; also B1, B2 without b_skip), H, B1 and B2
+    MsmSort sortW, sortH;
     MsmScratch scrA, scrB1, scrC, scrB2, scrH;
     ProofParts parts;
     uint64_t r[4], s[4];
