@@ -51,15 +51,18 @@ CALIB_BYTES = 384 << 20                 # device copy before the probes (profile
 # (`bench.py --sections none --no-cpu --no-tune --probe-reps 1`), scaled by the
 # calibration copy; profiles/probe_kernels_hbm_counters.md
 # G1 / G2 accumulation over the window tables of the 2^20 circuit (7 windows
-# of 18-19 bits over the GLV halves sharing one bucket set), 14,737,409 entries:
-HBM_BYTES_PER_ENTRY_G1 = (2.7586e9 + 0.1070e9) / 14737409
-HBM_BYTES_PER_ENTRY_G2 = (2.3723e9 + 0.0671e9) / 14737409
+# of 18-19 bits over the GLV halves sharing one bucket set), per entry read:
+# of the sort's 14,737,409 entries the A' probe reads the 11,054,444 of wires
+# with a finite A' base, the B2' probe the 11,052,135 with finite B' bases
+
+HBM_BYTES_PER_ENTRY_G1 = (1.9644e9 + 0.1069e9) / 11054444
+HBM_BYTES_PER_ENTRY_G2 = (1.8688e9 + 0.0671e9) / 11052135
 ALGO_BYTES_PER_ENTRY_G1 = 68            # y and x (or beta*x) of the base, 32 B each packed, + the 4 B entry
 ALGO_BYTES_PER_ENTRY_G2 = 132           # the same over Fp2, + the 4 B entry
 # NTT of 2^22, per transform (three pass pairs + the last pass)
-HBM_BYTES_NTT_2P22 = {"forward": 1.2290e9, "inverse": 1.3581e9, "coset": 1.3592e9}
+HBM_BYTES_NTT_2P22 = {"forward": 1.2250e9, "inverse": 1.3579e9, "coset": 1.3621e9}
 # SpMV of the 2^20 circuit (k_spmv_abc)
-HBM_BYTES_SPMV_2P20 = 1.3491e9 + 0.2017e9
+HBM_BYTES_SPMV_2P20 = 1.3459e9 + 0.2017e9
 
 
 def log(msg):
@@ -339,11 +342,13 @@ def section_probes(args, zk, ctx, circ):
         "unit_def": ("one signed-digit (point, bucket) entry = one XYZZ mixed add = 10 Fp products x 136 32x32 "
                      "MACs (the algorithmic count of a 256-bit Montgomery product, whatever the limb size)"),
         "probe": (f"G1 MSM over the resident A' bases, n={n_ext}, c={c_w}"
-                  f"{' over window tables (one bucket set)' if tabs[0] else ''}, alone on the device, "
+                  f"{' over window tables (one bucket set)' if tabs[0] else ''}, as a proof runs it (the "
+                  f"entries of the wires whose A' base is finite), alone on the device, "
                   f"{args.probe_reps} launches, HIP events on the MSM stream"),
         "msm_total_ms": r3(t1[0]), "sort_ms": r3(t1[1]), "reduction_ms": r3(t1[4]),
         "g2": {"kernel": "k_msm_accum29q (G2, 29-bit limbs, accumulator in LDS)", "achieved": round(g2_achieved, 3),
                "frac": round(g2_achieved / PEAK_MAC_TS, 4), "avg_launch_ms": r3(t2[2]),
+               "units_per_launch": int(t2[3]),
                "traffic": int(float(t2[3]) * HBM_BYTES_PER_ENTRY_G2),
                "algorithmic_bytes": int(float(t2[3]) * ALGO_BYTES_PER_ENTRY_G2),
                "msm_total_ms": r3(t2[0]), "reduction_ms": r3(t2[4]), "unit_def": "entry = Fp2 XYZZ mixed add = 30 Fp-mul x 136 MACs"},

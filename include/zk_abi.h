@@ -289,8 +289,11 @@ int zk_circuit_bases(const zk_circuit* c, int which, void** d_ptr, uint64_t* cou
  * resident base array `which` (as in zk_circuit_bases; 2 is G2, out[16]),
  * through the arrays the prover itself reads: the window tables when c has
  * them (window 0, or one with the tables' window count), else the 29-bit
- * copy at `window` (0 = tuned or default). Same result as zk_msm_g1_device /
- * zk_msm_g2_device over zk_circuit_bases' points; timing in zk_last_timing.
+ * copy at `window` (0 = tuned or default); for A', B1' and B2' of a circuit
+ * with wire classes (enough infinite bases), the prover's class-ordered sort,
+ * reading only the entries of finite bases. Same result as zk_msm_g1_device /
+ * zk_msm_g2_device over zk_circuit_bases' points; timing in zk_last_timing
+ * ([3]: the entries the accumulation read).
  * No counterpart in the reference (its MSMs run inside groth16Prove,
  * src/groth16.cu:1143-1397). */
 int zk_circuit_msm_device(zk_ctx* ctx, const zk_circuit* c, int which, const void* d_scalars, int window,

@@ -110,6 +110,31 @@ def test_circuit_msm_over_tables_skewed(ctx):
     check_all_arrays(ctx, circ, scalars)
 
 
+def test_wire_classes_leave_out_infinite_bases(ctx):
+    """A wire that no row of A (of B) uses has an infinite A' (B1', B2') base;
+    a quarter of a synthetic circuit's wires are such. Over the same scalars
+    the A and B MSMs read fewer entries than C, which reads them all, B1 and
+    B2 the same ones, and every result equals the MSM without classes."""
+    circ = ctx.synth_circuit(48, 6000, 8)
+    rng = np.random.default_rng(48)
+    _, n = circ.bases(0)
+    sc = rand_scalars(rng, n)
+    d_sc = ctx.dev_alloc(sc.nbytes)
+    try:
+        ctx.upload(d_sc, sc)
+        read = []
+        for which in range(4):
+            got, ginf = ctx.circuit_msm_device(circ, which, d_sc)
+            read.append(int(ctx.last_timing()[3]))
+            exp, einf = plain_msm(ctx, circ, which, d_sc)
+            assert ginf == einf and np.array_equal(got, exp), which
+    finally:
+        ctx.dev_free(d_sc)
+    a, b1, b2, c = read
+    assert b1 == b2
+    assert 0.5 * c < a < 0.95 * c and 0.5 * c < b1 < 0.95 * c, read
+
+
 def test_circuit_msm_window_without_tables(ctx):
     """A window whose window count differs from the tables' runs over table 0
     (plain buckets per window); one with the same count uses the tables."""

@@ -146,8 +146,11 @@ void put_result(const zkh::Xyzz<HT>& p, uint64_t* out, int* is_inf) {
 
 // MSM over device points; slots29, when given, are the points' 29-bit slots
 // already made (a circuit's resident arrays, W window tables when merged)
+// cls / range: scalar classes and the part of every bucket to read, as a
+// proof's witness MSMs do (MsmSort::run, msm_launch_main)
 void msm_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t n, int window, bool g2, uint64_t* out,
-                int* is_inf, const void* slots29 = nullptr, bool merged = false) {
+                int* is_inf, const void* slots29 = nullptr, bool merged = false, const uint8_t* cls = nullptr,
+                int range = zk::MSM_RANGE_ALL) {
     zk::Context& C = ctx->impl;
     hipStream_t st = C.stream;
     zk::MsmSort sort;
@@ -161,18 +164,29 @@ void msm_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t
         zk::msm_points_to29(d_points, n, p29.p, g2, st);
     }
     ZK_HIP(hipEventRecord(ctx->ev0, st));
-    sort.run(d_scalars, n, window, st, merged);
+    sort.run(d_scalars, n, window, st, merged, cls);
     ZK_HIP(hipEventRecord(ctx->ev1, st));
-    zk::msm_launch(sort, g2, scr, part, st, slots29 ? slots29 : p29.p,
-                   zk::msm_default_split(n, sort.g.c, g2, merged));
+    const void* pts = slots29 ? slots29 : p29.p;
+    zk::msm_launch_pre(sort, pts, g2, scr, st, zk::msm_default_split(n, sort.g.c, g2, merged));
+    zk::msm_launch_main(sort, g2, scr, part, st, pts, range);
     ZK_HIP(hipEventRecord(ctx->ev2, st));
     ZK_HIP(hipStreamSynchronize(st));
     ZK_HIP(hipEventElapsedTime(&ctx->last_ms[0], ctx->ev0, ctx->ev2));
     ZK_HIP(hipEventElapsedTime(&ctx->last_ms[1], ctx->ev0, ctx->ev1));
     ZK_HIP(hipEventElapsedTime(&ctx->last_ms[2], scr.ev_acc0, scr.ev_acc1));
     ZK_HIP(hipEventElapsedTime(&ctx->last_ms[4], scr.ev_acc1, scr.ev_end));
+    // entries the accumulation read: the sort's, or the parts of the range
+    const size_t nb = (size_t)sort.g.Wr * sort.g.K;
     uint32_t entries = 0;
-    ZK_HIP(hipMemcpy(&entries, sort.offsets.as<uint32_t>() + (size_t)sort.g.Wr * sort.g.K, 4, hipMemcpyDeviceToHost));
+    ZK_HIP(hipMemcpy(&entries, sort.offsets.as<uint32_t>() + nb, 4, hipMemcpyDeviceToHost));
+    if (sort.classes && range != zk::MSM_RANGE_ALL) {
+        std::vector<uint32_t> off(nb + 1), co(3 * nb);
+        ZK_HIP(hipMemcpy(off.data(), sort.offsets.p, 4 * (nb + 1), hipMemcpyDeviceToHost));
+        ZK_HIP(hipMemcpy(co.data(), sort.cls_off.p, 12 * nb, hipMemcpyDeviceToHost));
+        entries = 0;
+        for (size_t b = 0; b < nb; b++)
+            entries += range == zk::MSM_RANGE_A ? co[3 * b + 2] - co[3 * b] : co[3 * b + 1] - off[b];
+    }
     ctx->last_ms[3] = (float)entries;
     if (g2) put_result(zk::msm_finish_g2(part), out, is_inf);
     else put_result(zk::msm_finish_g1(part), out, is_inf);
@@ -765,7 +779,11 @@ int zk_circuit_msm_device(zk_ctx* ctx, const zk_circuit* c, int which, const voi
         // over the window tables unless a window with another window count is asked for
         const bool merged = tc && (!window || zk::msm_geom(n, window, false).W == zk::msm_geom(n, tc, false).W);
         const int cw = merged ? tc : (window ? window : ctx->impl.window_for(kind, n));
-        msm_device(ctx, std_pts[which]->p, d_scalars, n, cw, which == 2, out, is_inf, slots[which]->p, merged);
+        // A', B1', B2': the prover's wire classes, which leave out infinite bases
+        const uint8_t* cls = which <= 2 && k.wire_cls.bytes ? k.wire_cls.as<uint8_t>() : nullptr;
+        const int range = !cls ? zk::MSM_RANGE_ALL : which == 0 ? zk::MSM_RANGE_A : zk::MSM_RANGE_B;
+        msm_device(ctx, std_pts[which]->p, d_scalars, n, cw, which == 2, out, is_inf, slots[which]->p, merged, cls,
+                   range);
     });
 }
 
