@@ -348,8 +348,13 @@ int zk_pairing(const uint64_t p[8], const uint64_t q[16], uint64_t out[48]);
  * (no range check: inputs outside the form's documented bounds give
  * unspecified values): op 0 product, 1 square, 2 a*b + b*a with one
  * reduction, 3 a*a + a*b + b*a + b*b with one reduction, 4 negation of a
- * value < p, 5 a + 4p - b normalized, 6 reduction of a value < 16p below 2p,
- * 7 conversion to the standard form and back. */
+ * value < p (2p - a, unnormalized), 5 a + 4p - b normalized, 6 reduction of a
+ * value < 16p below 2p, 7 conversion to the standard form and back, 8
+ * negation 4p - a of a normalized a < 2p (unnormalized), 9 the G1 mixed
+ * addition's Y3 sum a*(a + 4p - b) + b*(4p - b) with one reduction (the
+ * difference and the negation unnormalized); on element pairs (c0, c1) of
+ * Fp2, n even: 10 the Fp2 product a*b, 11 a*b - b*b as the G2 Y3's one
+ * reduction per component. */
 int zk_field_op_device(zk_ctx* ctx, int field, int op, const uint64_t* a, const uint64_t* b, uint64_t* out, size_t n);
 int zk_field29_op_device(zk_ctx* ctx, int op, const uint32_t* a, const uint32_t* b, uint32_t* out, size_t n);
 

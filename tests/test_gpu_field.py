@@ -9,8 +9,10 @@ zk_field29_op_device).
   Python big ints at the edges of its documented input forms: N (limbs <
   2^29, value < 2p), L (a + 4p - b before normalisation: limbs < 3*2^29,
   value < 6p), normalized values up to 6p, sums of up to four products, and
-  values up to 16p for the reduction. Every result must be congruent to the
-  definition and inside its documented output form.
+  values up to 16p for the reduction; the unnormalized negations 2p - y and
+  4p - y as product operands, the G1 mixed addition's Y3 sum, and the Fp2
+  product and T*A - B*C of the G2 loops. Every result must be congruent to
+  the definition and inside its documented output form.
 """
 import numpy as np
 import pytest
@@ -164,3 +166,56 @@ def test_f29_sums_reduce_neg_roundtrip(ctx):
     ts = edge_values(rng, 100, 2 * P)
     for row, t in zip(ctx.field29_op(7, arr29([limbs29(t) for t in ts])), ts):
         check_n(row, t, bound=P)
+
+
+def n_form_edges(rng, k):
+    """normalized values below 2p, with one whose limbs below the top are all 2^29 - 1"""
+    vals = edge_values(rng, k, 2 * P)
+    vals.append(value29([M29] * 8 + [((2 * P) >> 232) - 1]))
+    return vals
+
+
+def test_f29_lazy_negation_and_g1_y3_at_bounds(ctx):
+    """4p - y without carries (limbs in (0, 2^30)) and the G1 mixed addition's
+    Y3 = R*(Q - X3) + Y1*(4p - PPP) with the difference and the negation left
+    unnormalized: a column then sums terms up to 2^60, 2^59 and 2^58."""
+    rng = np.random.default_rng(37)
+    rinv = pow(RP, -1, P)
+    ys = n_form_edges(rng, 200)
+    for row, y in zip(ctx.field29_op(8, arr29([limbs29(y) for y in ys])), ys):
+        assert all(0 < int(x) < (1 << 30) for x in row), "limbs of 4p - y lie in (0, 2^30)"
+        assert value29(row) == 4 * P - y
+    xs = n_form_edges(rng, 300)
+    zs = list(reversed(xs))
+    out = ctx.field29_op(9, arr29([limbs29(x) for x in xs]), arr29([limbs29(z) for z in zs]))
+    for row, x, z in zip(out, xs, zs):
+        check_n(row, (x * (x + 4 * P - z) + z * (4 * P - z)) * rinv)
+
+
+def test_f29q_products_at_bounds(ctx):
+    """Fp2 products of the G2 loops (element pairs): a*b with a's components
+    normalized up to 6p or 2p - y negations of y < p, b's in the N form; and
+    T*A - B*C (here A = B = C) with T up to 6p, each component one sum of four
+    products with one reduction and two unnormalized 4p - x negations."""
+    rng = np.random.default_rng(41)
+    rinv = pow(RP, -1, P)
+    hi6 = edge_values(rng, 150, 6 * P)
+    small = edge_values(rng, 100, P)
+    lazy = ctx.field29_op(4, arr29([limbs29(y) for y in small]))      # 2p - y, unnormalized
+    a_rows = [limbs29(v) for v in hi6] + [[int(x) for x in r] for r in lazy]
+    a_vals = hi6 + [2 * P - y for y in small]
+    b_vals = n_form_edges(rng, len(a_vals))
+    n = min(len(a_vals), len(b_vals)) // 2 * 2
+    A, B = arr29(a_rows[:n]), arr29([limbs29(v) for v in b_vals[:n]])
+    out = ctx.field29_op(10, A, B)
+    for i in range(n // 2):
+        a0, a1, b0, b1 = a_vals[2 * i], a_vals[2 * i + 1], b_vals[2 * i], b_vals[2 * i + 1]
+        check_n(out[2 * i], (a0 * b0 - a1 * b1) * rinv)
+        check_n(out[2 * i + 1], (a0 * b1 + a1 * b0) * rinv)
+    m = min(len(hi6), len(b_vals)) // 2 * 2
+    T = arr29([limbs29(v) for v in hi6[:m]])
+    out = ctx.field29_op(11, T, arr29([limbs29(v) for v in b_vals[:m]]))
+    for i in range(m // 2):
+        t0, t1, y0, y1 = hi6[2 * i], hi6[2 * i + 1], b_vals[2 * i], b_vals[2 * i + 1]
+        check_n(out[2 * i], ((t0 * y0 - t1 * y1) - (y0 * y0 - y1 * y1)) * rinv)
+        check_n(out[2 * i + 1], ((t0 * y1 + t1 * y0) - 2 * y0 * y1) * rinv)

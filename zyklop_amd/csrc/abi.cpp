@@ -1066,8 +1066,10 @@ int zk_field29_op_device(zk_ctx* ctx, int op, const uint32_t* a, const uint32_t*
     return guard([&] {
         need(ctx && a && out, "NULL argument");
         need(op >= 0 && op < zk::F29_COUNT, "unknown 29-bit op");
-        need(b || op == zk::F29_SQR || op == zk::F29_NEG || op == zk::F29_REDUCE || op == zk::F29_ROUNDTRIP,
+        need(b || op == zk::F29_SQR || op == zk::F29_NEG || op == zk::F29_REDUCE || op == zk::F29_ROUNDTRIP ||
+                 op == zk::F29_NEG4,
              "binary op without b");
+        need((op != zk::F29Q_MUL && op != zk::F29Q_MSM) || n % 2 == 0, "Fp2 ops take element pairs (even n)");
         std::lock_guard<std::mutex> lk(ctx->impl.mu);
         ZK_HIP(hipSetDevice(ctx->impl.device));
         const size_t bytes = std::max<size_t>(n, 1) * 36;

@@ -34,6 +34,25 @@ __global__ void k_field_std(int op, const void* __restrict__ a, const void* __re
 __global__ void k_field29(int op, const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
                           uint32_t* __restrict__ out, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (op == F29Q_MUL || op == F29Q_MSM) {   // element pairs (c0, c1)
+        if (i >= n / 2) return;
+        F29q x, y;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            x.c0.l[k] = a[18 * i + k];
+            x.c1.l[k] = a[18 * i + 9 + k];
+            y.c0.l[k] = b[18 * i + k];
+            y.c1.l[k] = b[18 * i + 9 + k];
+        }
+        // T*A - B*C with T = x and A = B = C = y, as in the G2 Y3
+        const F29q r = op == F29Q_MUL ? f29q_mul(x, y) : f29q_mul_sub_mul(x, y, y, y);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            out[18 * i + k] = r.c0.l[k];
+            out[18 * i + 9 + k] = r.c1.l[k];
+        }
+        return;
+    }
     if (i >= n) return;
     F29 x, y;
 #pragma unroll
@@ -58,6 +77,12 @@ __global__ void k_field29(int op, const uint32_t* __restrict__ a, const uint32_t
         case F29_NEG: r = f29_neg(x); break;
         case F29_SUB4: r = f29_norm(f29_sub4(x, y)); break;
         case F29_REDUCE: r = f29_reduce(x); break;
+        case F29_NEG4: r = f29_neg_lazy4(x); break;
+        case F29_Y3: {                 // the G1 Y3 form: R*(Q - X3) + Y1*(4p - PPP), R = Q = x, X3 = Y1 = PPP = y
+            const F29 p[2] = {x, y}, q[2] = {f29_sub4(x, y), f29_neg_lazy4(y)};
+            r = f29_mulsum<2>(p, q);
+            break;
+        }
         default: {                      // F29_ROUNDTRIP: to the standard form and back
             Fp s = f29_to_std(x);
             r = f29_from_std(s);
