@@ -116,9 +116,13 @@ float elapsed(TuneBufs& tb) {
 void time_window(Context& ctx, const Circuit& circ, int kind, int c, bool merged, const void* d_sc, size_t n,
                  int reps, TuneBufs& tb, float& sort_ms, float msm[4][MSM_SPLIT_MAX]) {
     hipStream_t st = ctx.stream;
-    struct M { const void* p29; bool g2; };
-    const M w4[4] = {{circ.A29.p, false}, {circ.B1_29.p, false}, {circ.C29.p, false}, {circ.B2_29.p, true}};
-    const M h1[1] = {{circ.H29.p, false}};
+    // the witness MSMs as a proof runs them: with the circuit's wire classes,
+    // A and B1, B2 read their parts of the buckets
+    struct M { const void* p29; bool g2; int range; };
+    const M w4[4] = {{circ.A29.p, false, MSM_RANGE_A}, {circ.B1_29.p, false, MSM_RANGE_B},
+                     {circ.C29.p, false, MSM_RANGE_ALL}, {circ.B2_29.p, true, MSM_RANGE_B}};
+    const M h1[1] = {{circ.H29.p, false, MSM_RANGE_ALL}};
+    const uint8_t* cls = kind == 0 && circ.wire_cls.bytes ? circ.wire_cls.as<uint8_t>() : nullptr;
     const M* ms = kind == 0 ? w4 : h1;
     const int nm = kind == 0 ? 4 : 1;
     const MsmGeom g = msm_geom(n, c, false, merged);
@@ -127,7 +131,7 @@ void time_window(Context& ctx, const Circuit& circ, int kind, int c, bool merged
         for (int s = 0; s < MSM_SPLIT_MAX; s++) msm[m][s] = -1;
     for (int it = 0; it <= reps; it++) {
         ZK_HIP(hipEventRecord(tb.e0, st));
-        tb.sort.run(d_sc, n, c, st, merged);
+        tb.sort.run(d_sc, n, c, st, merged, cls);
         ZK_HIP(hipEventRecord(tb.e1, st));
         const float t_sort = elapsed(tb);
         if (it > 0) sort_ms += t_sort / reps;
@@ -135,7 +139,9 @@ void time_window(Context& ctx, const Circuit& circ, int kind, int c, bool merged
             for (int s = 1; s <= MSM_SPLIT_MAX; s++) {
                 if ((size_t)g.Wr * g.K * s > ((size_t)1 << 21) && s > 1) continue;
                 ZK_HIP(hipEventRecord(tb.e0, st));
-                msm_launch(tb.sort, ms[m].g2, ms[m].g2 ? tb.g2 : tb.g1, tb.part, st, ms[m].p29, s);
+                MsmScratch& scr = ms[m].g2 ? tb.g2 : tb.g1;
+                msm_launch_pre(tb.sort, ms[m].p29, ms[m].g2, scr, st, s);
+                msm_launch_main(tb.sort, ms[m].g2, scr, tb.part, st, ms[m].p29, ms[m].range);
                 ZK_HIP(hipEventRecord(tb.e1, st));
                 const float t = elapsed(tb);
                 if (it > 0) msm[m][s - 1] = (msm[m][s - 1] < 0 ? 0 : msm[m][s - 1]) + t / reps;
