@@ -280,10 +280,21 @@ __device__ inline uint32_t load_key(const uint32_t* __restrict__ off, const uint
     return (nkey - 1) - ((e0 - s0) >> qs);
 }
 
+// With classes (cls_off non-null) blockIdx.y = 0, 1, 2 orders the buckets by
+// the length of their MSM_RANGE_A, MSM_RANGE_B and whole parts, each with its
+// own nkey histogram (hist + y * nkey) and cursors; every order holds all nb
+// buckets, so one exclusive scan over the three histograms gives order y its
+// positions offset by y * nb.
+__device__ inline int order_range(const uint32_t* cls_off) {
+    return !cls_off ? MSM_RANGE_ALL : blockIdx.y == 0 ? MSM_RANGE_A : blockIdx.y == 1 ? MSM_RANGE_B : MSM_RANGE_ALL;
+}
+
 __global__ void __launch_bounds__(1024) k_load_hist(const uint32_t* __restrict__ off, const uint32_t* __restrict__ cls_off,
-                                                    int range, uint32_t nb, uint32_t lmax, int qs, uint32_t nkey,
+                                                    uint32_t nb, uint32_t lmax, int qs, uint32_t nkey,
                                                     uint32_t* __restrict__ hist) {
     extern __shared__ uint32_t lh[];
+    const int range = order_range(cls_off);
+    hist += (size_t)blockIdx.y * nkey;
     for (uint32_t k = threadIdx.x; k < nkey; k += blockDim.x) lh[k] = 0;
     __syncthreads();
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -295,10 +306,15 @@ __global__ void __launch_bounds__(1024) k_load_hist(const uint32_t* __restrict__
 
 // each block reserves one range per key, then ranks its buckets inside it in LDS
 __global__ void __launch_bounds__(1024) k_load_scatter(const uint32_t* __restrict__ off,
-                                                       const uint32_t* __restrict__ cls_off, int range, uint32_t nb,
+                                                       const uint32_t* __restrict__ cls_off, uint32_t nb,
                                                        uint32_t lmax, int qs, uint32_t nkey,
-                                                       uint32_t* __restrict__ cursor, uint32_t* __restrict__ order) {
+                                                       uint32_t* __restrict__ cursor, uint32_t* __restrict__ order_a,
+                                                       uint32_t* __restrict__ order_b, uint32_t* __restrict__ order) {
     extern __shared__ uint32_t lh[];
+    const int range = order_range(cls_off);
+    cursor += (size_t)blockIdx.y * nkey;
+    uint32_t* out = range == MSM_RANGE_A ? order_a : range == MSM_RANGE_B ? order_b : order;
+    const uint32_t base = blockIdx.y * nb;   // the scan's offset of order y
     for (uint32_t k = threadIdx.x; k < nkey; k += blockDim.x) lh[k] = 0;
     __syncthreads();
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -311,7 +327,7 @@ __global__ void __launch_bounds__(1024) k_load_scatter(const uint32_t* __restric
     for (uint32_t k = threadIdx.x; k < nkey; k += blockDim.x)
         if (lh[k]) lh[k] = atomicAdd(&cursor[k], lh[k]);
     __syncthreads();
-    if (b < nb) order[lh[key] + rank] = b;
+    if (b < nb) out[lh[key] + rank - base] = b;
 }
 
 // Chunk lists of the oversized buckets (load key 0, first in the accumulation
@@ -402,10 +418,12 @@ static void sort_buffers(S& s, const SortShape& z, F&& f) {
 
 // the buffers of a sort with scalar classes over nb buckets
 template <class S, class F>
-static void class_buffers(S& s, size_t nb, F&& f) {
+static void class_buffers(S& s, const SortShape& z, F&& f) {
+    const size_t nb = z.nb, nkey = z.nkey;
     f(s.cls_off, nb * 12);
     f(s.order_a, nb * 4);
     f(s.order_b, nb * 4);
+    f(s.okeys, (size_t)4 * (6 * (size_t)nkey + cdiv(3 * (size_t)nkey, SCAN_TILE) + 1));
 }
 
 static void sort_reserve(MsmSort& s, const SortShape& z) {
@@ -415,21 +433,21 @@ static void sort_reserve(MsmSort& s, const SortShape& z) {
 size_t MsmSort::reserve(size_t n_, int c, bool merged, bool with_classes) {
     const SortShape z = sort_shape(n_, c, merged);
     sort_reserve(*this, z);
-    if (with_classes) class_buffers(*this, z.nb, [](DevBuf& b, size_t n) { b.ensure(n); });
+    if (with_classes) class_buffers(*this, z, [](DevBuf& b, size_t n) { b.ensure(n); });
     return z.max_chunks;
 }
 
 size_t MsmSort::count_reserve(size_t n_, int c, GrowCount& gc, bool merged, bool with_classes) const {
     const SortShape z = sort_shape(n_, c, merged);
     sort_buffers(*this, z, [&](const DevBuf& b, size_t n) { gc.need(b, n); });
-    if (with_classes) class_buffers(*this, z.nb, [&](const DevBuf& b, size_t n) { gc.need(b, n); });
+    if (with_classes) class_buffers(*this, z, [&](const DevBuf& b, size_t n) { gc.need(b, n); });
     return z.max_chunks;
 }
 
 void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st, bool merged, const uint8_t* cls) {
     const SortShape z = sort_shape(n_, c, merged);
     sort_reserve(*this, z);
-    if (cls) class_buffers(*this, z.nb, [](DevBuf& b, size_t n) { b.ensure(n); });
+    if (cls) class_buffers(*this, z, [](DevBuf& b, size_t n) { b.ensure(n); });
     n = n_;
     classes = cls != nullptr && n_ != 0;
     g = z.g;
@@ -482,24 +500,22 @@ void MsmSort::run(const void* d_scalars, size_t n_, int c, hipStream_t st, bool 
     ZK_HIP(hipGetLastError());
     const uint32_t* off = offsets.as<uint32_t>();
     uint32_t* hist = lkeys.as<uint32_t>();
-    uint32_t* cur = hist + nkey;
     const unsigned nblk_o = (unsigned)cdiv(nb, 1024);
     const uint32_t* co = classes ? cls_off.as<uint32_t>() : nullptr;
-    auto make_order = [&](int range, uint32_t* ord) {
-        fill_u32(hist, 0u, nkey, st);
-        k_load_hist<<<nblk_o, 1024, nkey * 4, st>>>(off, co, range, (uint32_t)nb, lmax, qs, nkey, hist);
-        ZK_HIP(hipGetLastError());
-        exclusive_scan(hist, cur, cur + nkey, nkey, st);
-        k_load_scatter<<<nblk_o, 1024, nkey * 4, st>>>(off, co, range, (uint32_t)nb, lmax, qs, nkey, cur, ord);
-        ZK_HIP(hipGetLastError());
-    };
-    // the parts' orders first: the chunk lists below read the whole buckets'
-    // histogram
-    if (classes) {
-        make_order(MSM_RANGE_A, order_a.as<uint32_t>());
-        make_order(MSM_RANGE_B, order_b.as<uint32_t>());
-    }
-    make_order(MSM_RANGE_ALL, order.as<uint32_t>());
+    // accumulation orders: the whole buckets', and with classes also those of
+    // the A and B parts, in one histogram, scan and scatter
+    const uint32_t nord = classes ? 3 : 1;
+    uint32_t* h = classes ? okeys.as<uint32_t>() : hist;
+    uint32_t* cur = h + nord * nkey;
+    fill_u32(h, 0u, nord * nkey, st);
+    k_load_hist<<<dim3(nblk_o, nord), 1024, nkey * 4, st>>>(off, co, (uint32_t)nb, lmax, qs, nkey, h);
+    ZK_HIP(hipGetLastError());
+    exclusive_scan(h, cur, cur + nord * nkey, nord * nkey, st);
+    k_load_scatter<<<dim3(nblk_o, nord), 1024, nkey * 4, st>>>(off, co, (uint32_t)nb, lmax, qs, nkey, cur,
+                                                              order_a.as<uint32_t>(), order_b.as<uint32_t>(),
+                                                              order.as<uint32_t>());
+    ZK_HIP(hipGetLastError());
+    hist = h + (nord - 1) * nkey;   // the whole buckets' histogram: hist[0] oversized
     // chunk lists of the oversized buckets, shared by every MSM over this sort
     fill_u32(counters.as<uint32_t>(), 0u, 16, st);
     k_chunk_list<<<64, 256, 0, st>>>(order.as<uint32_t>(), off, hist, counters.as<uint32_t>(), chunks.as<ChunkRec>(),

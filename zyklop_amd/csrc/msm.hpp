@@ -89,9 +89,9 @@ struct MsmSort {
     uint32_t lmax = 0, nkey = 0;
     int qs = 0;
     DevBuf half, matrix, mscan, pairs, offsets, entries, order, lkeys;
-    // with classes: cls_off, and the accumulation orders of MSM_RANGE_A / _B
-    // (by the length of those parts)
-    DevBuf cls_off, order_a, order_b;
+    // with classes: cls_off, the accumulation orders of MSM_RANGE_A / _B (by
+    // the length of those parts), and the histograms of the three orders
+    DevBuf cls_off, order_a, order_b, okeys;
     bool classes = false;         // the last run() had scalar classes
     // chunk lists of the oversized buckets (above lmax entries), built by run()
     DevBuf chunks, large, counters;

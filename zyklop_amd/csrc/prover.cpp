@@ -378,7 +378,7 @@ size_t ProveSlot::device_bytes() const {
     for (const MsmSort* q : {&sortW, &sortH})
         b += q->half.bytes + q->matrix.bytes + q->mscan.bytes + q->pairs.bytes + q->offsets.bytes + q->entries.bytes +
              q->order.bytes + q->lkeys.bytes + q->chunks.bytes + q->large.bytes + q->counters.bytes + q->cls_off.bytes +
-             q->order_a.bytes + q->order_b.bytes;
+             q->order_a.bytes + q->order_b.bytes + q->okeys.bytes;
     for (const MsmScratch* s : {&scrA, &scrB1, &scrC, &scrB2, &scrH})
         b += s->buckets.bytes + s->r1.bytes + s->t1.bytes + s->r2.bytes + s->t2.bytes + s->partials.bytes;
     return b;
