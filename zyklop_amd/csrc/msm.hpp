@@ -97,8 +97,8 @@ struct MsmSort {
     DevBuf chunks, large, counters;
     size_t n_chunk_max = 0;
     // sizes every buffer a sort of n scalars at window c needs (no launch),
-    // with_classes: also cls_off; returns the chunk-list capacity for
-    // MsmScratch::ensure
+    // with_classes: also cls_off, order_a, order_b and okeys; returns the
+    // chunk-list capacity for MsmScratch::ensure
     size_t reserve(size_t n, int c, bool merged = false, bool with_classes = false);
     // the device bytes reserve(n, c, merged, with_classes) would add now (into
     // gc); same return value
