@@ -291,7 +291,8 @@ int zk_circuit_bases(const zk_circuit* c, int which, void** d_ptr, uint64_t* cou
  * them (window 0, or one with the tables' window count), else the 29-bit
  * copy at `window` (0 = tuned or default); for A', B1' and B2' of a circuit
  * with wire classes (enough infinite bases), the prover's class-ordered sort,
- * reading only the entries of finite bases. Same result as zk_msm_g1_device /
+ * reading only the entries of finite bases; at the bucket split a proof of c
+ * would use (autotuned, else the default). Same result as zk_msm_g1_device /
  * zk_msm_g2_device over zk_circuit_bases' points; timing in zk_last_timing
  * ([3]: the entries the accumulation read).
  * No counterpart in the reference (its MSMs run inside groth16Prove,

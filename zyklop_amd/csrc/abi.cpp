@@ -147,10 +147,11 @@ void put_result(const zkh::Xyzz<HT>& p, uint64_t* out, int* is_inf) {
 // MSM over device points; slots29, when given, are the points' 29-bit slots
 // already made (a circuit's resident arrays, W window tables when merged)
 // cls / range: scalar classes and the part of every bucket to read, as a
-// proof's witness MSMs do (MsmSort::run, msm_launch_main)
+// proof's witness MSMs do (MsmSort::run, msm_launch_main); split: threads per
+// bucket, 0 for msm_default_split
 void msm_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t n, int window, bool g2, uint64_t* out,
                 int* is_inf, const void* slots29 = nullptr, bool merged = false, const uint8_t* cls = nullptr,
-                int range = zk::MSM_RANGE_ALL) {
+                int range = zk::MSM_RANGE_ALL, int split = 0) {
     zk::Context& C = ctx->impl;
     hipStream_t st = C.stream;
     zk::MsmSort sort;
@@ -167,7 +168,7 @@ void msm_device(zk_ctx* ctx, const void* d_points, const void* d_scalars, size_t
     sort.run(d_scalars, n, window, st, merged, cls);
     ZK_HIP(hipEventRecord(ctx->ev1, st));
     const void* pts = slots29 ? slots29 : p29.p;
-    zk::msm_launch_pre(sort, pts, g2, scr, st, zk::msm_default_split(n, sort.g.c, g2, merged));
+    zk::msm_launch_pre(sort, pts, g2, scr, st, split > 0 ? split : zk::msm_default_split(n, sort.g.c, g2, merged));
     zk::msm_launch_main(sort, g2, scr, part, st, pts, range);
     ZK_HIP(hipEventRecord(ctx->ev2, st));
     ZK_HIP(hipStreamSynchronize(st));
@@ -782,8 +783,10 @@ int zk_circuit_msm_device(zk_ctx* ctx, const zk_circuit* c, int which, const voi
         // A', B1', B2': the prover's wire classes, which leave out infinite bases
         const uint8_t* cls = which <= 2 && k.wire_cls.bytes ? k.wire_cls.as<uint8_t>() : nullptr;
         const int range = !cls ? zk::MSM_RANGE_ALL : which == 0 ? zk::MSM_RANGE_A : zk::MSM_RANGE_B;
+        // and the bucket split a proof would use (tuned, else the default)
+        const int split = ctx->impl.split_for(kind + (merged ? 2 : 0), n, cw, which == 2);
         msm_device(ctx, std_pts[which]->p, d_scalars, n, cw, which == 2, out, is_inf, slots[which]->p, merged, cls,
-                   range);
+                   range, split);
     });
 }
 

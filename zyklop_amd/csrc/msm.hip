@@ -22,7 +22,9 @@ int msm_default_c(size_t n) {
 
 int msm_default_split(size_t n, int c, bool g2, bool merged) {
     const MsmGeom g = msm_geom(n, c, g2, merged);
-    // resident waves per SIMD: G1 accumulation 6 (80 VGPRs), G2 about 3 (LDS)
+    // two rounds of waves at 6 (G1) and 3 (G2) waves per SIMD, i.e. three
+    // and two rounds at the 4 and 2 the accumulation kernels hold (128 VGPRs;
+    // 228 VGPRs and LDS); the autotuner picks the splits of a circuit's sizes
     const size_t target = (size_t)1024 * 64 * 2 * (g2 ? 3 : 6);
     const size_t nb = (size_t)g.Wr * g.K;
     return (int)std::max<size_t>(1, std::min<size_t>(3, target / nb));
